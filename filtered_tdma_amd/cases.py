@@ -1,0 +1,79 @@
+"""Synthetic tridiagonal batches and the counter-based generator behind them.
+
+Every value is a pure function of (seed, stream, index), so a batch can be
+rebuilt bit-for-bit on any host without storing it. The generator is
+splitmix64 (Steele, Lea, Flood, "Fast splittable pseudorandom number
+generators", OOPSLA 2014) applied to a 64-bit counter; the top 53 bits give a
+uniform double in [0, 1).
+
+Layout of every array is the reference's: row-major [n_row x n_sys], row j of
+system i at j*n_sys + i (libs/pascal_tdma/include/pascal_tdma.hpp:13).
+Index order of the RHS counter is system-major: idx = i*n_row + j.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+_GAMMA = 0x9E3779B97F4A7C15
+_STREAM_MUL = 0xD1B54A32D192ED03
+
+
+def _splitmix64(z: np.ndarray) -> np.ndarray:
+    z = z + np.uint64(_GAMMA)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, idx: np.ndarray) -> np.ndarray:
+    """U[0,1) doubles for counters `idx` of (seed, stream)."""
+    key = np.uint64((seed * _GAMMA + stream * _STREAM_MUL) & _M64)
+    with np.errstate(over="ignore"):
+        bits = _splitmix64(idx.astype(np.uint64) + key)
+    return (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _rhs(seed: int, n_sys: int, n_row: int, lo: float = -1.0, hi: float = 1.0,
+         chunk_rows: int = 64) -> np.ndarray:
+    """D[j, i] = lo + (hi-lo) * u(seed, 1, i*n_row + j), built in row chunks."""
+    D = np.empty((n_row, n_sys), dtype=np.float64)
+    cols = np.arange(n_sys, dtype=np.uint64) * np.uint64(n_row)
+    for j0 in range(0, n_row, chunk_rows):
+        j1 = min(n_row, j0 + chunk_rows)
+        idx = cols[None, :] + np.arange(j0, j1, dtype=np.uint64)[:, None]
+        D[j0:j1] = lo + (hi - lo) * uniform(seed, 1, idx)
+    return D
+
+
+def toeplitz_batch(n_sys: int, n_row: int, r_lo: float, r_hi: float, seed: int = 0,
+                   with_rhs: bool = True):
+    """Constant-coefficient batch: a = c = -r_s, b = 1 + 2 r_s per system,
+    r_s ~ U[r_lo, r_hi], D ~ U[-1, 1] (rows 0 and n_row-1 included; A[0] and
+    C[n_row-1] are present but not read by the non-periodic solvers).
+
+    This is the implicit diffusion operator (1 - r d^2/dz^2) of one ADI sweep;
+    rho_s = r_s/(1+2 r_s) < 1/2. Returns (A, B, C, D, r) with A, B, C, D of
+    shape [n_row, n_sys] (D is None when with_rhs is False).
+    """
+    r = r_lo + (r_hi - r_lo) * uniform(seed, 0, np.arange(n_sys, dtype=np.uint64))
+    A = np.broadcast_to(-r, (n_row, n_sys)).copy()
+    C = A.copy()
+    B = np.broadcast_to(1.0 + 2.0 * r, (n_row, n_sys)).copy()
+    D = _rhs(seed, n_sys, n_row) if with_rhs else None
+    return A, B, C, D, r
+
+
+def general_batch(n_sys: int, n_row: int, seed: int = 0, margin: float = 0.1):
+    """Row-wise diagonally dominant batch with independent random a, b, c:
+    a, c ~ U[-1, 1], b = +-(|a| + |c|)(1 + margin + u), D ~ U[-1, 1].
+    Exercises every coefficient independently (no Toeplitz structure)."""
+    n = n_row * n_sys
+    idx = np.arange(n, dtype=np.uint64)
+    A = (2.0 * uniform(seed, 2, idx) - 1.0).reshape(n_row, n_sys)
+    C = (2.0 * uniform(seed, 3, idx) - 1.0).reshape(n_row, n_sys)
+    u = uniform(seed, 4, idx).reshape(n_row, n_sys)
+    sgn = np.where(uniform(seed, 5, idx).reshape(n_row, n_sys) < 0.5, -1.0, 1.0)
+    B = sgn * (np.abs(A) + np.abs(C)) * (1.0 + margin + u)
+    D = _rhs(seed, n_sys, n_row)
+    return A, B, C, D

@@ -1,0 +1,747 @@
+/*
+ * ftdma_oracle.c -- CPU restatement of the reference's batched tridiagonal
+ * solvers: plain and cyclic Thomas, PaScaL (modified Thomas + exact reduced
+ * solve) and Filtered TDMA v1/v2, each with the GPU and the CPU semantics of
+ * the reference.
+ *
+ * TEST INFRASTRUCTURE ONLY. tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library as the checker or as the
+ * timed CPU baseline; the product (filtered_tdma_amd/) never links it.
+ *
+ * Every function below names the reference lines it restates. Operation
+ * order follows the reference statement by statement (same products, same
+ * association, reciprocal-then-multiply where the reference does so and
+ * division where it divides), so that with -ffp-contract=off the result is
+ * bit-for-bit the reference's for the same inputs.
+ *
+ * Where the reference is undefined (float->int conversion of +-inf/NaN in
+ * the J estimators, J<=0 in the CPU v2 driver, which walks past the array),
+ * the restatement defines it and says how next to the code.
+ */
+#include "ftdma_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static __thread char g_err[256];
+
+static int fail(int code, const char* msg) {
+    snprintf(g_err, sizeof g_err, "%s", msg);
+    return code;
+}
+
+const char* ftdo_last_error(void) { return g_err; }
+
+/* Row j of array X with leading dimension ld. */
+#define ROW(X, j) ((X) + (size_t)(j) * ld)
+
+/* Column chunk used to split work over OpenMP threads. Systems are
+ * independent, so every step below runs per chunk with no interaction. */
+#define CHUNK 512
+
+static int nchunks(int n_sys) { return (n_sys + CHUNK - 1) / CHUNK; }
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+/* libs/pascal_tdma/cpu/para_range.cpp:4-9 */
+void ftdo_para_range(int start, int end, int nproc, int rank, int* ista, int* iend) {
+    int len = end - start + 1;
+    int base = len / nproc;
+    int rem = len % nproc;
+    *ista = start + rank * base + (rank < rem ? rank : rem);
+    *iend = *ista + base - 1 + (rank < rem ? 1 : 0);
+}
+
+static void part_rows(int n_row, int P, int p, int* s, int* m) {
+    int a, b;
+    ftdo_para_range(0, n_row - 1, P, p, &a, &b);
+    *s = a;
+    *m = b - a + 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plain Thomas on columns [c0,c1).                                          */
+/* CPU: libs/pascal_tdma/cpu/tdma_local.cpp:6-52 (row 0 scaled by 1/b).     */
+/* GPU: libs/pascal_tdma/gpu/tdma_local_cuda.cu:32-64 (row 0 divided by b). */
+/* ------------------------------------------------------------------------ */
+static void thomas_cols(const double* A, const double* B, double* C, double* D,
+                        size_t ld, int c0, int c1, int n_row, int sem) {
+    for (int i = c0; i < c1; ++i) {
+        if (sem == FTDO_SEM_GPU) {
+            D[i] = D[i] / B[i];
+            C[i] = C[i] / B[i];
+        } else {
+            double inv_b = 1.0 / B[i];
+            D[i] *= inv_b;
+            C[i] *= inv_b;
+        }
+    }
+    for (int j = 1; j < n_row; ++j) {
+        const double* Aj = ROW(A, j);
+        const double* Bj = ROW(B, j);
+        double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        const double* Cm = ROW(C, j - 1);
+        const double* Dm = ROW(D, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            double r = 1.0 / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = r * (Dj[i] - Aj[i] * Dm[i]);
+            Cj[i] = r * Cj[i];
+        }
+    }
+    for (int j = n_row - 2; j >= 0; --j) {
+        const double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        const double* Dp = ROW(D, j + 1);
+        for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] - Cj[i] * Dp[i];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sherman-Morrison cyclic Thomas on columns [c0,c1); E is [n_row x ld].     */
+/* tdma_local.cpp:108-208 and tdma_local_cuda.cu:67-128 (same arithmetic).   */
+/* ------------------------------------------------------------------------ */
+static void cyclic_cols(const double* A, const double* B, double* C, double* D,
+                        double* E, size_t ld, int c0, int c1, int n_row) {
+    for (int j = 0; j < n_row; ++j) {
+        double* Ej = ROW(E, j);
+        for (int i = c0; i < c1; ++i) Ej[i] = 0.0;
+    }
+    {
+        const double* A1 = ROW(A, 1);
+        const double* CN = ROW(C, n_row - 1);
+        double* E1 = ROW(E, 1);
+        double* EN = ROW(E, n_row - 1);
+        for (int i = c0; i < c1; ++i) {
+            E1[i] = -A1[i];
+            EN[i] = -CN[i];
+        }
+    }
+    {
+        const double* B1 = ROW(B, 1);
+        double* C1 = ROW(C, 1);
+        double* D1 = ROW(D, 1);
+        double* E1 = ROW(E, 1);
+        for (int i = c0; i < c1; ++i) {
+            double inv_b = 1.0 / B1[i];
+            D1[i] *= inv_b;
+            E1[i] *= inv_b;
+            C1[i] *= inv_b;
+        }
+    }
+    for (int j = 2; j < n_row; ++j) {
+        const double* Aj = ROW(A, j);
+        const double* Bj = ROW(B, j);
+        double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        double* Ej = ROW(E, j);
+        const double* Cm = ROW(C, j - 1);
+        const double* Dm = ROW(D, j - 1);
+        const double* Em = ROW(E, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            double r = 1.0 / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = r * (Dj[i] - Aj[i] * Dm[i]);
+            Ej[i] = r * (Ej[i] - Aj[i] * Em[i]);
+            Cj[i] = r * Cj[i];
+        }
+    }
+    for (int j = n_row - 2; j >= 1; --j) {
+        const double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        double* Ej = ROW(E, j);
+        const double* Dp = ROW(D, j + 1);
+        const double* Ep = ROW(E, j + 1);
+        for (int i = c0; i < c1; ++i) {
+            Dj[i] = Dj[i] - Cj[i] * Dp[i];
+            Ej[i] = Ej[i] - Cj[i] * Ep[i];
+        }
+    }
+    {
+        const double* D1 = ROW(D, 1);
+        const double* DN = ROW(D, n_row - 1);
+        const double* E1 = ROW(E, 1);
+        const double* EN = ROW(E, n_row - 1);
+        for (int i = c0; i < c1; ++i) {
+            D[i] = (D[i] - A[i] * DN[i] - C[i] * D1[i]) / (B[i] + A[i] * EN[i] + C[i] * E1[i]);
+        }
+    }
+    for (int j = 1; j < n_row; ++j) {
+        double* Dj = ROW(D, j);
+        const double* Ej = ROW(E, j);
+        for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] + D[i] * Ej[i];
+    }
+}
+
+int ftdo_thomas(double* A, const double* B, double* C, double* D,
+                int n_sys, int n_row, int sem, int nthreads) {
+    if (n_sys < 0 || n_row < 1) return fail(-1, "thomas: need n_sys >= 0 and n_row >= 1");
+    const size_t ld = (size_t)n_sys;
+    set_threads(nthreads);
+    const int nc = nchunks(n_sys);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        thomas_cols(A, B, C, D, ld, c0, c1, n_row, sem);
+    }
+    return 0;
+}
+
+int ftdo_thomas_cyclic(double* A, const double* B, double* C, double* D,
+                       int n_sys, int n_row, int sem, int nthreads) {
+    (void)sem; /* CPU and GPU cyclic Thomas share one operation order */
+    if (n_sys < 0 || n_row < 3) return fail(-1, "thomas_cyclic: need n_row >= 3");
+    const size_t ld = (size_t)n_sys;
+    double* E = (double*)malloc(sizeof(double) * (size_t)n_row * ld + 8);
+    if (!E) return fail(-3, "thomas_cyclic: out of memory");
+    set_threads(nthreads);
+    const int nc = nchunks(n_sys);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        cyclic_cols(A, B, C, D, E, ld, c0, c1, n_row);
+    }
+    free(E);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PaScaL modified Thomas on one partition (pointers at its first row).      */
+/* CPU: pascal_tdma.cpp:131-183. GPU: tdma_local_cuda.cu:134-203 (rows 0,1  */
+/* divided by b instead of scaled by 1/b). Writes the two reduced rows      */
+/* (B == 1 implied) into R[0..5][ld]: a0,c0,d0 (row 0) and aN,cN,dN.         */
+/* ------------------------------------------------------------------------ */
+static void modthomas_cols(double* A, const double* B, double* C, double* D,
+                           size_t ld, int c0, int c1, int m, int sem, double* R) {
+    double* A0 = A; double* C0 = C; double* D0 = D;
+    double* A1 = ROW(A, 1); double* C1 = ROW(C, 1); double* D1 = ROW(D, 1);
+    const double* B0 = B; const double* B1 = ROW(B, 1);
+    for (int i = c0; i < c1; ++i) {
+        if (sem == FTDO_SEM_GPU) {
+            A0[i] = A0[i] / B0[i]; C0[i] = C0[i] / B0[i]; D0[i] = D0[i] / B0[i];
+            A1[i] = A1[i] / B1[i]; C1[i] = C1[i] / B1[i]; D1[i] = D1[i] / B1[i];
+        } else {
+            double r0 = 1.0 / B0[i];
+            A0[i] *= r0; C0[i] *= r0; D0[i] *= r0;
+            double r1 = 1.0 / B1[i];
+            A1[i] *= r1; C1[i] *= r1; D1[i] *= r1;
+        }
+    }
+    for (int j = 2; j < m; ++j) {
+        double* Aj = ROW(A, j); const double* Bj = ROW(B, j);
+        double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+        const double* Am = ROW(A, j - 1);
+        const double* Cm = ROW(C, j - 1);
+        const double* Dm = ROW(D, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            double inv = 1.0 / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = inv * (Dj[i] - Aj[i] * Dm[i]);
+            Cj[i] = inv * Cj[i];
+            Aj[i] = -inv * Aj[i] * Am[i];
+        }
+    }
+    for (int j = m - 3; j >= 1; --j) {
+        double* Aj = ROW(A, j); double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+        const double* Ap = ROW(A, j + 1);
+        const double* Cp = ROW(C, j + 1);
+        const double* Dp = ROW(D, j + 1);
+        for (int i = c0; i < c1; ++i) {
+            Dj[i] = Dj[i] - Cj[i] * Dp[i];
+            Aj[i] = Aj[i] - Cj[i] * Ap[i];
+            Cj[i] = -Cj[i] * Cp[i];
+        }
+    }
+    const double* AN = ROW(A, m - 1);
+    const double* CN = ROW(C, m - 1);
+    const double* DN = ROW(D, m - 1);
+    for (int i = c0; i < c1; ++i) {
+        double r = 1.0 / (1.0 - A1[i] * C0[i]);
+        D0[i] = r * (D0[i] - C0[i] * D1[i]);
+        A0[i] = r * A0[i];
+        C0[i] = -r * C0[i] * C1[i];
+        R[0 * ld + i] = A0[i]; R[1 * ld + i] = C0[i]; R[2 * ld + i] = D0[i];
+        R[3 * ld + i] = AN[i]; R[4 * ld + i] = CN[i]; R[5 * ld + i] = DN[i];
+    }
+}
+
+/* Reduced system of 2P rows ([row 0 of p, row m-1 of p] for p = 0..P-1,
+ * pascal_tdma_kernels.cuh:50-62), B == 1, solved with tdma_many (or
+ * tdma_cyclic_many) as pascal_tdma.cpp:201,322 and pascal_tdma_cuda.cu:277,
+ * 389 do. Rows r: Ar, Cr, Dr each [2P x ld]; scratch E for the cyclic case. */
+static void reduced_cols(double* Ar, double* Br, double* Cr, double* Dr, double* Er,
+                         size_t ld, int c0, int c1, int n_rd, int cyclic, int sem) {
+    if (cyclic)
+        cyclic_cols(Ar, Br, Cr, Dr, Er, ld, c0, c1, n_rd);
+    else
+        thomas_cols(Ar, Br, Cr, Dr, ld, c0, c1, n_rd, sem);
+}
+
+/* Interior update x_j = d_j - (a_j x0 + c_j xN): pascal_tdma.cpp:212-226,
+ * tdma_local_cuda.cu:205-224. */
+static void update_cols(const double* A, const double* C, double* D, size_t ld,
+                        int c0, int c1, int m, const double* x0, const double* xN) {
+    double* DN = ROW(D, m - 1);
+    for (int i = c0; i < c1; ++i) {
+        D[i] = x0[i];
+        DN[i] = xN[i];
+    }
+    for (int j = 1; j < m - 1; ++j) {
+        const double* Aj = ROW(A, j);
+        const double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] - (Aj[i] * x0[i] + Cj[i] * xN[i]);
+    }
+}
+
+static int check_parts(int n_row, int P) {
+    if (P < 1) return fail(-1, "partitions must be >= 1");
+    if (P > 1) {
+        int s, m;
+        part_rows(n_row, P, P - 1, &s, &m); /* the last partitions are the shortest */
+        if (m < 3) return fail(-2, "every partition needs at least 3 rows");
+    }
+    return 0;
+}
+
+int ftdo_pascal(double* A, const double* B, double* C, double* D,
+                int n_sys, int n_row, int P, int cyclic, int sem, int nthreads) {
+    int rc = check_parts(n_row, P);
+    if (rc) return rc;
+    if (P == 1) {
+        /* pascal_tdma.cpp:116-119, pascal_tdma_cuda.cu:266-270 */
+        return cyclic ? ftdo_thomas_cyclic(A, B, C, D, n_sys, n_row, sem, nthreads)
+                      : ftdo_thomas(A, B, C, D, n_sys, n_row, sem, nthreads);
+    }
+    const size_t ld = (size_t)n_sys;
+    const int n_rd = 2 * P;
+    double* buf = (double*)malloc(sizeof(double) * ld * (size_t)(4 * n_rd + 6 * P) + 8);
+    if (!buf) return fail(-3, "pascal: out of memory");
+    double* Ar = buf;
+    double* Br = Ar + ld * n_rd;
+    double* Cr = Br + ld * n_rd;
+    double* Dr = Cr + ld * n_rd;
+    double* Rp = Dr + ld * n_rd; /* 6 rows per partition */
+    set_threads(nthreads);
+    const int nc = nchunks(n_sys);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        for (int p = 0; p < P; ++p) {
+            int s, m;
+            part_rows(n_row, P, p, &s, &m);
+            double* R = Rp + (size_t)6 * p * ld;
+            modthomas_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, sem, R);
+            for (int i = c0; i < c1; ++i) {
+                Ar[(size_t)(2 * p) * ld + i] = R[0 * ld + i];
+                Cr[(size_t)(2 * p) * ld + i] = R[1 * ld + i];
+                Dr[(size_t)(2 * p) * ld + i] = R[2 * ld + i];
+                Ar[(size_t)(2 * p + 1) * ld + i] = R[3 * ld + i];
+                Cr[(size_t)(2 * p + 1) * ld + i] = R[4 * ld + i];
+                Dr[(size_t)(2 * p + 1) * ld + i] = R[5 * ld + i];
+                Br[(size_t)(2 * p) * ld + i] = 1.0;
+                Br[(size_t)(2 * p + 1) * ld + i] = 1.0;
+            }
+        }
+        /* the cyclic reduced solve needs an E scratch of 2P rows: reuse Rp
+         * (6P >= 2P rows), which is no longer read after the copy above */
+        reduced_cols(Ar, Br, Cr, Dr, Rp, ld, c0, c1, n_rd, cyclic, sem);
+        for (int p = 0; p < P; ++p) {
+            int s, m;
+            part_rows(n_row, P, p, &s, &m);
+            update_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m,
+                        Dr + (size_t)(2 * p) * ld, Dr + (size_t)(2 * p + 1) * ld);
+        }
+    }
+    free(buf);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spectral bounds from column 0 (tdma_backend.cpp:33-44,                   */
+/* filtered_tdma_kernels.cuh:46-67): |A/B|, |C/B| per row, 0 where B == 0,  */
+/* trailing slot n_row = 0.                                                  */
+/* ------------------------------------------------------------------------ */
+void ftdo_set_rho(const double* A, const double* B, const double* C,
+                  int n_sys, int n_row, double* A_rho, double* C_rho) {
+    const size_t ld = (size_t)n_sys;
+    for (int k = 0; k < n_row; ++k) {
+        double bk = B[(size_t)k * ld];
+        A_rho[k] = (bk != 0.0) ? fabs(A[(size_t)k * ld] / bk) : 0.0;
+        C_rho[k] = (bk != 0.0) ? fabs(C[(size_t)k * ld] / bk) : 0.0;
+    }
+    A_rho[n_row] = 0.0;
+    C_rho[n_row] = 0.0;
+}
+
+/* float -> int truncation toward zero. The reference writes static_cast<int>
+ * / (int) of values that can be +-inf or NaN (eps/B with B == 0), which C++
+ * leaves undefined; here it saturates (NaN -> 0), which is what the CUDA and
+ * AMD conversion instructions do. Returned as int64 so that "+ 1" cannot
+ * overflow. */
+static int64_t trunc_sat(double x) {
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return INT_MAX;
+    if (x <= -2147483648.0) return INT_MIN;
+    return (int64_t)(int)x;
+}
+
+/* ftdo_cal_J: filtered_tdma.cpp:63-115 (CPU) and filtered_tdma_kernels.cuh:
+ * 245-259 (GPU). rho==0 or rho>=0.5 -> n_row-1 (full propagation). */
+int ftdo_cal_J(double rho, double bound, double eps, int n_row, int which) {
+    if (rho == 0.0 || rho >= 0.5) return n_row - 1;
+    double lambda_p = (which == FTDO_J_GPU_RHS) ? (1.0 + sqrt(1.0 - 4.0 * rho * rho)) * 0.5
+                                                : (1.0 + sqrt(1.0 - 4.0 * rho * rho)) / 2.0;
+    double q = rho / lambda_p;
+    double Bv;
+    if (which == FTDO_J_CPU_V1)
+        Bv = bound * q / (1.0 - q);
+    else
+        Bv = q * (2.0 + q) / ((1.0 - q) * (1.0 - 2.0 * rho)) * bound;
+    int64_t J = trunc_sat(log(eps / Bv) / log(q)) + 1;
+    if (J > n_row - 1) J = n_row - 1;
+    if (which == FTDO_J_GPU_RHS && J < 0) J = 0;
+    return (int)J;
+}
+
+static double part_rho(const double* A_rho, const double* C_rho, int s, int m) {
+    /* k in [1, m-1] of the partition's local rows (filtered_tdma.cpp:68,
+     * filtered_tdma_kernels.cuh:232) */
+    double rho = 0.0;
+    if (!A_rho || !C_rho) return 0.0;
+    for (int k = 1; k < m; ++k) {
+        double a = fabs(A_rho[s + k]), c = fabs(C_rho[s + k]);
+        double v = a > c ? a : c;
+        if (v > rho) rho = v;
+    }
+    return rho;
+}
+
+/* ---------------------------- Filtered steps ---------------------------- */
+
+/* Forward sweep. v1: filtered_tdma_kernels.hpp:35-65 / .cuh:69-94 (full).
+ * v2 GPU: .cuh:120-155 literally, including J == 0, where the second loop
+ * starts at row 1 and eliminates the already-normalised row 1 once more.
+ * v2 CPU: filtered_tdma_kernels.hpp:68-110 (the driver keeps J >= 1 here). */
+static void ffwd_cols(double* A, const double* B, double* C, double* D, size_t ld,
+                      int c0, int c1, int m, int J_full /* last row with a-update */,
+                      int gpu_v2) {
+    double* A1 = ROW(A, 1); double* C1 = ROW(C, 1); double* D1 = ROW(D, 1);
+    const double* B1 = ROW(B, 1);
+    for (int i = c0; i < c1; ++i) {
+        double r0 = 1.0 / B[i];
+        A[i] *= r0; C[i] *= r0; D[i] *= r0;
+        double r1 = 1.0 / B1[i];
+        A1[i] *= r1; C1[i] *= r1; D1[i] *= r1;
+    }
+    int j = 2;
+    for (; j <= J_full && j < m; ++j) {
+        double* Aj = ROW(A, j); const double* Bj = ROW(B, j);
+        double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+        const double* Am = ROW(A, j - 1);
+        const double* Cm = ROW(C, j - 1);
+        const double* Dm = ROW(D, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            double inv = 1.0 / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = inv * (Dj[i] - Aj[i] * Dm[i]);
+            Cj[i] = inv * Cj[i];
+            Aj[i] = -inv * Aj[i] * Am[i];
+        }
+    }
+    if (gpu_v2 && J_full == 0) {
+        /* k_fwd_pass_v2 with J == 0: its skip-A loop runs j = 1..m-1 with the
+         * sliding window already holding row 1 */
+        const double* Bj = B1;
+        for (int i = c0; i < c1; ++i) {
+            double c_prev = C1[i], d_prev = D1[i];
+            double Aj = A1[i];
+            double inv = 1.0 / (Bj[i] - Aj * c_prev);
+            D1[i] = inv * (D1[i] - Aj * d_prev);
+            C1[i] = inv * C1[i];
+        }
+        j = 2;
+    }
+    for (; j < m; ++j) {
+        const double* Aj = ROW(A, j); const double* Bj = ROW(B, j);
+        double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+        const double* Cm = ROW(C, j - 1);
+        const double* Dm = ROW(D, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            double inv = 1.0 / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = inv * (Dj[i] - Aj[i] * Dm[i]);
+            Cj[i] = inv * Cj[i];
+        }
+    }
+}
+
+/* Backward sweep + pack_reduced (row 0 keeps d0, a0; c0 coupling dropped).
+ * v1: filtered_tdma_kernels.hpp:117-135,178-189 / .cuh:96-118.
+ * v2 GPU: .cuh:157-194: a-update only for j < J seeded from A[J] (J >= 2),
+ *   c-update only for j > lo = (m-2)-J.
+ * v2 CPU: filtered_tdma_kernels.hpp:138-172 (J <= m-2 by the driver clamp). */
+static void fbwd_cols(double* A, double* C, double* D, size_t ld, int c0, int c1, int m,
+                      int version, int J, int sem) {
+    if (version == 1) {
+        for (int j = m - 3; j >= 1; --j) {
+            double* Aj = ROW(A, j); double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+            const double* Ap = ROW(A, j + 1);
+            const double* Cp = ROW(C, j + 1);
+            const double* Dp = ROW(D, j + 1);
+            for (int i = c0; i < c1; ++i) {
+                double cj = Cj[i];
+                Dj[i] = Dj[i] - cj * Dp[i];
+                Aj[i] = Aj[i] - cj * Ap[i];
+                Cj[i] = -cj * Cp[i];
+            }
+        }
+    } else if (sem == FTDO_SEM_GPU) {
+        const int lo = (m - 2) - J;
+        for (int i = c0; i < c1; ++i) {
+            double d_next = ROW(D, m - 2)[i];
+            double c_next = ROW(C, m - 2)[i];
+            double a_next = (J >= 2) ? ROW(A, J)[i] : 0.0;
+            for (int j = m - 3; j >= 1; --j) {
+                double cj = ROW(C, j)[i];
+                double d_new = ROW(D, j)[i] - cj * d_next;
+                ROW(D, j)[i] = d_new;
+                d_next = d_new;
+                if (J >= 2 && j < J) {
+                    double a_new = ROW(A, j)[i] - cj * a_next;
+                    ROW(A, j)[i] = a_new;
+                    a_next = a_new;
+                }
+                if (j > lo) {
+                    double c_new = -cj * c_next;
+                    ROW(C, j)[i] = c_new;
+                    c_next = c_new;
+                }
+            }
+        }
+    } else {
+        const int lo = (m - 2) - J;
+        for (int j = m - 3; j >= 1; --j) {
+            const double* Cj = ROW(C, j); double* Dj = ROW(D, j);
+            const double* Dp = ROW(D, j + 1);
+            for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] - Cj[i] * Dp[i];
+        }
+        if (J >= 2) {
+            for (int j = J - 1; j >= 1; --j) {
+                double* Aj = ROW(A, j); const double* Cj = ROW(C, j);
+                const double* Ap = ROW(A, j + 1);
+                for (int i = c0; i < c1; ++i) Aj[i] = Aj[i] - Cj[i] * Ap[i];
+            }
+        }
+        for (int j = m - 3; j >= lo + 1; --j) {
+            double* Cj = ROW(C, j); const double* Cp = ROW(C, j + 1);
+            for (int i = c0; i < c1; ++i) Cj[i] = Cj[i] * -Cp[i];
+        }
+    }
+    /* pack_reduced: filtered_tdma_kernels.hpp:178-189, .cuh:114-117,187-193 */
+    const double* A1 = ROW(A, 1);
+    const double* D1 = ROW(D, 1);
+    for (int i = c0; i < c1; ++i) {
+        double r = 1.0 / (1.0 - A1[i] * C[i]);
+        D[i] = r * (D[i] - C[i] * D1[i]);
+        A[i] = r * A[i];
+    }
+}
+
+/* Truncated correction: filtered_tdma_kernels.hpp:227-250 / .cuh:196-218. */
+static void ffinal_cols(const double* A, const double* C, double* D, size_t ld,
+                        int c0, int c1, int m, int left_end, int right_beg) {
+    const double* DN = ROW(D, m - 1);
+    for (int j = 1; j <= left_end; ++j) {
+        const double* Aj = ROW(A, j);
+        double* Dj = ROW(D, j);
+        for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] - Aj[i] * D[i];
+    }
+    for (int j = right_beg; j < m - 1; ++j) {
+        const double* Cj = ROW(C, j);
+        double* Dj = ROW(D, j);
+        for (int i = c0; i < c1; ++i) Dj[i] = Dj[i] - Cj[i] * DN[i];
+    }
+}
+
+int ftdo_filtered(double* A, const double* B, double* C, double* D,
+                  int n_sys, int n_row, int P, int version, int cyclic, int sem,
+                  const double* A_rho, const double* C_rho,
+                  double eps_value, int eps_direct,
+                  const int* J_force, int* J_out, int nthreads) {
+    int rc = check_parts(n_row, P);
+    if (rc) return rc;
+    if (version != 1 && version != 2) return fail(-1, "filtered: version must be 1 or 2");
+    if (P == 1) {
+        /* filtered_tdma.cpp:172,194,219,242; filtered_tdma_cuda.cu:108,135,170,199 */
+        if (J_out) J_out[0] = -1;
+        return cyclic ? ftdo_thomas_cyclic(A, B, C, D, n_sys, n_row, sem, nthreads)
+                      : ftdo_thomas(A, B, C, D, n_sys, n_row, sem, nthreads);
+    }
+    const size_t ld = (size_t)n_sys;
+    /* exchange buffers: per partition [C_{m-1} | D_{m-1}] (to the right) and
+     * [A_0 | D_0] (to the left): filtered_tdma.cpp:151-163, .cuh:263-282 */
+    double* xbuf = (double*)malloc(sizeof(double) * ld * (size_t)(4 * P) + 8);
+    int* Jp = (int*)malloc(sizeof(int) * (size_t)P);
+    double* maxb = (double*)malloc(sizeof(double) * (size_t)P);
+    if (!xbuf || !Jp || !maxb) {
+        free(xbuf); free(Jp); free(maxb);
+        return fail(-3, "filtered: out of memory");
+    }
+    set_threads(nthreads);
+    const int nc = nchunks(n_sys);
+    int sv[4096], mv[4096];
+    if (P > 4096) {
+        free(xbuf); free(Jp); free(maxb);
+        return fail(-1, "filtered: at most 4096 partitions");
+    }
+    for (int p = 0; p < P; ++p) part_rows(n_row, P, p, &sv[p], &mv[p]);
+    double epsv[4096];
+    for (int p = 0; p < P; ++p) {
+        /* eps_ = eps_constant / (n_row_local * nprocs)^2 (filtered_tdma.cpp:50-51) */
+        double Nl = (double)(mv[p] * P);
+        epsv[p] = eps_direct ? eps_value : eps_value / (Nl * Nl);
+    }
+
+    /* v2: J before the sweeps */
+    if (version == 2) {
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            double mb = 0.0;
+            if (sem == FTDO_SEM_GPU) {
+                /* max over systems of |D row 0| and |D row m-1| (.cuh:235-244) */
+                const double* D0 = ROW(D, s);
+                const double* DN = ROW(D, s + m - 1);
+                for (int i = 0; i < n_sys; ++i) {
+                    double a = fabs(D0[i]), b = fabs(DN[i]);
+                    if (a > mb) mb = a;
+                    if (b > mb) mb = b;
+                }
+            } else {
+                /* max over every local row and system (filtered_tdma.cpp:75-80) */
+                for (int j = 0; j < m; ++j) {
+                    const double* Dj = ROW(D, s + j);
+                    for (int i = 0; i < n_sys; ++i) {
+                        double a = fabs(Dj[i]);
+                        if (a > mb) mb = a;
+                    }
+                }
+            }
+            int J;
+            if (J_force) {
+                J = J_force[p];
+            } else if (sem == FTDO_SEM_GPU) {
+                J = ftdo_cal_J(part_rho(A_rho, C_rho, s, m), mb, epsv[p], m, FTDO_J_GPU_RHS);
+            } else {
+                J = ftdo_cal_J(part_rho(A_rho, C_rho, s, m), mb, epsv[p], m, FTDO_J_CPU_RHS);
+                if (J > m - 2) J = m - 2; /* filtered_tdma.cpp:224 */
+                if (J < 1) J = 1;         /* the reference is undefined for J < 1 here */
+            }
+            Jp[p] = J;
+        }
+    }
+
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            int Jfull = (version == 1) ? m : Jp[p];
+            ffwd_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, Jfull,
+                      version == 2 && sem == FTDO_SEM_GPU);
+            fbwd_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, version,
+                      version == 2 ? Jp[p] : 0, sem);
+            double* sr = xbuf + (size_t)(4 * p) * ld;
+            const double* CN = ROW(C, s + m - 1);
+            const double* DN = ROW(D, s + m - 1);
+            const double* A0 = ROW(A, s);
+            const double* D0 = ROW(D, s);
+            for (int i = c0; i < c1; ++i) {
+                sr[0 * ld + i] = CN[i]; /* send right: C */
+                sr[1 * ld + i] = DN[i]; /* send right: D */
+                sr[2 * ld + i] = A0[i]; /* send left : A */
+                sr[3 * ld + i] = D0[i]; /* send left : D */
+            }
+        }
+        /* interface 2x2 blocks: filtered_tdma_kernels.hpp:197-218, .cuh:284-303 */
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            int has_left = cyclic || p > 0;
+            int has_right = cyclic || p < P - 1;
+            int pl = (p + P - 1) % P, pr = (p + 1) % P;
+            const double* C_left = xbuf + (size_t)(4 * pl + 0) * ld;
+            const double* D_left = xbuf + (size_t)(4 * pl + 1) * ld;
+            const double* A_right = xbuf + (size_t)(4 * pr + 2) * ld;
+            const double* D0_right = xbuf + (size_t)(4 * pr + 3) * ld;
+            double* A0 = ROW(A, s);
+            double* D0 = ROW(D, s);
+            const double* CN = ROW(C, s + m - 1);
+            double* DN = ROW(D, s + m - 1);
+            if (has_left)
+                for (int i = c0; i < c1; ++i)
+                    D0[i] = (D0[i] - A0[i] * D_left[i]) / (1.0 - C_left[i] * A0[i]);
+            if (has_right)
+                for (int i = c0; i < c1; ++i)
+                    DN[i] = (DN[i] - CN[i] * D0_right[i]) / (1.0 - A_right[i] * CN[i]);
+        }
+    }
+
+    /* v1: J after the interface solve */
+    if (version == 1) {
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            const double* D0 = ROW(D, s);
+            const double* DN = ROW(D, s + m - 1);
+            double m0 = 0.0, mN = 0.0;
+            for (int i = 0; i < n_sys; ++i) {
+                double a = fabs(D0[i]), b = fabs(DN[i]);
+                if (a > m0) m0 = a;
+                if (b > mN) mN = b;
+            }
+            int J;
+            if (J_force)
+                J = J_force[p];
+            else if (sem == FTDO_SEM_GPU)
+                J = ftdo_cal_J(part_rho(A_rho, C_rho, s, m), m0 > mN ? m0 : mN, epsv[p], m,
+                               FTDO_J_GPU_RHS);
+            else
+                J = ftdo_cal_J(part_rho(A_rho, C_rho, s, m), m0 + mN, epsv[p], m, FTDO_J_CPU_V1);
+            Jp[p] = J;
+        }
+    }
+
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p], J = Jp[p];
+            int left_end, right_beg;
+            if (version == 2 && sem == FTDO_SEM_CPU) {
+                /* filtered_tdma.cpp:236-237 */
+                left_end = J;
+                right_beg = (m - 2) - J + 1 > 1 ? (m - 2) - J + 1 : 1;
+            } else {
+                /* filtered_tdma.cpp:185-186 and k_final_pass .cuh:206-208 */
+                left_end = J < m - 2 ? J : m - 2;
+                right_beg = (m - 1) - J > 1 ? (m - 1) - J : 1;
+            }
+            ffinal_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, left_end, right_beg);
+        }
+    }
+    if (J_out)
+        for (int p = 0; p < P; ++p) J_out[p] = Jp[p];
+    free(xbuf);
+    free(Jp);
+    free(maxb);
+    return 0;
+}

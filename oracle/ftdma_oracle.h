@@ -1,0 +1,77 @@
+/*
+ * ftdma_oracle.h -- CPU restatement of the reference's batched tridiagonal
+ * solvers. TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it as the checker (or as the timed CPU
+ * baseline). No product code links or calls it.
+ *
+ * Layout (libs/pascal_tdma/include/pascal_tdma.hpp:13): row-major
+ * [n_row x n_sys], row j of system i at j*n_sys + i. Every routine works in
+ * place like the reference: D <- solution, A and C clobbered, B read-only.
+ *
+ * Partition p of P covers rows para_range(0, n_row-1, P, p)
+ * (libs/pascal_tdma/cpu/para_range.cpp:4-9): the MPI z-slab split of the
+ * reference, emulated in one address space.
+ */
+#ifndef FTDMA_ORACLE_H
+#define FTDMA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Normalisation order / J policy of the reference path being restated. */
+enum {
+    FTDO_SEM_GPU = 0, /* the CUDA path behind TdmaBackendGPU (drop-in target) */
+    FTDO_SEM_CPU = 1  /* the host path behind TdmaBackend (libs/.../cpu)      */
+};
+
+/* J estimator selector for ftdo_cal_J. */
+enum {
+    FTDO_J_GPU_RHS = 0, /* k_cal_J_rhs_bound, filtered_tdma_kernels.cuh:220-261 */
+    FTDO_J_CPU_V1  = 1, /* FilteredTDMA::cal_J_v1, filtered_tdma.cpp:89-115      */
+    FTDO_J_CPU_RHS = 2  /* FilteredTDMA::cal_J_rhs_bound, filtered_tdma.cpp:63-87 */
+};
+
+void ftdo_para_range(int start, int end, int nproc, int rank, int* ista, int* iend);
+
+/* Plain Thomas, all systems (tdma_local.cpp:6-52 / tdma_local_cuda.cu:32-64). */
+int ftdo_thomas(double* A, const double* B, double* C, double* D,
+                int n_sys, int n_row, int sem, int nthreads);
+
+/* Sherman-Morrison periodic Thomas (tdma_local.cpp:108-208). */
+int ftdo_thomas_cyclic(double* A, const double* B, double* C, double* D,
+                       int n_sys, int n_row, int sem, int nthreads);
+
+/* PaScaL with P partitions (pascal_tdma.cpp:113-348, pascal_tdma_cuda.cu:264-398). */
+int ftdo_pascal(double* A, const double* B, double* C, double* D,
+                int n_sys, int n_row, int P, int cyclic, int sem, int nthreads);
+
+/* Column-0 spectral bounds, n_row+1 entries each (tdma_backend.cpp:33-44). */
+void ftdo_set_rho(const double* A, const double* B, const double* C,
+                  int n_sys, int n_row, double* A_rho, double* C_rho);
+
+/* J estimators. `bound` is max_b for the RHS bounds and max|x0|+max|xN|
+ * for FTDO_J_CPU_V1. n_row is the partition's local row count. */
+int ftdo_cal_J(double rho, double bound, double eps, int n_row, int which);
+
+/* Filtered TDMA v1/v2 with P partitions.
+ *   A_rho/C_rho : n_row+1 global column-0 bounds (ftdo_set_rho), or NULL (= 0,
+ *                 the reference's state before any set_rho call).
+ *   eps_value   : eps_constant, turned into eps_ = eps_constant/(m_p*P)^2 per
+ *                 partition (filtered_tdma.hpp:60-63); if eps_direct != 0 it
+ *                 is used as eps_ unchanged.
+ *   J_force     : if non-NULL, P values of J used instead of the estimator.
+ *   J_out       : if non-NULL, receives the P values of J used.           */
+int ftdo_filtered(double* A, const double* B, double* C, double* D,
+                  int n_sys, int n_row, int P, int version, int cyclic, int sem,
+                  const double* A_rho, const double* C_rho,
+                  double eps_value, int eps_direct,
+                  const int* J_force, int* J_out, int nthreads);
+
+/* Error text of the last failing call in this thread. */
+const char* ftdo_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

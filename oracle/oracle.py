@@ -1,0 +1,179 @@
+"""ctypes front end of the CPU oracle (oracle/ftdma_oracle.c) and of the
+reference CPU build in oracle/_ref (when present).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. Product code never imports it.
+
+All arrays are numpy float64 [n_row, n_sys] (C order), solved in place like
+the reference: D <- x, A and C clobbered, B untouched. Every function returns
+the solved D and leaves the caller's arrays alone unless `inplace=True`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import shutil
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libftdma_oracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+SEM_GPU, SEM_CPU = 0, 1
+J_GPU_RHS, J_CPU_V1, J_CPU_RHS = 0, 1, 2
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle library with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "libftdma_oracle.so"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        c_int, c_double = ctypes.c_int, ctypes.c_double
+        L.ftdo_thomas.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int]
+        L.ftdo_thomas_cyclic.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int]
+        L.ftdo_pascal.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int, c_int]
+        L.ftdo_set_rho.argtypes = [dp, dp, dp, c_int, c_int, dp, dp]
+        L.ftdo_set_rho.restype = None
+        L.ftdo_cal_J.argtypes = [c_double, c_double, c_double, c_int, c_int]
+        L.ftdo_filtered.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int, c_int,
+                                    dp, dp, c_double, c_int, ip, ip, c_int]
+        L.ftdo_para_range.argtypes = [c_int, c_int, c_int, c_int, ip, ip]
+        L.ftdo_para_range.restype = None
+        L.ftdo_last_error.restype = ctypes.c_char_p
+        for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_cal_J,
+                  L.ftdo_filtered):
+            f.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _prep(A, B, C, D, inplace):
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (A, B, C, D)]
+    if not inplace:
+        arrs = [x.copy() for x in arrs]
+    n_row, n_sys = arrs[3].shape
+    for x in arrs:
+        if x.shape != (n_row, n_sys):
+            raise ValueError("A, B, C, D must share one [n_row, n_sys] shape")
+    return arrs, n_sys, n_row
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().ftdo_last_error().decode())
+
+
+def thomas(A, B, C, D, sem=SEM_GPU, cyclic=False, nthreads=0, inplace=False):
+    (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
+    f = lib().ftdo_thomas_cyclic if cyclic else lib().ftdo_thomas
+    _check(f(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, sem, nthreads))
+    return D
+
+
+def pascal(A, B, C, D, P, sem=SEM_GPU, cyclic=False, nthreads=0, inplace=False):
+    (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
+    _check(lib().ftdo_pascal(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, int(cyclic), sem,
+                             nthreads))
+    return D
+
+
+def set_rho(A, B, C):
+    n_row, n_sys = np.shape(A)
+    A, B, C = (np.ascontiguousarray(x, dtype=np.float64) for x in (A, B, C))
+    ar = np.zeros(n_row + 1)
+    cr = np.zeros(n_row + 1)
+    lib().ftdo_set_rho(_p(A), _p(B), _p(C), n_sys, n_row, _p(ar), _p(cr))
+    return ar, cr
+
+
+def cal_J(rho, bound, eps, n_row, which=J_GPU_RHS):
+    return lib().ftdo_cal_J(float(rho), float(bound), float(eps), int(n_row), int(which))
+
+
+def filtered(A, B, C, D, P, version, eps, eps_direct=False, sem=SEM_GPU, cyclic=False,
+             rho=None, J_force=None, nthreads=0, inplace=False, return_J=False):
+    """rho: None (reference state before any set_rho: all zero -> full J),
+    'auto' (column-0 bounds of these A, B, C, as set_rho_device), or a
+    (A_rho, C_rho) pair of length n_row+1."""
+    if isinstance(rho, str) and rho == "auto":
+        rho = set_rho(A, B, C)
+    (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
+    ar = cr = None
+    if rho is not None:
+        ar = np.ascontiguousarray(rho[0], dtype=np.float64)
+        cr = np.ascontiguousarray(rho[1], dtype=np.float64)
+    Jout = np.zeros(P, dtype=np.int32)
+    jf = None
+    if J_force is not None:
+        jf = np.ascontiguousarray(np.broadcast_to(np.asarray(J_force, dtype=np.int32), (P,)))
+    ip = ctypes.POINTER(ctypes.c_int)
+    _check(lib().ftdo_filtered(
+        _p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, version, int(cyclic), sem,
+        _p(ar), _p(cr), float(eps), int(bool(eps_direct)),
+        jf.ctypes.data_as(ip) if jf is not None else None, Jout.ctypes.data_as(ip), nthreads))
+    return (D, Jout) if return_J else D
+
+
+def para_range(start, end, nproc, rank):
+    a, b = ctypes.c_int(), ctypes.c_int()
+    lib().ftdo_para_range(start, end, nproc, rank, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+# --------------------------- reference CPU build ---------------------------
+
+def ref_available() -> bool:
+    return os.path.exists(REF_DRIVER) and os.path.exists(MPIEXEC)
+
+
+_KIND = {"filtered": 0, "filtered_v2": 1, "pascal": 2}
+
+
+def ref_run(mode, kind, A, B, C, D, P, cyclic=False, do_rho=True, eps_constant=0.0,
+            J=None, timeout=300):
+    """Solve with the reference's own CPU code (oracle/_ref/ref_driver).
+    mode 'backend' runs TdmaBackend under mpiexec -n P; mode 'steps' runs the
+    filtered step functions with forced J in one process."""
+    if not ref_available():
+        raise RuntimeError("reference CPU build missing: make -C oracle -f Makefile.ref")
+    n_row, n_sys = np.shape(D)
+    Jv = np.zeros(P, dtype=np.int32) if J is None else np.broadcast_to(
+        np.asarray(J, dtype=np.int32), (P,)).copy()
+    tmp = tempfile.mkdtemp(prefix="ftdref_")
+    try:
+        fin = os.path.join(tmp, "in.bin")
+        fout = os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(b"FTDI")
+            f.write(struct.pack("<6i", n_sys, n_row, P, _KIND[kind], int(cyclic), int(do_rho)))
+            f.write(struct.pack("<d", float(eps_constant)))
+            f.write(Jv.astype("<i4").tobytes())
+            for x in (A, B, C, D):
+                f.write(np.ascontiguousarray(x, dtype="<f8").tobytes())
+        if mode == "backend":
+            cmd = [MPIEXEC, "-n", str(P), REF_DRIVER, "backend", fin, fout]
+        else:
+            cmd = [REF_DRIVER, "steps", fin, fout]
+        subprocess.run(cmd, check=True, timeout=timeout, capture_output=True)
+        return np.fromfile(fout, dtype="<f8").reshape(n_row, n_sys)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
