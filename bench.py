@@ -1,0 +1,174 @@
+"""Throughput of the batched tridiagonal solve on one MI355X.
+
+Workload (the reference's per-GPU share of a 512^3 z-sweep): n_sys = 262144
+(512 x 512) systems of n_row = 512 rows in FP64, P = 8 partitions,
+a = c = -r_s, b = 1 + 2 r_s with r_s ~ U[0.5, 20] per system, D ~ U[-1, 1]
+(seed 0, counter-based generator in filtered_tdma_amd/cases.py), eps_ = 1e-10.
+
+A step is one solve of the whole batch. Every timed step solves its own
+pristine copy of D (A, B, C are not modified by the fused kernels), so no
+restore traffic enters the timed region. The headline `value` is the exact
+FP64 PaScaL path in rows/s; the filtered kinds are reported beside it.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_SYS, N_ROW, P, EPS = 262144, 512, 8, 1e-10
+R_LO, R_HI, SEED = 0.5, 20.0, 0
+BYTES_PER_ROW = 40  # read A, B, C, D and write x, FP64
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_262144x512.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kinds", default="pascal,filtered_v2,filtered")
+    return ap.parse_args()
+
+
+def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
+    """Wall-clock of `steps` solves (each on its own copy of D) between
+    synchronisations, plus per-solve device time from events on the stream
+    the kernels run on."""
+    from filtered_tdma_amd import TdmaBackendGPU
+
+    be = TdmaBackendGPU(kind, N_SYS, N_ROW, P, 0.0)
+    be.set_eps(EPS)
+    be.set_rho_device(tA, tB, tC)
+    stream = torch.cuda.current_stream()
+    for k in range(warmup):
+        copies[k % len(copies)].copy_(D0)
+        be.solve(tA, tB, tC, copies[k % len(copies)])
+    for c in copies:
+        c.copy_(D0)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        be.solve(tA, tB, tC, copies[k])
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    path = be.last_path()
+    J = be.last_J() if kind != "pascal" else None
+    be.close()
+    return wall, ev_ms, path, J
+
+
+def cpu_baseline(A, B, C, D):
+    """The oracle's PaScaL (P = 8) on the host cores, whole batch, in place on
+    copies made outside the timed region; reps until ~10 s have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    nth = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    times = []
+    budget = time.perf_counter() + 10.0
+    while time.perf_counter() < budget or not times:
+        work = [a.copy() for a in (A, B, C, D)]
+        t = time.perf_counter()
+        O.pascal(*work, P, sem=O.SEM_GPU, nthreads=nth, inplace=True)
+        times.append(time.perf_counter() - t)
+        if len(times) >= 8:
+            break
+    best = min(times)
+    return {"value": N_SYS * N_ROW / best, "unit": "rows/s", "cores": nth, "kind": "port",
+            "sample": f"oracle pascal P={P} on the full {N_SYS}x{N_ROW} batch, "
+                      f"{len(times)} reps, best of reps, OpenMP over systems"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(systems shard across GPUs with no communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from filtered_tdma_amd import cases
+
+    A, B, C, D, _ = cases.toeplitz_batch(N_SYS, N_ROW, R_LO, R_HI, seed=SEED)
+    tA, tB, tC = (torch.from_numpy(x).cuda() for x in (A, B, C))
+    D0 = torch.from_numpy(D).cuda()
+    copies = [torch.empty_like(D0) for _ in range(args.steps)]
+    rows = N_SYS * N_ROW
+    results = {}
+    for kind in args.kinds.split(","):
+        wall, ev_ms, path, J = time_kind(kind, tA, tB, tC, D0, copies, args.steps, args.warmup,
+                                         torch)
+        ms = wall * 1e3 / args.steps
+        ev_avg = statistics.mean(ev_ms)
+        results[kind] = {
+            "rows_per_s": rows / (wall / args.steps),
+            "ms_per_step": ms,
+            "event_ms_mean": ev_avg,
+            "event_ms_median": statistics.median(ev_ms),
+            "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ev_avg * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "path": path,
+            "J": J,
+        }
+        print(f"[bench] {kind}: {ms:.4f} ms/step (events {ev_avg:.4f} ms) {path} J={J}",
+              file=sys.stderr, flush=True)
+
+    head = results["pascal"] if "pascal" in results else next(iter(results.values()))
+    achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("pascal_bytes_per_launch")
+    line = {
+        "metric": "rows solved/s",
+        "value": head["rows_per_s"],
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched tridiagonal solve, pascal, n_sys={N_SYS} n_row={N_ROW} P={P}, "
+                        f"a=c=-r b=1+2r r~U[{R_LO},{R_HI}], D~U[-1,1] seed {SEED}, eps_={EPS}",
+            "n_sys": N_SYS, "n_row": N_ROW, "partitions": P, "kind": "pascal",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "backends": results,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(A, B, C, D)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+"""Loader of libftdma.so, the HIP engine behind include/ftdma.h.
+
+There is no fallback: if the library is missing or does not load, every
+entry point raises. Build it with `python -m filtered_tdma_amd.build`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libftdma.so")
+
+# every symbol include/ftdma.h declares
+EXPORTS = (
+    "ftd_kind_from_string", "ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr",
+    "ftd_solve_cyclic", "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
+    "ftd_set_profiling", "ftd_last_J", "ftd_last_gpu_ms", "ftd_last_comm_ms", "ftd_set_tuning",
+    "ftd_last_path", "ftd_last_error_string",
+)
+
+_lib = None
+
+
+class FtdError(RuntimeError):
+    """A non-zero ftd_status, with the library's error text."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ftdma status {status}: {msg}")
+        self.status = status
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FtdError(-1, f"{LIB_PATH} is not built; run `python -m filtered_tdma_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    L.ftd_kind_from_string.argtypes = [ctypes.c_char_p]
+    L.ftd_kind_from_string.restype = c_int
+    L.ftd_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, c_double, vp]
+    L.ftd_destroy.argtypes = [vp]
+    L.ftd_solve.argtypes = [vp, dp, dp, dp, dp]
+    L.ftd_solve_nr.argtypes = [vp, dp, dp, dp, dp, c_int, c_int]
+    L.ftd_solve_cyclic.argtypes = [vp, dp, dp, dp, dp]
+    L.ftd_set_rho.argtypes = [vp, dp, dp, dp]
+    L.ftd_set_eps_constant.argtypes = [vp, c_double]
+    L.ftd_set_eps.argtypes = [vp, c_double]
+    L.ftd_set_stream.argtypes = [vp, vp]
+    L.ftd_set_profiling.argtypes = [vp, c_int]
+    L.ftd_last_J.argtypes = [vp, ip, c_int]
+    L.ftd_last_gpu_ms.argtypes = [vp]
+    L.ftd_last_gpu_ms.restype = c_double
+    L.ftd_last_comm_ms.argtypes = [vp]
+    L.ftd_last_comm_ms.restype = c_double
+    L.ftd_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
+    L.ftd_last_path.argtypes = [vp]
+    L.ftd_last_path.restype = ctypes.c_char_p
+    L.ftd_last_error_string.restype = ctypes.c_char_p
+    for name in ("ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr", "ftd_solve_cyclic",
+                 "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
+                 "ftd_set_profiling", "ftd_last_J", "ftd_set_tuning"):
+        getattr(L, name).restype = c_int
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise FtdError(status, load().ftd_last_error_string().decode())

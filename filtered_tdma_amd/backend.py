@@ -1,0 +1,143 @@
+"""Python mirror of the reference's TdmaBackendGPU
+(apps/heat_gpu/tdma_backend_gpu.hpp:21-70) over the C ABI of libftdma.so.
+
+Same names and meaning as the reference class: Kind, parse, solve,
+solve_cyclic, set_rho_device, set_eps_constant, last_comm_ms, last_gpu_ms,
+plus last_J (FilteredTDMACUDA::last_J, filtered_tdma_cuda.cu:83-87). Errors
+raise FtdError instead of calling exit(1).
+
+Arrays are anything exposing a device pointer: torch CUDA tensors (float64,
+contiguous, shape [n_row, n_sys]) or raw integer addresses. torch is used as
+plumbing for device memory and streams only; every solve runs the HIP
+kernels of libftdma.so.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+
+from . import _lib
+
+
+class Kind(enum.IntEnum):
+    """TdmaBackendGPU::Kind (tdma_backend_gpu.hpp:23)."""
+    FILTERED = 0
+    FILTERED_V2 = 1
+    PASCAL = 2
+
+
+def _ptr(x, n_row=None, n_sys=None):
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    import torch
+
+    if not isinstance(x, torch.Tensor):
+        raise TypeError("arrays must be torch tensors or integer device addresses")
+    if not x.is_cuda or x.dtype != torch.float64 or not x.is_contiguous():
+        raise ValueError("arrays must be contiguous float64 device tensors")
+    if n_row is not None and tuple(x.shape) != (n_row, n_sys):
+        raise ValueError(f"expected shape ({n_row}, {n_sys}), got {tuple(x.shape)}")
+    return ctypes.c_void_p(x.data_ptr())
+
+
+class TdmaBackendGPU:
+    Kind = Kind
+
+    @staticmethod
+    def parse(s: str) -> Kind:
+        """tdma_backend_gpu.cu:6-13: case-insensitive, unknown -> FILTERED."""
+        return Kind(_lib.load().ftd_kind_from_string(None if s is None else s.encode()))
+
+    def __init__(self, kind, n_sys: int, n_row: int, partitions: int = 1,
+                 eps_constant: float = 0.0, stream=None):
+        L = _lib.load()
+        if isinstance(kind, str):
+            kind = self.parse(kind)
+        self.kind = Kind(kind)
+        self.n_sys, self.n_row, self.partitions = int(n_sys), int(n_row), int(partitions)
+        h = ctypes.c_void_p()
+        _lib.check(L.ftd_create(ctypes.byref(h), int(self.kind), self.n_sys, self.n_row,
+                                self.partitions, float(eps_constant), self._stream(stream)))
+        self._h = h
+
+    @staticmethod
+    def _stream(stream):
+        if stream is None:
+            try:
+                import torch
+
+                if torch.cuda.is_available():
+                    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            except ImportError:
+                pass
+            return ctypes.c_void_p(0)
+        if isinstance(stream, int):
+            return ctypes.c_void_p(stream)
+        return ctypes.c_void_p(stream.cuda_stream)
+
+    def _args(self, A, B, C, D, n_row=None, n_sys=None):
+        n_row = self.n_row if n_row is None else n_row
+        n_sys = self.n_sys if n_sys is None else n_sys
+        return [_ptr(x, n_row, n_sys) for x in (A, B, C, D)]
+
+    def solve(self, A, B, C, D) -> None:
+        _lib.check(_lib.load().ftd_solve(self._h, *self._args(A, B, C, D)))
+
+    def solve_nr(self, A, B, C, D, n_sys: int, n_row: int) -> None:
+        _lib.check(_lib.load().ftd_solve_nr(self._h, *self._args(A, B, C, D, n_row, n_sys),
+                                            int(n_sys), int(n_row)))
+
+    def solve_cyclic(self, A, B, C, D) -> None:
+        _lib.check(_lib.load().ftd_solve_cyclic(self._h, *self._args(A, B, C, D)))
+
+    def set_rho_device(self, A, B, C) -> None:
+        a, b, c, _ = self._args(A, B, C, A)
+        _lib.check(_lib.load().ftd_set_rho(self._h, a, b, c))
+
+    def set_eps_constant(self, eps_constant: float) -> None:
+        _lib.check(_lib.load().ftd_set_eps_constant(self._h, float(eps_constant)))
+
+    def set_eps(self, eps: float) -> None:
+        _lib.check(_lib.load().ftd_set_eps(self._h, float(eps)))
+
+    def set_stream(self, stream) -> None:
+        _lib.check(_lib.load().ftd_set_stream(self._h, self._stream(stream)))
+
+    def set_profiling(self, enable: bool) -> None:
+        _lib.check(_lib.load().ftd_set_profiling(self._h, int(bool(enable))))
+
+    def set_tuning(self, rows_per_lane=0, max_threads=0, systems_per_tile=0,
+                   lanes_per_system=0) -> None:
+        _lib.check(_lib.load().ftd_set_tuning(self._h, rows_per_lane, max_threads,
+                                              systems_per_tile, lanes_per_system))
+
+    def last_J(self) -> list:
+        out = (ctypes.c_int * self.partitions)()
+        _lib.check(_lib.load().ftd_last_J(self._h, out, self.partitions))
+        return list(out)
+
+    def last_gpu_ms(self) -> float:
+        return _lib.load().ftd_last_gpu_ms(self._h)
+
+    def last_comm_ms(self) -> float:
+        return _lib.load().ftd_last_comm_ms(self._h)
+
+    def last_path(self) -> str:
+        return _lib.load().ftd_last_path(self._h).decode()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().ftd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,439 @@
+// ftdma_api.cpp -- C ABI of the batched tridiagonal engine (include/ftdma.h).
+//
+// Host side of the drop-in for TdmaBackendGPU
+// (apps/heat_gpu/tdma_backend_gpu.{hpp,cu}): it owns the device scratch of a
+// handle, picks a kernel path per solve and orders every launch on the
+// handle's stream. No MPI: the reference's z-slab ranks become partitions of
+// each system on one GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ftdma.h"
+#include "ftdma_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+ftd_status fail(ftd_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+ftd_status hip_fail(hipError_t e, const char* what) {
+    return fail(FTD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define FTD_HIP(call)                                     \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+#define FTD_TRY(expr)                  \
+    do {                               \
+        ftd_status s_ = (expr);        \
+        if (s_ != FTD_OK) return s_;   \
+    } while (0)
+
+void part_of(int n, int P, int p, int& s, int& m) {
+    int base = n / P, rem = n % P;
+    s = p * base + (p < rem ? p : rem);
+    m = base + (p < rem ? 1 : 0);
+}
+
+int pow2_floor(int v) {
+    int r = 1;
+    while (r * 2 <= v) r *= 2;
+    return r;
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ftd_handle {
+    int kind = FTD_KIND_FILTERED;
+    int n_sys = 0, n_row = 0, P = 1;
+    double eps_value = 0.0;  // eps_constant, or eps_ itself when eps_direct
+    int eps_direct = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int profiling = 0;
+    double last_ms = 0.0;
+    char path[96] = "none";
+    int t_M = 0, t_T = 0, t_S = 0, t_Q = 0;  // forced tuning (0 = automatic)
+    int force_general = 0;                   // use the general kernels only
+    double* rho = nullptr;                   // A_rho[n_row+1] then C_rho[n_row+1]
+    int* J = nullptr;                        // J per partition
+    DevBuf partial, xbuf, R, E;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+ftd_status ensure(DevBuf& b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return FTD_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return hip_fail(e, "hipFree");
+    }
+    hipError_t e = hipMalloc(&b.p, bytes + 256);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(FTD_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return FTD_OK;
+}
+
+// ---------------- PaScaL / Thomas (exact) ----------------
+
+struct FusedShape {
+    int M = 0, T = 0, S = 0, Q = 0;
+};
+
+// Fused-kernel shape for an exact solve of n_row rows: Q lanes per system
+// (sub-partitions of <= M rows, >= 3 rows each), S systems per workgroup.
+bool pick_pascal(const ftd_handle* h, int n_row, int cyclic, FusedShape& out) {
+    static const int Ms[] = {32, 16, 64, 8};
+    static const int Ts[] = {256, 512, 256, 1024};
+    const int ncand = h->t_M ? 1 : 4;
+    for (int k = 0; k < ncand; ++k) {
+        const int M = h->t_M ? h->t_M : Ms[k];
+        const int T = h->t_M ? (h->t_T ? h->t_T : 256) : Ts[k];
+        if (!ftd::pascal_fused_supported(M, T)) continue;
+        int Q = h->t_Q ? h->t_Q : (n_row + M - 1) / M;
+        if (cyclic && Q < 2) Q = 2;
+        if (Q < 1 || Q > T) continue;
+        int s_, m_min, m_max;
+        part_of(n_row, Q, Q - 1, s_, m_min);
+        part_of(n_row, Q, 0, s_, m_max);
+        if (m_min < 3 || m_max > M) continue;
+        const int S = h->t_S ? h->t_S : std::min(64, pow2_floor(T / Q));
+        if (!is_pow2(S) || S > 64 || S * Q > T) continue;
+        if (!h->t_S && !h->t_M && S < 16) continue;  // keep >= 128-byte row segments
+        out.M = M;
+        out.T = T;
+        out.S = S;
+        out.Q = Q;
+        return true;
+    }
+    return false;
+}
+
+ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D, int n_sys,
+                       int n_row, int cyclic) {
+    hipStream_t st = h->stream;
+    if (cyclic && n_row < 3) return fail(FTD_ERR_UNSUPPORTED, "periodic solve needs n_row >= 3");
+    FusedShape fs;
+    if (!h->force_general && n_row >= 3 && pick_pascal(h, n_row, cyclic, fs)) {
+        std::snprintf(h->path, sizeof h->path, "pascal_fused M=%d T=%d S=%d Q=%d", fs.M, fs.T,
+                      fs.S, fs.Q);
+        FTD_HIP(ftd::launch_pascal_fused(fs.M, fs.T, A, B, C, D, n_sys, n_row, fs.Q, fs.S, cyclic,
+                                         st));
+        return FTD_OK;
+    }
+    if (h->t_M) return fail(FTD_ERR_UNSUPPORTED, "forced fused shape does not fit this system");
+    int P = h->P;
+    if (P > 1) {
+        int s_, ml;
+        part_of(n_row, P, P - 1, s_, ml);
+        if (ml < 3) P = 1;
+    }
+    if (P == 1 && !cyclic) {
+        std::snprintf(h->path, sizeof h->path, "thomas_general");
+        FTD_HIP(ftd::launch_gen_thomas(A, B, C, D, n_sys, n_row, st));
+        return FTD_OK;
+    }
+    if (P == 1) {
+        FTD_TRY(ensure(h->E, sizeof(double) * (size_t)n_row * n_sys));
+        std::snprintf(h->path, sizeof h->path, "thomas_cyclic_general");
+        FTD_HIP(ftd::launch_gen_thomas_cyclic(A, B, C, D, (double*)h->E.p, n_sys, n_row, st));
+        return FTD_OK;
+    }
+    FTD_TRY(ensure(h->R, sizeof(double) * (size_t)6 * P * n_sys));
+    FTD_TRY(ensure(h->E, sizeof(double) * (size_t)2 * P * n_sys));
+    std::snprintf(h->path, sizeof h->path, "pascal_general P=%d", P);
+    FTD_HIP(ftd::launch_gen_pascal(A, B, C, D, n_sys, n_row, P, cyclic, (double*)h->R.p,
+                                   (double*)h->E.p, st));
+    return FTD_OK;
+}
+
+// ---------------- Filtered ----------------
+
+ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double* D, int cyclic) {
+    const int n_sys = h->n_sys, n_row = h->n_row, P = h->P;
+    const int version = (h->kind == FTD_KIND_FILTERED_V2) ? 2 : 1;
+    hipStream_t st = h->stream;
+    int s_, mmax, mmin;
+    part_of(n_row, P, 0, s_, mmax);
+    part_of(n_row, P, P - 1, s_, mmin);
+    if (mmin < 3) return fail(FTD_ERR_UNSUPPORTED, "every partition needs at least 3 rows");
+
+    // fused shape: smallest compiled M holding the largest partition
+    FusedShape fs;
+    if (!h->force_general) {
+        static const int Ms[] = {8, 16, 32, 64};
+        static const int Ts[] = {1024, 512, 256, 256};
+        for (int k = 0; k < 4; ++k) {
+            if (h->t_M && Ms[k] != h->t_M) continue;
+            if (Ms[k] < mmax) continue;
+            const int T = (h->t_M && h->t_T) ? h->t_T : Ts[k];
+            if (!ftd::filtered_fused_supported(Ms[k], T) || P > T) continue;
+            const int S = h->t_S ? h->t_S : std::min(64, pow2_floor(T / P));
+            if (!is_pow2(S) || S > 64 || S * P > T) continue;
+            fs.M = Ms[k];
+            fs.T = T;
+            fs.S = S;
+            break;
+        }
+    }
+    if (h->t_M && !fs.M) return fail(FTD_ERR_UNSUPPORTED, "forced fused shape does not fit this system");
+
+    int* Jp = h->J;
+    const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
+    if (fs.M) {
+        const int tiles = (n_sys + fs.S - 1) / fs.S;
+        if (version == 2) {
+            int nt = 0;
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * P));
+            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+                                          h->eps_direct, Jp, st));
+            std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d", fs.M, fs.T,
+                          fs.S);
+            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
+                                               cyclic, Jp, nullptr, nullptr, st));
+        } else {
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+            FTD_TRY(ensure(h->xbuf, xbytes));
+            std::snprintf(h->path, sizeof h->path, "filtered_v1_fused M=%d T=%d S=%d", fs.M, fs.T,
+                          fs.S);
+            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
+                                               cyclic, Jp, (double*)h->xbuf.p,
+                                               (double*)h->partial.p, st));
+            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
+                                          h->eps_value, h->eps_direct, Jp, st));
+            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
+                                               cyclic, Jp, (double*)h->xbuf.p, nullptr, st));
+        }
+        return FTD_OK;
+    }
+
+    // general path: sweeps in place, interface, J, final
+    const int tiles = (n_sys + 255) / 256;
+    FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+    FTD_TRY(ensure(h->xbuf, xbytes));
+    std::snprintf(h->path, sizeof h->path, "filtered_v%d_general", version);
+    int nt = 0;
+    if (version == 2) {
+        FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+        FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+                                      h->eps_direct, Jp, st));
+    }
+    FTD_HIP(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
+    FTD_HIP(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
+                                           (double*)h->partial.p, &nt, st));
+    if (version == 1)
+        FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+                                      h->eps_direct, Jp, st));
+    FTD_HIP(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
+    return FTD_OK;
+}
+
+ftd_status do_solve(ftd_handle* h, double* A, double* B, double* C, double* D, int n_sys,
+                    int n_row, int cyclic) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (n_sys < 0 || n_row < 1) return fail(FTD_ERR_INVALID, "need n_sys >= 0 and n_row >= 1");
+    if (n_sys == 0) return FTD_OK;
+    const bool filtered = h->kind != FTD_KIND_PASCAL && h->P > 1;
+    if (filtered && (n_sys != h->n_sys || n_row != h->n_row))
+        return fail(FTD_ERR_INVALID, "filtered solves use the dimensions of ftd_create");
+    if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
+    ftd_status s = filtered ? solve_filtered(h, A, B, C, D, cyclic)
+                            : solve_exact(h, A, B, C, D, n_sys, n_row, cyclic);
+    if (s != FTD_OK) return s;
+    if (h->profiling) {
+        FTD_HIP(hipEventRecord(h->ev1, h->stream));
+        FTD_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0.0f;
+        FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+extern "C" {
+
+int ftd_kind_from_string(const char* s) {
+    if (!s) return FTD_KIND_FILTERED;
+    std::string t;
+    for (const char* c = s; *c; ++c) t.push_back((char)std::tolower((unsigned char)*c));
+    if (t == "pascal" || t == "pascal_tdma") return FTD_KIND_PASCAL;
+    if (t == "filtered_v2") return FTD_KIND_FILTERED_V2;
+    return FTD_KIND_FILTERED;
+}
+
+ftd_status ftd_create(ftd_handle** out, int kind, int n_sys, int n_row, int partitions,
+                      double eps_constant, void* stream) {
+    if (!out) return fail(FTD_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (kind < FTD_KIND_FILTERED || kind > FTD_KIND_PASCAL)
+        return fail(FTD_ERR_INVALID, "kind must be FTD_KIND_FILTERED, _FILTERED_V2 or _PASCAL");
+    if (n_sys < 0 || n_row < 1 || partitions < 1)
+        return fail(FTD_ERR_INVALID, "need n_sys >= 0, n_row >= 1, partitions >= 1");
+    if (partitions > n_row) return fail(FTD_ERR_INVALID, "more partitions than rows");
+    ftd_handle* h = new (std::nothrow) ftd_handle();
+    if (!h) return fail(FTD_ERR_ALLOC, "out of host memory");
+    h->kind = kind;
+    h->n_sys = n_sys;
+    h->n_row = n_row;
+    h->P = partitions;
+    h->eps_value = eps_constant;
+    h->stream = (hipStream_t)stream;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipMalloc(&h->rho, sizeof(double) * 2 * (size_t)(n_row + 1));
+    if (e == hipSuccess) e = hipMemset(h->rho, 0, sizeof(double) * 2 * (size_t)(n_row + 1));
+    if (e == hipSuccess) e = hipMalloc(&h->J, sizeof(int) * (size_t)partitions);
+    if (e == hipSuccess) {
+        std::vector<int> neg(partitions, -1);
+        e = hipMemcpy(h->J, neg.data(), sizeof(int) * partitions, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e != hipSuccess) {
+        ftd_destroy(h);
+        return hip_fail(e, "ftd_create");
+    }
+    *out = h;
+    return FTD_OK;
+}
+
+ftd_status ftd_destroy(ftd_handle* h) {
+    if (!h) return FTD_OK;
+    if (h->rho) hipFree(h->rho);
+    if (h->J) hipFree(h->J);
+    for (DevBuf* b : {&h->partial, &h->xbuf, &h->R, &h->E})
+        if (b->p) hipFree(b->p);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    delete h;
+    return FTD_OK;
+}
+
+ftd_status ftd_solve(ftd_handle* h, double* A, double* B, double* C, double* D) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    return do_solve(h, A, B, C, D, h->n_sys, h->n_row, 0);
+}
+
+ftd_status ftd_solve_nr(ftd_handle* h, double* A, double* B, double* C, double* D, int n_sys,
+                        int n_row) {
+    return do_solve(h, A, B, C, D, n_sys, n_row, 0);
+}
+
+ftd_status ftd_solve_cyclic(ftd_handle* h, double* A, double* B, double* C, double* D) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    return do_solve(h, A, B, C, D, h->n_sys, h->n_row, 1);
+}
+
+ftd_status ftd_set_rho(ftd_handle* h, const double* A, const double* B, const double* C) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (h->kind == FTD_KIND_PASCAL) return FTD_OK;
+    if (!A || !B || !C) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (h->n_sys == 0) return FTD_OK;
+    FTD_HIP(ftd::launch_set_rho(A, B, C, h->n_sys, h->n_row, h->rho, h->stream));
+    return FTD_OK;
+}
+
+ftd_status ftd_set_eps_constant(ftd_handle* h, double eps_constant) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (h->kind == FTD_KIND_PASCAL) return FTD_OK;
+    h->eps_value = eps_constant;
+    h->eps_direct = 0;
+    return FTD_OK;
+}
+
+ftd_status ftd_set_eps(ftd_handle* h, double eps) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    h->eps_value = eps;
+    h->eps_direct = 1;
+    return FTD_OK;
+}
+
+ftd_status ftd_set_stream(ftd_handle* h, void* stream) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    h->stream = (hipStream_t)stream;
+    return FTD_OK;
+}
+
+ftd_status ftd_set_profiling(ftd_handle* h, int enable) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    h->profiling = enable ? 1 : 0;
+    if (!enable) h->last_ms = 0.0;
+    return FTD_OK;
+}
+
+ftd_status ftd_last_J(ftd_handle* h, int* J, int n) {
+    if (!h || !J) return fail(FTD_ERR_INVALID, "null argument");
+    if (n < 0 || n > h->P) return fail(FTD_ERR_INVALID, "n must be in [0, partitions]");
+    if (n == 0) return FTD_OK;
+    FTD_HIP(hipMemcpyAsync(J, h->J, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    FTD_HIP(hipStreamSynchronize(h->stream));
+    return FTD_OK;
+}
+
+double ftd_last_gpu_ms(const ftd_handle* h) { return h ? h->last_ms : 0.0; }
+
+double ftd_last_comm_ms(const ftd_handle* h) {
+    (void)h;
+    return 0.0;
+}
+
+ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int systems_per_tile,
+                          int lanes_per_system) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (rows_per_lane == 0 || rows_per_lane == -1) {
+        h->t_M = h->t_T = h->t_S = h->t_Q = 0;
+        h->force_general = rows_per_lane == -1;
+        return FTD_OK;
+    }
+    h->force_general = 0;
+    if (!ftd::pascal_fused_supported(rows_per_lane, max_threads) &&
+        !ftd::filtered_fused_supported(rows_per_lane, max_threads))
+        return fail(FTD_ERR_UNSUPPORTED, "no compiled fused kernel with this rows_per_lane/max_threads");
+    if (systems_per_tile && (!is_pow2(systems_per_tile) || systems_per_tile > 64))
+        return fail(FTD_ERR_INVALID, "systems_per_tile must be a power of two <= 64");
+    if (lanes_per_system < 0) return fail(FTD_ERR_INVALID, "lanes_per_system must be >= 0");
+    h->t_M = rows_per_lane;
+    h->t_T = max_threads;
+    h->t_S = systems_per_tile;
+    h->t_Q = lanes_per_system;
+    return FTD_OK;
+}
+
+const char* ftd_last_path(const ftd_handle* h) { return h ? h->path : "none"; }
+
+const char* ftd_last_error_string(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,985 @@
+// ftdma_kernels.hip -- gfx950 kernels of the batched tridiagonal engine.
+//
+// Arrays are row-major [n_row x n_sys] (row j of system i at j*n_sys + i), so
+// the lanes of a wavefront that hold consecutive systems read consecutive
+// addresses of one row: every row load of a wave is one or more contiguous
+// 128..512-byte segments.
+//
+// Two families of kernels:
+//
+//  * fused kernels (k_pascal_fused, k_filtered_fused): one lane per
+//    (system, partition). A lane keeps the rows of its partition in
+//    registers (M rows x 3 doubles), the partitions of S systems meet in LDS
+//    for the reduced solve, and D is written once. HBM traffic is the
+//    algorithmic minimum: read A,B,C,D once, write x once (40 B/row).
+//
+//  * general kernels (k_gen_*): one lane per (system, partition) walking its
+//    rows in global memory, in place in A, C, D as the reference contract
+//    allows (A and C clobbered). Used where a partition has more rows than
+//    the fused kernels hold in registers.
+//
+// Arithmetic follows the reference statement by statement (reciprocal then
+// multiply where it multiplies, division where it divides); the library is
+// built with -ffp-contract=off so that no product is fused into a sum, which
+// makes the results bit-identical to the CPU restatement for the same
+// partitioning.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "ftdma_kernels.h"
+
+namespace ftd {
+
+// para_range over [0, n) in P parts (libs/pascal_tdma/cpu/para_range.cpp:4-9)
+__host__ __device__ inline void part_of(int n, int P, int p, int& s, int& m) {
+    int base = n / P, rem = n % P;
+    s = p * base + (p < rem ? p : rem);
+    m = base + (p < rem ? 1 : 0);
+}
+
+// float -> int truncation that saturates instead of being undefined, NaN -> 0
+__device__ inline long long trunc_sat(double x) {
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647LL;
+    if (x <= -2147483648.0) return -2147483648LL;
+    return (long long)(int)x;
+}
+
+// k_cal_J_rhs_bound's formula (filtered_tdma_kernels.cuh:245-259) with the
+// clamp to [0, n_row-1]
+__device__ inline int cal_J_gpu(double rho, double max_b, double eps, int n_row) {
+    if (rho == 0.0 || rho >= 0.5) return n_row - 1;
+    double lambda_p = (1.0 + sqrt(1.0 - 4.0 * rho * rho)) * 0.5;
+    double q = rho / lambda_p;
+    double Bv = q * (2.0 + q) / ((1.0 - q) * (1.0 - 2.0 * rho)) * max_b;
+    long long J = trunc_sat(log(eps / Bv) / log(q)) + 1;
+    if (J > n_row - 1) J = n_row - 1;
+    if (J < 0) J = 0;
+    return (int)J;
+}
+
+// max over aligned groups of `width` lanes (width a power of two <= 64)
+__device__ inline double group_max(double v, int width) {
+    for (int off = width >> 1; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// column-0 spectral bounds (filtered_tdma_kernels.cuh:46-67)
+// ---------------------------------------------------------------------------
+__global__ void k_set_rho(const double* __restrict__ A, const double* __restrict__ B,
+                          const double* __restrict__ C, int n_sys, int n_row,
+                          double* __restrict__ A_rho, double* __restrict__ C_rho) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n_row) return;
+    if (k < n_row) {
+        double bk = B[(size_t)k * n_sys];
+        A_rho[k] = (bk != 0.0) ? fabs(A[(size_t)k * n_sys] / bk) : 0.0;
+        C_rho[k] = (bk != 0.0) ? fabs(C[(size_t)k * n_sys] / bk) : 0.0;
+    } else {
+        A_rho[k] = 0.0;
+        C_rho[k] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// J machinery: per-tile partial maxima -> per-partition J
+// ---------------------------------------------------------------------------
+
+// max over systems of |D[row 0 of p]| and |D[row m_p-1 of p]|, one block of
+// 256 systems per (tile, partition); partial[tile * P + p]
+__global__ __launch_bounds__(256) void k_boundary_maxabs(const double* __restrict__ D, int n_sys,
+                                                         int n_row, int P,
+                                                         double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int p = blockIdx.y;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < n_sys) {
+        double a = fabs(D[(size_t)s * n_sys + i]);
+        double b = fabs(D[(size_t)(s + m - 1) * n_sys + i]);
+        v = fmax(a, b);
+    }
+    v = group_max(v, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        partial[(size_t)blockIdx.x * P + p] = w;
+    }
+}
+
+// J_p from rho (column-0 bounds of the partition's rows 1..m_p-1), the
+// reduced max_b and eps_p; one block per partition
+__global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ partial, int n_tiles,
+                                                   int P, int n_row,
+                                                   const double* __restrict__ A_rho,
+                                                   const double* __restrict__ C_rho,
+                                                   double eps_value, int eps_direct,
+                                                   int* __restrict__ J_out) {
+    __shared__ double red_b[4], red_r[4];
+    const int p = blockIdx.x;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    double vb = 0.0, vr = 0.0;
+    for (int t = threadIdx.x; t < n_tiles; t += 256) vb = fmax(vb, partial[(size_t)t * P + p]);
+    for (int k = 1 + threadIdx.x; k < m; k += 256)
+        vr = fmax(vr, fmax(fabs(A_rho[s + k]), fabs(C_rho[s + k])));
+    vb = group_max(vb, 64);
+    vr = group_max(vr, 64);
+    if ((threadIdx.x & 63) == 0) {
+        red_b[threadIdx.x >> 6] = vb;
+        red_r[threadIdx.x >> 6] = vr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mb = fmax(fmax(red_b[0], red_b[1]), fmax(red_b[2], red_b[3]));
+        double rho = fmax(fmax(red_r[0], red_r[1]), fmax(red_r[2], red_r[3]));
+        double Nl = (double)(m * P);
+        double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        J_out[p] = cal_J_gpu(rho, mb, eps, m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused PaScaL: modified Thomas per (system, sub-partition) in registers,
+// reduced 2Q-row system in LDS, interior update, single store of x.
+// Restates modified_thomas_kernel / tdma_many_kernel / tdma_cyclic_many_kernel
+// / update_solution_kernel (libs/pascal_tdma/gpu/tdma_local_cuda.cu:67-224)
+// for Q partitions of one GPU.
+// ---------------------------------------------------------------------------
+template <int M, int MAXT, bool CYCLIC>
+__global__ __launch_bounds__(MAXT) void k_pascal_fused(const double* __restrict__ A,
+                                                       const double* __restrict__ B,
+                                                       const double* __restrict__ C,
+                                                       double* __restrict__ D, int n_sys,
+                                                       int n_row, int Q, int lgS) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int i_raw = blockIdx.x * S + sl;
+    const bool live = i_raw < n_sys;
+    const int i = live ? i_raw : n_sys - 1;  // dead lanes shadow the last system
+    int r0, cnt;
+    part_of(n_row, Q, q, r0, cnt);
+    const size_t ld = (size_t)n_sys;
+    const size_t base = (size_t)r0 * ld + (size_t)i;
+
+    double a[M], c[M], d[M];
+    double aN = 0.0, cN = 0.0, dN = 0.0;
+
+    // forward sweep: rows 0,1 divided by b, then modified elimination
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j < cnt) {
+            const size_t o = base + (size_t)j * ld;
+            const double aj = A[o], bj = B[o], cj = C[o], dj = D[o];
+            if (j < 2) {
+                a[j] = aj / bj;
+                c[j] = cj / bj;
+                d[j] = dj / bj;
+            } else {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = -inv * aj * a[j - 1];
+            }
+            if (j == cnt - 1) {
+                aN = a[j];
+                cN = c[j];
+                dN = d[j];
+            }
+        }
+    }
+    // backward sweep rows cnt-3 .. 1
+#pragma unroll
+    for (int j = M - 3; j >= 1; --j) {
+        if (j <= cnt - 3) {
+            const double cj = c[j];
+            d[j] = d[j] - cj * d[j + 1];
+            a[j] = a[j] - cj * a[j + 1];
+            c[j] = -cj * c[j + 1];
+        }
+    }
+    // row-0 2x2 reduction
+    const double r = 1.0 / (1.0 - a[1] * c[0]);
+    const double d0 = r * (d[0] - c[0] * d[1]);
+    const double a0 = r * a[0];
+    const double c0 = -r * c[0] * c[1];
+
+    const int NR = 2 * Q;
+    double* rA = lds;
+    double* rC = rA + NR * S;
+    double* rD = rC + NR * S;
+    double* rE = rD + NR * S;  // cyclic only
+    rA[(2 * q) * S + sl] = a0;
+    rC[(2 * q) * S + sl] = c0;
+    rD[(2 * q) * S + sl] = d0;
+    rA[(2 * q + 1) * S + sl] = aN;
+    rC[(2 * q + 1) * S + sl] = cN;
+    rD[(2 * q + 1) * S + sl] = dN;
+    __syncthreads();
+
+    // reduced solve, B == 1: one lane per system
+    if (q == 0) {
+        double* Ar = rA + sl;
+        double* Cr = rC + sl;
+        double* Dr = rD + sl;
+        if (!CYCLIC) {
+            for (int k = 1; k < NR; ++k) {
+                const double inv = 1.0 / (1.0 - Ar[k * S] * Cr[(k - 1) * S]);
+                Dr[k * S] = inv * (Dr[k * S] - Ar[k * S] * Dr[(k - 1) * S]);
+                Cr[k * S] = inv * Cr[k * S];
+            }
+            for (int k = NR - 2; k >= 0; --k) Dr[k * S] = Dr[k * S] - Cr[k * S] * Dr[(k + 1) * S];
+        } else {
+            double* Er = rE + sl;
+            for (int k = 0; k < NR; ++k) Er[k * S] = 0.0;
+            Er[1 * S] = -Ar[1 * S];
+            Er[(NR - 1) * S] = -Cr[(NR - 1) * S];
+            for (int k = 2; k < NR; ++k) {
+                const double inv = 1.0 / (1.0 - Ar[k * S] * Cr[(k - 1) * S]);
+                Dr[k * S] = inv * (Dr[k * S] - Ar[k * S] * Dr[(k - 1) * S]);
+                Er[k * S] = inv * (Er[k * S] - Ar[k * S] * Er[(k - 1) * S]);
+                Cr[k * S] = inv * Cr[k * S];
+            }
+            for (int k = NR - 2; k >= 1; --k) {
+                Dr[k * S] = Dr[k * S] - Cr[k * S] * Dr[(k + 1) * S];
+                Er[k * S] = Er[k * S] - Cr[k * S] * Er[(k + 1) * S];
+            }
+            Dr[0] = (Dr[0] - Ar[0] * Dr[(NR - 1) * S] - Cr[0] * Dr[1 * S]) /
+                    (1.0 + Ar[0] * Er[(NR - 1) * S] + Cr[0] * Er[1 * S]);
+            for (int k = 1; k < NR; ++k) Dr[k * S] = Dr[k * S] + Dr[0] * Er[k * S];
+        }
+    }
+    __syncthreads();
+
+    const double x0 = rD[(2 * q) * S + sl];
+    const double xN = rD[(2 * q + 1) * S + sl];
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < cnt) {
+                double x;
+                if (j == 0)
+                    x = x0;
+                else if (j == cnt - 1)
+                    x = xN;
+                else
+                    x = d[j] - (a[j] * x0 + c[j] * xN);
+                D[base + (size_t)j * ld] = x;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused Filtered v1/v2 (GPU semantics, filtered_tdma_kernels.cuh:69-303):
+// one lane per (system, partition) with its m_p <= M rows in registers.
+//   PASS 0: v2 (J_p known) -> x stored.
+//   PASS 1: v1 first pass  -> x0, xN of every (partition, system) to xbuf,
+//           per-tile max of |x0|,|xN| to partial (J needs all systems).
+//   PASS 2: v1 second pass -> recompute the sweeps, take x0, xN from xbuf and
+//           J_p, store x.
+// ---------------------------------------------------------------------------
+template <int M, int MAXT, int VERSION, int PASS>
+__global__ __launch_bounds__(MAXT) void k_filtered_fused(const double* __restrict__ A,
+                                                         const double* __restrict__ B,
+                                                         const double* __restrict__ C,
+                                                         double* __restrict__ D, int n_sys,
+                                                         int n_row, int P, int lgS, int cyclic,
+                                                         const int* __restrict__ Jp,
+                                                         double* __restrict__ xbuf,
+                                                         double* __restrict__ partial) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int p = t >> lgS;
+    const int i_raw = blockIdx.x * S + sl;
+    const bool live = i_raw < n_sys;
+    const int i = live ? i_raw : n_sys - 1;
+    int r0, m;
+    part_of(n_row, P, p, r0, m);
+    const size_t ld = (size_t)n_sys;
+    const size_t base = (size_t)r0 * ld + (size_t)i;
+    const int J = (VERSION == 2 || PASS == 2) ? Jp[p] : m;
+
+    double a[M], c[M], d[M];
+    double aN = 0.0, cN = 0.0, dN = 0.0;
+    double b1 = 1.0;
+
+    // forward sweep (k_fwd_pass_v1 / k_fwd_pass_v2)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j < m) {
+            const size_t o = base + (size_t)j * ld;
+            const double aj = A[o], bj = B[o], cj = C[o], dj = D[o];
+            if (j < 2) {
+                const double r = 1.0 / bj;
+                a[j] = aj * r;
+                c[j] = cj * r;
+                d[j] = dj * r;
+                if (j == 1) b1 = bj;
+            } else if (VERSION == 1 || j <= J) {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = -inv * aj * a[j - 1];
+            } else {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = aj;  // left raw, as the skip-A loop leaves A[j]
+            }
+            if (VERSION == 2 && j == 1 && J == 0) {
+                // k_fwd_pass_v2 with J == 0 starts its skip-A loop at row 1 and
+                // eliminates the normalised row 1 against itself
+                const double inv = 1.0 / (b1 - a[1] * c[1]);
+                d[1] = inv * (d[1] - a[1] * d[1]);
+                c[1] = inv * c[1];
+            }
+            if (j == m - 1) {
+                aN = a[j];
+                cN = c[j];
+                dN = d[j];
+            }
+        }
+    }
+    // backward sweep (k_bwd_pass_v1 / k_bwd_pass_v2)
+    const int lo = (m - 2) - J;
+#pragma unroll
+    for (int j = M - 3; j >= 1; --j) {
+        if (j <= m - 3) {
+            const double cj = c[j];
+            d[j] = d[j] - cj * d[j + 1];
+            if (VERSION == 1) {
+                a[j] = a[j] - cj * a[j + 1];
+                c[j] = -cj * c[j + 1];
+            } else {
+                if (J >= 2 && j < J) {
+                    // seed A[J]; with J == m-1 the first update (j = m-3)
+                    // reads A[m-1], as k_bwd_pass_v2 does
+                    const double an = (J == m - 1 && j == m - 3) ? aN : a[j + 1];
+                    a[j] = a[j] - cj * an;
+                }
+                if (j > lo) c[j] = -cj * c[j + 1];
+            }
+        }
+    }
+    // pack_reduced: row 0 keeps d0, a0 (c0 coupling dropped)
+    const double rr = 1.0 / (1.0 - a[1] * c[0]);
+    const double d0p = rr * (d[0] - c[0] * d[1]);
+    const double a0p = rr * a[0];
+
+    double x0, xN;
+    if (PASS == 2) {
+        x0 = xbuf[(size_t)(2 * p) * ld + i];
+        xN = xbuf[(size_t)(2 * p + 1) * ld + i];
+    } else {
+        // neighbour exchange through LDS: [C|D] of row m-1 to the right,
+        // [A|D] of row 0 to the left (filtered_tdma_cuda.cu:93-101)
+        double* sC = lds;
+        double* sDr = sC + P * S;
+        double* sA = sDr + P * S;
+        double* sDl = sA + P * S;
+        sC[p * S + sl] = cN;
+        sDr[p * S + sl] = dN;
+        sA[p * S + sl] = a0p;
+        sDl[p * S + sl] = d0p;
+        __syncthreads();
+        const bool has_left = cyclic || p > 0;
+        const bool has_right = cyclic || p < P - 1;
+        const int pl = (p + P - 1) % P, pr = (p + 1) % P;
+        x0 = d0p;
+        xN = dN;
+        if (has_left) {  // k_solve_both (filtered_tdma_kernels.cuh:284-303)
+            const double C_left = sC[pl * S + sl], D_left = sDr[pl * S + sl];
+            x0 = (d0p - a0p * D_left) / (1.0 - C_left * a0p);
+        }
+        if (has_right) {
+            const double A_right = sA[pr * S + sl], D0_right = sDl[pr * S + sl];
+            xN = (dN - cN * D0_right) / (1.0 - A_right * cN);
+        }
+    }
+
+    if (PASS == 1) {
+        if (live) {
+            xbuf[(size_t)(2 * p) * ld + i] = x0;
+            xbuf[(size_t)(2 * p + 1) * ld + i] = xN;
+        }
+        // dead lanes shadow a live system, so they do not change the max
+        double v = group_max(fmax(fabs(x0), fabs(xN)), S < 64 ? S : 64);
+        if (S <= 64) {
+            if (sl == 0) partial[(size_t)blockIdx.x * P + p] = v;
+        }
+        return;
+    }
+
+    // k_final_pass: truncated correction
+    const int jL = J < m - 2 ? J : m - 2;
+    const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < m) {
+                double x;
+                if (j == 0) {
+                    x = x0;
+                } else if (j == m - 1) {
+                    x = xN;
+                } else {
+                    x = d[j];
+                    if (j <= jL) x = x - a[j] * x0;
+                    if (j >= jR) x = x - c[j] * xN;
+                }
+                D[base + (size_t)j * ld] = x;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// general kernels: one lane per (system, partition), rows walked in global
+// memory, in place (A and C clobbered, as the reference allows)
+// ---------------------------------------------------------------------------
+
+// plain Thomas, P == 1 (tdma_many_kernel, tdma_local_cuda.cu:32-64)
+__global__ __launch_bounds__(256) void k_gen_thomas(const double* __restrict__ A,
+                                                    const double* __restrict__ B,
+                                                    double* __restrict__ C, double* __restrict__ D,
+                                                    int n_sys, int n_row) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sys) return;
+    const size_t ld = (size_t)n_sys;
+    double b = B[i], cp = C[i] / b, dp = D[i] / b;
+    C[i] = cp;
+    D[i] = dp;
+    for (int j = 1; j < n_row; ++j) {
+        const size_t o = (size_t)j * ld + i;
+        const double aj = A[o];
+        const double r = 1.0 / (B[o] - aj * cp);
+        dp = r * (D[o] - aj * dp);
+        cp = r * C[o];
+        C[o] = cp;
+        D[o] = dp;
+    }
+    double xn = dp;
+    for (int j = n_row - 2; j >= 0; --j) {
+        const size_t o = (size_t)j * ld + i;
+        xn = D[o] - C[o] * xn;
+        D[o] = xn;
+    }
+}
+
+// Sherman-Morrison cyclic Thomas, P == 1 (tdma_cyclic_many_kernel,
+// tdma_local_cuda.cu:67-128); E is [n_row x n_sys] scratch
+__global__ __launch_bounds__(256) void k_gen_thomas_cyclic(const double* __restrict__ A,
+                                                           const double* __restrict__ B,
+                                                           double* __restrict__ C,
+                                                           double* __restrict__ D,
+                                                           double* __restrict__ E, int n_sys,
+                                                           int n_row) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sys) return;
+    const size_t ld = (size_t)n_sys;
+    for (int j = 0; j < n_row; ++j) E[(size_t)j * ld + i] = 0.0;
+    E[ld + i] = -A[ld + i];
+    E[(size_t)(n_row - 1) * ld + i] = -C[(size_t)(n_row - 1) * ld + i];
+    {
+        const double inv_b = 1.0 / B[ld + i];
+        D[ld + i] *= inv_b;
+        E[ld + i] *= inv_b;
+        C[ld + i] *= inv_b;
+    }
+    for (int j = 2; j < n_row; ++j) {
+        const size_t o = (size_t)j * ld + i, om = o - ld;
+        const double aj = A[o];
+        const double r = 1.0 / (B[o] - aj * C[om]);
+        D[o] = r * (D[o] - aj * D[om]);
+        E[o] = r * (E[o] - aj * E[om]);
+        C[o] = r * C[o];
+    }
+    for (int j = n_row - 2; j >= 1; --j) {
+        const size_t o = (size_t)j * ld + i, op = o + ld;
+        const double cj = C[o];
+        D[o] = D[o] - cj * D[op];
+        E[o] = E[o] - cj * E[op];
+    }
+    const size_t oN = (size_t)(n_row - 1) * ld + i;
+    D[i] = (D[i] - A[i] * D[oN] - C[i] * D[ld + i]) / (B[i] + A[i] * E[oN] + C[i] * E[ld + i]);
+    const double x0 = D[i];
+    for (int j = 1; j < n_row; ++j) {
+        const size_t o = (size_t)j * ld + i;
+        D[o] = D[o] + x0 * E[o];
+    }
+}
+
+// PaScaL partition sweep in place (modified_thomas_kernel,
+// tdma_local_cuda.cu:134-203); reduced rows to R[(2p+{0,1})*3 + {a,c,d}][n_sys]
+__global__ __launch_bounds__(256) void k_gen_pascal_sweep(double* __restrict__ A,
+                                                          const double* __restrict__ B,
+                                                          double* __restrict__ C,
+                                                          double* __restrict__ D, int n_sys,
+                                                          int n_row, int P,
+                                                          double* __restrict__ R) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n_sys) return;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    const size_t ld = (size_t)n_sys;
+    double* Ap = A + (size_t)s * ld + i;
+    const double* Bp = B + (size_t)s * ld + i;
+    double* Cp = C + (size_t)s * ld + i;
+    double* Dp = D + (size_t)s * ld + i;
+    for (int j = 0; j < 2; ++j) {
+        const size_t o = (size_t)j * ld;
+        const double b = Bp[o];
+        Ap[o] = Ap[o] / b;
+        Cp[o] = Cp[o] / b;
+        Dp[o] = Dp[o] / b;
+    }
+    double am = Ap[ld], cm = Cp[ld], dm = Dp[ld];
+    for (int j = 2; j < m; ++j) {
+        const size_t o = (size_t)j * ld;
+        const double aj = Ap[o];
+        const double inv = 1.0 / (Bp[o] - aj * cm);
+        dm = inv * (Dp[o] - aj * dm);
+        cm = inv * Cp[o];
+        am = -inv * aj * am;
+        Ap[o] = am;
+        Cp[o] = cm;
+        Dp[o] = dm;
+    }
+    const double aN = am, cN = cm, dN = dm;
+    double an = Ap[(size_t)(m - 2) * ld], cn = Cp[(size_t)(m - 2) * ld], dn = Dp[(size_t)(m - 2) * ld];
+    for (int j = m - 3; j >= 1; --j) {
+        const size_t o = (size_t)j * ld;
+        const double cj = Cp[o];
+        dn = Dp[o] - cj * dn;
+        an = Ap[o] - cj * an;
+        cn = -cj * cn;
+        Ap[o] = an;
+        Cp[o] = cn;
+        Dp[o] = dn;
+    }
+    const double r = 1.0 / (1.0 - an * Cp[0]);
+    const double d0 = r * (Dp[0] - Cp[0] * dn);
+    const double a0 = r * Ap[0];
+    const double c0 = -r * Cp[0] * cn;
+    double* Rp = R + (size_t)(6 * p) * ld + i;
+    Rp[0 * ld] = a0;
+    Rp[1 * ld] = c0;
+    Rp[2 * ld] = d0;
+    Rp[3 * ld] = aN;
+    Rp[4 * ld] = cN;
+    Rp[5 * ld] = dN;
+}
+
+// reduced 2P-row solve per system, B == 1, in R (x left in the d slots);
+// E is [2P x n_sys] scratch for the cyclic case
+__global__ __launch_bounds__(256) void k_gen_pascal_reduced(double* __restrict__ R,
+                                                            double* __restrict__ E, int n_sys,
+                                                            int P, int cyclic) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sys) return;
+    const size_t ld = (size_t)n_sys;
+    const int NR = 2 * P;
+    // reduced row k lives at R[(3k + {0,1,2}) * ld]
+#define RA(k) R[(size_t)(3 * (k) + 0) * ld + i]
+#define RC(k) R[(size_t)(3 * (k) + 1) * ld + i]
+#define RD(k) R[(size_t)(3 * (k) + 2) * ld + i]
+#define RE(k) E[(size_t)(k) * ld + i]
+    if (!cyclic) {
+        for (int k = 1; k < NR; ++k) {
+            const double inv = 1.0 / (1.0 - RA(k) * RC(k - 1));
+            RD(k) = inv * (RD(k) - RA(k) * RD(k - 1));
+            RC(k) = inv * RC(k);
+        }
+        for (int k = NR - 2; k >= 0; --k) RD(k) = RD(k) - RC(k) * RD(k + 1);
+    } else {
+        for (int k = 0; k < NR; ++k) RE(k) = 0.0;
+        RE(1) = -RA(1);
+        RE(NR - 1) = -RC(NR - 1);
+        for (int k = 2; k < NR; ++k) {
+            const double inv = 1.0 / (1.0 - RA(k) * RC(k - 1));
+            RD(k) = inv * (RD(k) - RA(k) * RD(k - 1));
+            RE(k) = inv * (RE(k) - RA(k) * RE(k - 1));
+            RC(k) = inv * RC(k);
+        }
+        for (int k = NR - 2; k >= 1; --k) {
+            RD(k) = RD(k) - RC(k) * RD(k + 1);
+            RE(k) = RE(k) - RC(k) * RE(k + 1);
+        }
+        RD(0) = (RD(0) - RA(0) * RD(NR - 1) - RC(0) * RD(1)) /
+                (1.0 + RA(0) * RE(NR - 1) + RC(0) * RE(1));
+        const double x0 = RD(0);
+        for (int k = 1; k < NR; ++k) RD(k) = RD(k) + x0 * RE(k);
+    }
+#undef RA
+#undef RC
+#undef RD
+#undef RE
+}
+
+// interior update (update_solution_kernel, tdma_local_cuda.cu:205-224)
+__global__ __launch_bounds__(256) void k_gen_pascal_update(const double* __restrict__ A,
+                                                           const double* __restrict__ C,
+                                                           double* __restrict__ D, int n_sys,
+                                                           int n_row, int P,
+                                                           const double* __restrict__ R) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n_sys) return;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    const size_t ld = (size_t)n_sys;
+    const double x0 = R[(size_t)(6 * p + 2) * ld + i];
+    const double xN = R[(size_t)(6 * p + 5) * ld + i];
+    double* Dp = D + (size_t)s * ld + i;
+    const double* Ap = A + (size_t)s * ld + i;
+    const double* Cp = C + (size_t)s * ld + i;
+    Dp[0] = x0;
+    Dp[(size_t)(m - 1) * ld] = xN;
+    for (int j = 1; j < m - 1; ++j) {
+        const size_t o = (size_t)j * ld;
+        Dp[o] = Dp[o] - (Ap[o] * x0 + Cp[o] * xN);
+    }
+}
+
+// Filtered sweeps in place: k_fwd_pass_v{1,2} + k_bwd_pass_v{1,2} with the
+// fused pack (filtered_tdma_kernels.cuh:69-194)
+template <int VERSION>
+__global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__ A,
+                                                            const double* __restrict__ B,
+                                                            double* __restrict__ C,
+                                                            double* __restrict__ D, int n_sys,
+                                                            int n_row, int P,
+                                                            const int* __restrict__ Jp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n_sys) return;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    const size_t ld = (size_t)n_sys;
+    double* Ap = A + (size_t)s * ld + i;
+    const double* Bp = B + (size_t)s * ld + i;
+    double* Cp = C + (size_t)s * ld + i;
+    double* Dp = D + (size_t)s * ld + i;
+    const int J = (VERSION == 2) ? Jp[p] : m;
+    for (int j = 0; j < 2; ++j) {
+        const size_t o = (size_t)j * ld;
+        const double r0 = 1.0 / Bp[o];
+        Ap[o] *= r0;
+        Cp[o] *= r0;
+        Dp[o] *= r0;
+    }
+    double am = Ap[ld], cm = Cp[ld], dm = Dp[ld];
+    int j = 2;
+    for (; j <= J && j < m; ++j) {
+        const size_t o = (size_t)j * ld;
+        const double aj = Ap[o];
+        const double inv = 1.0 / (Bp[o] - aj * cm);
+        dm = inv * (Dp[o] - aj * dm);
+        cm = inv * Cp[o];
+        am = -inv * aj * am;
+        Ap[o] = am;
+        Cp[o] = cm;
+        Dp[o] = dm;
+    }
+    if (VERSION == 2) {
+        for (j = (J + 1 > 1 ? J + 1 : 1); j < m; ++j) {
+            const size_t o = (size_t)j * ld;
+            const double aj = Ap[o];
+            const double inv = 1.0 / (Bp[o] - aj * cm);
+            dm = inv * (Dp[o] - aj * dm);
+            cm = inv * Cp[o];
+            Cp[o] = cm;
+            Dp[o] = dm;
+        }
+    }
+    // backward
+    const size_t onm2 = (size_t)(m - 2) * ld;
+    double dn = Dp[onm2], cn = Cp[onm2];
+    double an = (VERSION == 1) ? Ap[onm2] : (J >= 2 ? Ap[(size_t)J * ld] : 0.0);
+    const int lo = (m - 2) - J;
+    for (j = m - 3; j >= 1; --j) {
+        const size_t o = (size_t)j * ld;
+        const double cj = Cp[o];
+        dn = Dp[o] - cj * dn;
+        Dp[o] = dn;
+        if (VERSION == 1 || (J >= 2 && j < J)) {
+            an = Ap[o] - cj * an;
+            Ap[o] = an;
+        }
+        if (VERSION == 1 || j > lo) {
+            cn = -cj * cn;
+            Cp[o] = cn;
+        }
+    }
+    const double A1 = Ap[ld], C0 = Cp[0], D1 = Dp[ld];
+    const double r = 1.0 / (1.0 - A1 * C0);
+    Dp[0] = r * (Dp[0] - C0 * D1);
+    Ap[0] = r * Ap[0];
+}
+
+// interface 2x2 solves (k_solve_both) -> xbuf rows (2p, 2p+1), plus per-block
+// max of |x0|,|xN| for v1's J
+__global__ __launch_bounds__(256) void k_gen_filtered_iface(const double* __restrict__ A,
+                                                            const double* __restrict__ C,
+                                                            const double* __restrict__ D,
+                                                            int n_sys, int n_row, int P,
+                                                            int cyclic, double* __restrict__ xbuf,
+                                                            double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int p = blockIdx.y;
+    const int i_raw = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i_raw < n_sys;
+    const int i = live ? i_raw : n_sys - 1;
+    const size_t ld = (size_t)n_sys;
+    int s, m, sl, ml, sr, mr;
+    part_of(n_row, P, p, s, m);
+    const int pl = (p + P - 1) % P, pr = (p + 1) % P;
+    part_of(n_row, P, pl, sl, ml);
+    part_of(n_row, P, pr, sr, mr);
+    const double a0 = A[(size_t)s * ld + i], d0 = D[(size_t)s * ld + i];
+    const double cN = C[(size_t)(s + m - 1) * ld + i], dN = D[(size_t)(s + m - 1) * ld + i];
+    double x0 = d0, xN = dN;
+    if (cyclic || p > 0) {
+        const double C_left = C[(size_t)(sl + ml - 1) * ld + i];
+        const double D_left = D[(size_t)(sl + ml - 1) * ld + i];
+        x0 = (d0 - a0 * D_left) / (1.0 - C_left * a0);
+    }
+    if (cyclic || p < P - 1) {
+        const double A_right = A[(size_t)sr * ld + i];
+        const double D0_right = D[(size_t)sr * ld + i];
+        xN = (dN - cN * D0_right) / (1.0 - A_right * cN);
+    }
+    if (live) {
+        xbuf[(size_t)(2 * p) * ld + i] = x0;
+        xbuf[(size_t)(2 * p + 1) * ld + i] = xN;
+    }
+    double v = group_max(fmax(fabs(x0), fabs(xN)), 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[(size_t)blockIdx.x * P + p] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// truncated final correction (k_final_pass, filtered_tdma_kernels.cuh:196-218)
+__global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __restrict__ A,
+                                                            const double* __restrict__ C,
+                                                            double* __restrict__ D, int n_sys,
+                                                            int n_row, int P,
+                                                            const int* __restrict__ Jp,
+                                                            const double* __restrict__ xbuf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n_sys) return;
+    int s, m;
+    part_of(n_row, P, p, s, m);
+    const size_t ld = (size_t)n_sys;
+    const double x0 = xbuf[(size_t)(2 * p) * ld + i];
+    const double xN = xbuf[(size_t)(2 * p + 1) * ld + i];
+    double* Dp = D + (size_t)s * ld + i;
+    const double* Ap = A + (size_t)s * ld + i;
+    const double* Cp = C + (size_t)s * ld + i;
+    Dp[0] = x0;
+    Dp[(size_t)(m - 1) * ld] = xN;
+    const int J = Jp[p];
+    const int jL = J < m - 2 ? J : m - 2;
+    const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+    for (int j = 1; j <= jL; ++j) {
+        const size_t o = (size_t)j * ld;
+        Dp[o] = Dp[o] - Ap[o] * x0;
+    }
+    for (int j = jR; j < m - 1; ++j) {
+        const size_t o = (size_t)j * ld;
+        Dp[o] = Dp[o] - Cp[o] * xN;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline int log2i(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+hipError_t launch_set_rho(const double* A, const double* B, const double* C, int n_sys, int n_row,
+                          double* rho, hipStream_t st) {
+    const int n = n_row + 1;
+    hipLaunchKernelGGL(k_set_rho, dim3((n + 255) / 256), dim3(256), 0, st, A, B, C, n_sys, n_row,
+                       rho, rho + n);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_maxabs(const double* D, int n_sys, int n_row, int P, double* partial,
+                                  int* n_tiles, hipStream_t st) {
+    const int nt = (n_sys + 255) / 256;
+    *n_tiles = nt;
+    hipLaunchKernelGGL(k_boundary_maxabs, dim3(nt, P), dim3(256), 0, st, D, n_sys, n_row, P,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_J(const double* partial, int n_tiles, int P, int n_row, const double* rho,
+                            double eps_value, int eps_direct, int* J, hipStream_t st) {
+    hipLaunchKernelGGL(k_compute_J, dim3(P), dim3(256), 0, st, partial, n_tiles, P, n_row, rho,
+                       rho + (n_row + 1), eps_value, eps_direct, J);
+    return hipGetLastError();
+}
+
+// -------- fused PaScaL dispatch over the compiled (M, MAXT) variants --------
+
+#define FTD_PASCAL_VARIANTS(X) \
+    X(8, 1024)                  \
+    X(16, 512)                  \
+    X(16, 1024)                 \
+    X(32, 256)                  \
+    X(32, 512)                  \
+    X(64, 256)
+
+int pascal_fused_supported(int M, int maxt) {
+#define X(m_, t_) \
+    if (M == m_ && maxt == t_) return 1;
+    FTD_PASCAL_VARIANTS(X)
+#undef X
+    return 0;
+}
+
+size_t pascal_fused_lds(int Q, int S, int cyclic) {
+    return sizeof(double) * (size_t)(cyclic ? 4 : 3) * (size_t)(2 * Q) * (size_t)S;
+}
+
+hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B, const double* C,
+                               double* D, int n_sys, int n_row, int Q, int S, int cyclic,
+                               hipStream_t st) {
+    const int lgS = log2i(S);
+    const dim3 grid((n_sys + S - 1) / S), block(S * Q);
+    const size_t lds = pascal_fused_lds(Q, S, cyclic);
+#define X(m_, t_)                                                                                \
+    if (M == m_ && maxt == t_) {                                                                 \
+        if (cyclic)                                                                              \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, true>), grid, block, lds, st, A, B, C, D, \
+                               n_sys, n_row, Q, lgS);                                            \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false>), grid, block, lds, st, A, B, C,   \
+                               D, n_sys, n_row, Q, lgS);                                         \
+        return hipGetLastError();                                                                \
+    }
+    FTD_PASCAL_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// -------- fused Filtered dispatch --------
+
+#define FTD_FILTERED_VARIANTS(X) \
+    X(8, 1024)                    \
+    X(16, 512)                    \
+    X(32, 256)                    \
+    X(64, 256)
+
+int filtered_fused_supported(int M, int maxt) {
+#define X(m_, t_) \
+    if (M == m_ && maxt == t_) return 1;
+    FTD_FILTERED_VARIANTS(X)
+#undef X
+    return 0;
+}
+
+size_t filtered_fused_lds(int P, int S) { return sizeof(double) * 4 * (size_t)P * (size_t)S; }
+
+hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
+                                 const double* B, const double* C, double* D, int n_sys, int n_row,
+                                 int P, int S, int cyclic, const int* Jp, double* xbuf,
+                                 double* partial, hipStream_t st) {
+    const int lgS = log2i(S);
+    const dim3 grid((n_sys + S - 1) / S), block(S * P);
+    const size_t lds = (pass == 2) ? 0 : filtered_fused_lds(P, S);
+#define LAUNCH(m_, t_, v_, p_)                                                                   \
+    hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_>), grid, block, lds, st, A, B, C, D,     \
+                       n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial)
+#define X(m_, t_)                                         \
+    if (M == m_ && maxt == t_) {                          \
+        if (version == 2)                                 \
+            LAUNCH(m_, t_, 2, 0);                         \
+        else if (pass == 1)                               \
+            LAUNCH(m_, t_, 1, 1);                         \
+        else                                              \
+            LAUNCH(m_, t_, 1, 2);                         \
+        return hipGetLastError();                         \
+    }
+    FTD_FILTERED_VARIANTS(X)
+#undef X
+#undef LAUNCH
+    return hipErrorInvalidValue;
+}
+
+// -------- general kernels --------
+
+hipError_t launch_gen_thomas(const double* A, const double* B, double* C, double* D, int n_sys,
+                             int n_row, hipStream_t st) {
+    hipLaunchKernelGGL(k_gen_thomas, dim3((n_sys + 255) / 256), dim3(256), 0, st, A, B, C, D,
+                       n_sys, n_row);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_thomas_cyclic(const double* A, const double* B, double* C, double* D,
+                                    double* E, int n_sys, int n_row, hipStream_t st) {
+    hipLaunchKernelGGL(k_gen_thomas_cyclic, dim3((n_sys + 255) / 256), dim3(256), 0, st, A, B, C,
+                       D, E, n_sys, n_row);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_pascal(double* A, const double* B, double* C, double* D, int n_sys,
+                             int n_row, int P, int cyclic, double* R, double* E, hipStream_t st) {
+    const dim3 g2((n_sys + 255) / 256, P), g1((n_sys + 255) / 256);
+    hipLaunchKernelGGL(k_gen_pascal_sweep, g2, dim3(256), 0, st, A, B, C, D, n_sys, n_row, P, R);
+    hipLaunchKernelGGL(k_gen_pascal_reduced, g1, dim3(256), 0, st, R, E, n_sys, P, cyclic);
+    hipLaunchKernelGGL(k_gen_pascal_update, g2, dim3(256), 0, st, A, C, D, n_sys, n_row, P, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_filtered_sweep(int version, double* A, const double* B, double* C, double* D,
+                                     int n_sys, int n_row, int P, const int* Jp, hipStream_t st) {
+    const dim3 g2((n_sys + 255) / 256, P);
+    if (version == 2)
+        hipLaunchKernelGGL(k_gen_filtered_sweep<2>, g2, dim3(256), 0, st, A, B, C, D, n_sys, n_row,
+                           P, Jp);
+    else
+        hipLaunchKernelGGL(k_gen_filtered_sweep<1>, g2, dim3(256), 0, st, A, B, C, D, n_sys, n_row,
+                           P, Jp);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_filtered_iface(const double* A, const double* C, const double* D, int n_sys,
+                                     int n_row, int P, int cyclic, double* xbuf, double* partial,
+                                     int* n_tiles, hipStream_t st) {
+    const int nt = (n_sys + 255) / 256;
+    *n_tiles = nt;
+    hipLaunchKernelGGL(k_gen_filtered_iface, dim3(nt, P), dim3(256), 0, st, A, C, D, n_sys, n_row,
+                       P, cyclic, xbuf, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_filtered_final(const double* A, const double* C, double* D, int n_sys,
+                                     int n_row, int P, const int* Jp, const double* xbuf,
+                                     hipStream_t st) {
+    const dim3 g2((n_sys + 255) / 256, P);
+    hipLaunchKernelGGL(k_gen_filtered_final, g2, dim3(256), 0, st, A, C, D, n_sys, n_row, P, Jp,
+                       xbuf);
+    return hipGetLastError();
+}
+
+}  // namespace ftd

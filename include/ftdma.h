@@ -1,0 +1,136 @@
+/*
+ * ftdma.h -- C ABI of the MI355X (gfx950) batched tridiagonal engine.
+ *
+ * Drop-in for the solve(A,B,C,D,n_sys,n_row) path behind the reference's
+ * runtime backend switch, class TdmaBackendGPU
+ * (apps/heat_gpu/tdma_backend_gpu.hpp:21-70, tdma_backend_gpu.cu:6-66).
+ * Each entry point below names the reference member it replaces.
+ *
+ * Data contract (same as the reference):
+ *   - A, B, C, D are device pointers to row-major [n_row x n_sys] float64
+ *     arrays: row j of system i at j*n_sys + i;
+ *   - D is overwritten with the solution; A and C may be clobbered; B is
+ *     only read;
+ *   - A[row 0] and C[row n_row-1] are not read by the non-periodic solves;
+ *     the periodic solves read them as the wrap-around couplings.
+ *
+ * One MI355X replaces the reference's MPI z-slab split: `partitions` (the
+ * reference's nprocs) splits every system's rows into P contiguous blocks
+ * with para_range (libs/pascal_tdma/cpu/para_range.cpp:4-9). Filtered
+ * results depend on P exactly as the reference's do on nprocs; PaScaL is
+ * exact for any P, which then only affects rounding.
+ *
+ * Every call is ordered on the handle's stream and returns without a host
+ * synchronisation, except ftd_last_J and solves with profiling enabled.
+ * Calls on one handle must not overlap (the handle owns scratch memory).
+ */
+#ifndef FTDMA_H
+#define FTDMA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ftd_handle ftd_handle;
+
+typedef enum {
+    FTD_OK = 0,
+    FTD_ERR_INVALID = 1,     /* bad argument (null pointer, bad size, bad kind) */
+    FTD_ERR_UNSUPPORTED = 2, /* a shape the engine cannot solve (e.g. < 3 rows per partition) */
+    FTD_ERR_HIP = 3,         /* HIP runtime error (message in ftd_last_error_string) */
+    FTD_ERR_ALLOC = 4        /* device allocation failed */
+} ftd_status;
+
+/* TdmaBackendGPU::Kind, same order (tdma_backend_gpu.hpp:23). */
+enum { FTD_KIND_FILTERED = 0, FTD_KIND_FILTERED_V2 = 1, FTD_KIND_PASCAL = 2 };
+
+/* TdmaBackendGPU::parse (tdma_backend_gpu.cu:6-13): case-insensitive;
+ * "pascal" | "pascal_tdma" -> PASCAL, "filtered_v2" -> FILTERED_V2, any other
+ * string (including NULL) -> FILTERED. */
+int ftd_kind_from_string(const char* s);
+
+/* TdmaBackendGPU::TdmaBackendGPU(kind, n_sys, n_row, myrank, nprocs, comm,
+ * left_rank, right_rank, eps_constant) (tdma_backend_gpu.cu:15-33), with the
+ * MPI arguments replaced by `partitions`; n_row is the full row count of a
+ * system (the sum of the reference's per-rank n_row). eps_ of partition p is
+ * eps_constant / (m_p * partitions)^2 (filtered_tdma_cuda.cu:52-53).
+ * `stream` is a hipStream_t (NULL = the null stream). */
+ftd_status ftd_create(ftd_handle** out, int kind, int n_sys, int n_row, int partitions,
+                      double eps_constant, void* stream);
+
+/* TdmaBackendGPU::~TdmaBackendGPU: frees the handle's device scratch. */
+ftd_status ftd_destroy(ftd_handle* h);
+
+/* TdmaBackendGPU::solve (tdma_backend_gpu.cu:35-46):
+ * FILTERED -> solve_filtered_v1, FILTERED_V2 -> solve_filtered_v2,
+ * PASCAL -> PaScaLTDMACUDA::solve; partitions == 1 -> plain Thomas. */
+ftd_status ftd_solve(ftd_handle* h, double* A, double* B, double* C, double* D);
+
+/* PaScaLTDMACUDA::solve(A, B, C, D, n_sys, n_row) (pascal_tdma_cuda.hpp:28-29):
+ * dimensions given per call. PASCAL accepts any n_sys, n_row; the filtered
+ * kinds require the dimensions the handle was created with. */
+ftd_status ftd_solve_nr(ftd_handle* h, double* A, double* B, double* C, double* D, int n_sys,
+                        int n_row);
+
+/* TdmaBackendGPU::solve_cyclic (tdma_backend_gpu.cu:48-56): periodic systems,
+ * row 0 coupled to row n_row-1 through A[0] and C[n_row-1]. */
+ftd_status ftd_solve_cyclic(ftd_handle* h, double* A, double* B, double* C, double* D);
+
+/* TdmaBackendGPU::set_rho_device (tdma_backend_gpu.cu:58-62,
+ * filtered_tdma_kernels.cuh:46-67): |A/B| and |C/B| of system 0 per row,
+ * used by the filtered J estimator. No-op for PASCAL. Until it is called the
+ * bounds are 0, which selects full propagation (J = n_row_p - 1). */
+ftd_status ftd_set_rho(ftd_handle* h, const double* A, const double* B, const double* C);
+
+/* TdmaBackendGPU::set_eps_constant (tdma_backend_gpu.cu:64-66). No-op for PASCAL. */
+ftd_status ftd_set_eps_constant(ftd_handle* h, double eps_constant);
+
+/* Extension without a reference counterpart: set eps_ itself (the same value
+ * for every partition) instead of eps_constant / (m_p P)^2. */
+ftd_status ftd_set_eps(ftd_handle* h, double eps);
+
+/* Extension: change the stream later calls are ordered on. */
+ftd_status ftd_set_stream(ftd_handle* h, void* stream);
+
+/* Extension: when enabled, every solve is bracketed by HIP events and waits
+ * for them, so that ftd_last_gpu_ms reports its device time -- what the
+ * reference's *_profile variants do on every TdmaBackendGPU::solve
+ * (tdma_backend_gpu.cu:35-46). Off by default (no host synchronisation). */
+ftd_status ftd_set_profiling(ftd_handle* h, int enable);
+
+/* FilteredTDMACUDA::last_J (filtered_tdma_cuda.cu:83-87), one value per
+ * partition: J of the most recent filtered solve (blocking device->host
+ * copy of n <= partitions ints). -1 where no filtered solve ran yet. */
+ftd_status ftd_last_J(ftd_handle* h, int* J, int n);
+
+/* TdmaBackendGPU::last_gpu_ms (tdma_backend_gpu.hpp:60-63): device time of
+ * the most recent solve in ms when profiling is on, else 0. */
+double ftd_last_gpu_ms(const ftd_handle* h);
+
+/* TdmaBackendGPU::last_comm_ms (tdma_backend_gpu.hpp:54-57): always 0 --
+ * the partitions of one GPU exchange nothing over a network. */
+double ftd_last_comm_ms(const ftd_handle* h);
+
+/* Extension (autotuner hook): force the fused-kernel shape of later solves.
+ * rows_per_lane in {8,16,32,64} (0 = automatic), max_threads in
+ * {256,512,1024}, systems_per_tile a power of two <= 64 (0 = automatic),
+ * lanes_per_system (PaScaL sub-partitions, 0 = automatic). rows_per_lane
+ * == -1 selects the general kernels (partitions walked in global memory,
+ * in place) for every later solve. Returns
+ * FTD_ERR_UNSUPPORTED for combinations that are not compiled. */
+ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
+                          int systems_per_tile, int lanes_per_system);
+
+/* Extension (diagnostics): the path the last solve took, as a short
+ * static string, e.g. "pascal_fused M=32 T=256 S=16 Q=16" or
+ * "filtered_general". */
+const char* ftd_last_path(const ftd_handle* h);
+
+/* Text of the last error reported by any ftd_* call in this thread. */
+const char* ftd_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTDMA_H */

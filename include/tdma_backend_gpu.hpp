@@ -1,0 +1,76 @@
+// tdma_backend_gpu.hpp -- header-only C++ mirror of the reference's
+// TdmaBackendGPU (apps/heat_gpu/tdma_backend_gpu.hpp:21-70) over the C ABI
+// of libftdma.so (ftdma.h). Same member names and meaning; the MPI
+// constructor arguments (myrank, nprocs, comm, left_rank, right_rank) are
+// replaced by the number of partitions each system's rows are split into on
+// one GPU. Errors throw std::runtime_error (the reference exits).
+//
+//   TdmaBackendGPU be(TdmaBackendGPU::parse("filtered_v2"), n_sys, n_row,
+//                     /*partitions=*/8, /*eps_constant=*/dt);
+//   be.set_eps_constant(dt);
+//   be.set_rho_device(dA, dB, dC);
+//   be.solve(dA, dB, dC, dD);
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ftdma.h"
+
+class TdmaBackendGPU {
+public:
+    enum class Kind { FILTERED = FTD_KIND_FILTERED, FILTERED_V2 = FTD_KIND_FILTERED_V2,
+                      PASCAL = FTD_KIND_PASCAL };
+
+    /// Case-insensitive: "pascal" | "pascal_tdma" -> PASCAL, "filtered_v2" ->
+    /// FILTERED_V2, anything else -> FILTERED (tdma_backend_gpu.cu:6-13).
+    static Kind parse(const std::string& s) {
+        return static_cast<Kind>(ftd_kind_from_string(s.c_str()));
+    }
+
+    TdmaBackendGPU(Kind kind, int n_sys, int n_row, int partitions, double eps_constant,
+                   void* stream = nullptr)
+        : kind_(kind), partitions_(partitions) {
+        check(ftd_create(&h_, static_cast<int>(kind), n_sys, n_row, partitions, eps_constant,
+                         stream));
+    }
+    ~TdmaBackendGPU() { ftd_destroy(h_); }
+    TdmaBackendGPU(const TdmaBackendGPU&) = delete;
+    TdmaBackendGPU& operator=(const TdmaBackendGPU&) = delete;
+
+    void solve(double* d_A, double* d_B, double* d_C, double* d_D) {
+        check(ftd_solve(h_, d_A, d_B, d_C, d_D));
+    }
+    void solve(double* d_A, double* d_B, double* d_C, double* d_D, int n_sys, int n_row) {
+        check(ftd_solve_nr(h_, d_A, d_B, d_C, d_D, n_sys, n_row));
+    }
+    void solve_cyclic(double* d_A, double* d_B, double* d_C, double* d_D) {
+        check(ftd_solve_cyclic(h_, d_A, d_B, d_C, d_D));
+    }
+    void set_rho_device(const double* d_A, const double* d_B, const double* d_C) {
+        check(ftd_set_rho(h_, d_A, d_B, d_C));
+    }
+    void set_eps_constant(double eps_constant) { check(ftd_set_eps_constant(h_, eps_constant)); }
+
+    Kind kind() const { return kind_; }
+    double last_comm_ms() const { return ftd_last_comm_ms(h_); }
+    double last_gpu_ms() const { return ftd_last_gpu_ms(h_); }
+
+    /// FilteredTDMACUDA::last_J, one value per partition (blocking).
+    std::vector<int> last_J() const {
+        std::vector<int> J(partitions_);
+        check(ftd_last_J(h_, J.data(), partitions_));
+        return J;
+    }
+
+    ftd_handle* handle() { return h_; }
+
+private:
+    static void check(ftd_status s) {
+        if (s != FTD_OK) throw std::runtime_error(ftd_last_error_string());
+    }
+    Kind kind_;
+    int partitions_;
+    ftd_handle* h_ = nullptr;
+};
