@@ -68,6 +68,15 @@ int ftdo_filtered(double* A, const double* B, double* C, double* D,
                   double eps_value, int eps_direct,
                   const int* J_force, int* J_out, int nthreads);
 
+/* Heat ADI harness (apps/heat_cpu on a parts[0] x parts[1] x parts[2] process
+ * grid): nx, ny, nz interior cells, kind 0/1/2 = filtered / filtered_v2 /
+ * pascal (a direction with one part is plain Thomas), periodic[3] flags, Nt
+ * steps of dt. theta_out (optional) receives the (nz+2)(ny+2)(nx+2) field
+ * with ghosts, l2_out (optional) the L2 error of main.cpp:59-81. */
+int ftdo_heat_run(int nx, int ny, int nz, const int* parts, int kind, int sem,
+                  const int* periodic, double dt, int Nt, double* theta_out, double* l2_out,
+                  int nthreads);
+
 /* Error text of the last failing call in this thread. */
 const char* ftdo_last_error(void);
 

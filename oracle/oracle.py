@@ -177,3 +177,47 @@ def ref_run(mode, kind, A, B, C, D, P, cyclic=False, do_rho=True, eps_constant=0
         return np.fromfile(fout, dtype="<f8").reshape(n_row, n_sys)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+# ------------------------------------------------------------- heat harness
+
+def heat_dt(option, rho, n):
+    """dt policy of GlobalParams::load (global.cpp:80-97) for n interior
+    cells: (dt, Nt) for option 'order'; dt only for 'strong'."""
+    dx = 2.0 / n
+    dt = rho / (1.0 - 2.0 * rho) * (2.0 * dx * dx)
+    if option == "order":
+        dt_n = rho / (1.0 - 2.0 * rho) * (2.0 * (2.0 / 512.0) * (2.0 / 512.0))
+        tmax = dt_n * 128.0
+        import math
+        q = tmax / dt
+        nt = int(math.floor(q + 0.5)) if q >= 0 else -int(math.floor(-q + 0.5))
+        return dt, nt
+    return dt, None
+
+
+def heat_run(nx, ny, nz, parts, kind, dt, Nt, periodic=(0, 0, 0), sem=SEM_CPU, nthreads=0,
+             return_theta=False):
+    """Run the heat ADI restatement on a parts = (px, py, pz) grid (an int
+    means (1, 1, pz)); returns L2 (and the field with ghosts)."""
+    L = lib()
+    if isinstance(parts, int):
+        parts = (1, 1, parts)
+    if not hasattr(L, "_heat_ready"):
+        L.ftdo_heat_run.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int),
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.ftdo_heat_run.restype = ctypes.c_int
+        L._heat_ready = True
+    kinds = {"filtered": 0, "filtered_v2": 1, "pascal": 2}
+    per = (ctypes.c_int * 3)(*periodic)
+    th = np.zeros((nz + 2, ny + 2, nx + 2)) if return_theta else None
+    l2 = ctypes.c_double()
+    pp = (ctypes.c_int * 3)(*parts)
+    rc = L.ftdo_heat_run(nx, ny, nz, pp, kinds[kind], sem, per, float(dt), int(Nt),
+                         _p(th) if th is not None else None, ctypes.byref(l2), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"heat_run failed ({rc}): " + L.ftdo_last_error().decode())
+    return (l2.value, th) if return_theta else l2.value

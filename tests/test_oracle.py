@@ -255,3 +255,59 @@ def test_filtered_gpu_vs_cpu_semantics_same_when_J_full():
     xg = O.filtered(A, B, C, D, 3, 1, 1e-4, sem=O.SEM_GPU)
     xc = O.filtered(A, B, C, D, 3, 1, 1e-4, sem=O.SEM_CPU)
     np.testing.assert_array_equal(xg, xc)
+
+
+# ----------------------------------------------------------- heat harness
+
+HEAT_GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "heat_l2.json")
+
+
+def _heat_cases():
+    if not os.path.exists(HEAT_GOLDEN):
+        return []
+    with open(HEAT_GOLDEN) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _heat_cases(),
+                         ids=lambda c: f"{c['kind']}-n{c['n']}-r{c['rho']}-"
+                                       f"{'x'.join(map(str, c['parts']))}-per{c['periodic']}")
+def test_heat_oracle_matches_reference_heat_cpu(case):
+    """L2 printed by the reference heat_cpu (6 significant digits) for the
+    same input; the restatement must print the same text."""
+    dt, nt = O.heat_dt("order", case["rho"], case["n"])
+    per = (1, 1, 1) if case["periodic"] else (0, 0, 0)
+    l2 = O.heat_run(case["n"], case["n"], case["n"], tuple(case["parts"]), case["kind"], dt, nt,
+                    periodic=per, sem=O.SEM_CPU)
+    assert "%g" % l2 == case["l2"]
+
+
+@pytest.mark.parametrize("kind,parts,want", [
+    # apps/heat_cpu/results/order_accuracy/order_cpu_20260628_201654.txt:10-34
+    ("pascal", 4, "0.00153948"), ("filtered", 4, "0.0015409"), ("filtered_v2", 2, "0.00153948"),
+])
+def test_heat_oracle_held_32cube(kind, parts, want):
+    dt, nt = O.heat_dt("order", 0.25, 32)
+    assert nt == 1
+    assert "%g" % O.heat_run(32, 32, 32, parts, kind, dt, nt) == want
+
+
+def test_heat_oracle_held_periodic_and_gpu_np2():
+    """README.md:279-281 (periodic, pascal) and Report/progress_2026-06-24.md:
+    63-67 (periodic, z-split np=2 on the GPU path: 4.39367e-5 / 4.39369e-5)."""
+    dt, nt = O.heat_dt("order", 0.25, 64)
+    assert "%.5e" % O.heat_run(64, 64, 64, 1, "pascal", dt, nt, periodic=(1, 1, 1)) == "4.39360e-05"
+    assert "%g" % O.heat_run(64, 64, 64, 2, "filtered", dt, nt, periodic=(1, 1, 1),
+                             sem=O.SEM_GPU) == "4.39367e-05"
+    assert "%g" % O.heat_run(64, 64, 64, 2, "filtered_v2", dt, nt, periodic=(1, 1, 1),
+                             sem=O.SEM_GPU) == "4.39369e-05"
+
+
+def test_heat_second_order_convergence():
+    """Halving h at a fixed final time divides the L2 error by ~4 (periodic,
+    pascal; option=order keeps T = 128 dt_512 for N = 64, 128)."""
+    e = []
+    for n in (64, 128):
+        dt, nt = O.heat_dt("order", 0.25, n)
+        e.append(O.heat_run(n, n, n, 1, "pascal", dt, nt, periodic=(1, 1, 1)))
+    assert 3.9 < e[0] / e[1] < 4.1
