@@ -17,6 +17,8 @@ EXPORTS = (
     "ftd_solve_cyclic", "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
     "ftd_set_profiling", "ftd_last_J", "ftd_last_gpu_ms", "ftd_last_comm_ms", "ftd_set_tuning",
     "ftd_last_path", "ftd_last_error_string",
+    "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
+    "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string",
 )
 
 _lib = None
@@ -60,6 +62,20 @@ def load() -> ctypes.CDLL:
     L.ftd_last_path.argtypes = [vp]
     L.ftd_last_path.restype = ctypes.c_char_p
     L.ftd_last_error_string.restype = ctypes.c_char_p
+    ip3 = ctypes.POINTER(ctypes.c_int)
+    L.ftd_heat_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, ip3, c_int, ip3,
+                                  c_double, vp]
+    L.ftd_heat_destroy.argtypes = [vp]
+    L.ftd_heat_init.argtypes = [vp]
+    L.ftd_heat_step.argtypes = [vp, c_int, c_int]
+    L.ftd_heat_field.argtypes = [vp, vp]
+    L.ftd_heat_l2.argtypes = [vp, ctypes.POINTER(c_double)]
+    L.ftd_heat_last_step_ms.argtypes = [vp]
+    L.ftd_heat_last_step_ms.restype = c_double
+    L.ftd_heat_last_error_string.restype = ctypes.c_char_p
+    for name in ("ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step",
+                 "ftd_heat_field", "ftd_heat_l2"):
+        getattr(L, name).restype = c_int
     for name in ("ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr", "ftd_solve_cyclic",
                  "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
                  "ftd_set_profiling", "ftd_last_J", "ftd_set_tuning"):
@@ -71,3 +87,8 @@ def load() -> ctypes.CDLL:
 def check(status: int) -> None:
     if status != 0:
         raise FtdError(status, load().ftd_last_error_string().decode())
+
+
+def check_heat(status: int) -> None:
+    if status != 0:
+        raise FtdError(status, load().ftd_heat_last_error_string().decode())

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libftdma.so")
-SOURCES = [os.path.join(CSRC, "ftdma_kernels.hip"), os.path.join(CSRC, "ftdma_api.cpp")]
+SOURCES = [os.path.join(CSRC, "ftdma_kernels.hip"), os.path.join(CSRC, "ftdma_api.cpp"),
+           os.path.join(CSRC, "ftdma_heat.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -36,7 +37,10 @@ FLAGS = [
 
 
 def build(resource_usage: bool = False, verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, "-x", "hip", SOURCES[0], "-x", "hip", SOURCES[1], "-o", LIB + ".tmp"]
+    srcs = []
+    for src in SOURCES:
+        srcs += ["-x", "hip", src]
+    cmd = [HIPCC, *FLAGS, *srcs, "-o", LIB + ".tmp"]
     if resource_usage:
         cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
     if verbose:

@@ -110,14 +110,14 @@ struct FusedShape {
 // (sub-partitions of <= M rows, >= 3 rows each), S systems per workgroup.
 bool pick_pascal(const ftd_handle* h, int n_row, int cyclic, FusedShape& out) {
     static const int Ms[] = {32, 16, 64, 8};
-    static const int Ts[] = {256, 512, 256, 1024};
+    static const int Ts[] = {256, 512, 256, 512};
     const int ncand = h->t_M ? 1 : 4;
     for (int k = 0; k < ncand; ++k) {
         const int M = h->t_M ? h->t_M : Ms[k];
         const int T = h->t_M ? (h->t_T ? h->t_T : 256) : Ts[k];
         if (!ftd::pascal_fused_supported(M, T)) continue;
         int Q = h->t_Q ? h->t_Q : (n_row + M - 1) / M;
-        if (cyclic && Q < 2) Q = 2;
+        if (Q < 2) Q = 2;  // one partition is the plain Thomas kernel
         if (Q < 1 || Q > T) continue;
         int s_, m_min, m_max;
         part_of(n_row, Q, Q - 1, s_, m_min);
@@ -140,6 +140,12 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
     hipStream_t st = h->stream;
     if (cyclic && n_row < 3) return fail(FTD_ERR_UNSUPPORTED, "periodic solve needs n_row >= 3");
     FusedShape fs;
+    if (!h->force_general && !cyclic && !h->t_M && n_row >= 3 && ftd::thomas_fused_rows(n_row)) {
+        // short systems: plain Thomas in registers, the reference's P == 1 path
+        std::snprintf(h->path, sizeof h->path, "thomas_fused M=%d", ftd::thomas_fused_rows(n_row));
+        FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, st));
+        return FTD_OK;
+    }
     if (!h->force_general && n_row >= 3 && pick_pascal(h, n_row, cyclic, fs)) {
         std::snprintf(h->path, sizeof h->path, "pascal_fused M=%d T=%d S=%d Q=%d", fs.M, fs.T,
                       fs.S, fs.Q);
@@ -188,7 +194,7 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     FusedShape fs;
     if (!h->force_general) {
         static const int Ms[] = {8, 16, 32, 64};
-        static const int Ts[] = {1024, 512, 256, 256};
+        static const int Ts[] = {512, 512, 256, 256};
         for (int k = 0; k < 4; ++k) {
             if (h->t_M && Ms[k] != h->t_M) continue;
             if (Ms[k] < mmax) continue;

@@ -1,0 +1,486 @@
+// ftdma_heat.hip -- the 3D heat ADI harness on one MI355X: the caller of the
+// tridiagonal path in the reference (apps/heat_gpu/solve_theta.cu:391-650,
+// SolveTheta::profile; apps/heat_cpu/solve_theta.cpp:18-334), with the
+// ghost-cell Dirichlet / periodic boundaries of MPISubdomain
+// (apps/heat_cpu/mpi_subdomain.cpp:90-236) and the L2 error of main.cpp:59-81.
+//
+// One address space holds the whole (nx+2)(ny+2)(nz+2) field with one ghost
+// layer per face; the reference's (px x py x pz) ranks become partitions of
+// the x, y and z sweeps (each sweep is one batched solve of this library).
+// Per step (solve_theta.cpp:97-334):
+//   Dirichlet ghosts -> explicit RHS -> Z: build, solve, copy -> Y: build,
+//   solve, copy -> X: build, solve, copy into theta -> periodic ghosts.
+// Transcendental factors (cos/sin of the cell centres) are tabulated on the
+// host with the C library, so the device arithmetic is exactly the CPU
+// restatement's; the whole step can be captured once into a hipGraph and
+// replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ftdma.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Dims {
+    int nx, ny, nz, nx1, ny1;
+};
+
+__device__ __forceinline__ size_t cell(const Dims& g, int i, int j, int k) {
+    return ((size_t)k * g.ny1 + j) * g.nx1 + i;
+}
+
+// u_ghost = 2 u_BC - u_adjacent, u_BC = -cos(pi a) cos(pi b) on each face
+// (mpi_subdomain.cpp:180-236); face f: 0 = x, 1 = y, 2 = z
+__global__ void k_dirichlet(double* __restrict__ th, Dims g, int face,
+                            const double* __restrict__ cx, const double* __restrict__ cy,
+                            const double* __restrict__ cz) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (face == 0) {
+        if (u > g.ny || v > g.nz) return;
+        const double ubc = -cy[u] * cz[v];
+        th[cell(g, 0, u, v)] = 2.0 * ubc - th[cell(g, 1, u, v)];
+        th[cell(g, g.nx + 1, u, v)] = 2.0 * ubc - th[cell(g, g.nx, u, v)];
+    } else if (face == 1) {
+        if (u > g.nx || v > g.nz) return;
+        const double ubc = -cx[u] * cz[v];
+        th[cell(g, u, 0, v)] = 2.0 * ubc - th[cell(g, u, 1, v)];
+        th[cell(g, u, g.ny + 1, v)] = 2.0 * ubc - th[cell(g, u, g.ny, v)];
+    } else {
+        if (u > g.nx || v > g.ny) return;
+        const double ubc = -cx[u] * cy[v];
+        th[cell(g, u, v, 0)] = 2.0 * ubc - th[cell(g, u, v, 1)];
+        th[cell(g, u, v, g.nz + 1)] = 2.0 * ubc - th[cell(g, u, v, g.nz)];
+    }
+}
+
+// periodic wrap of one face pair over the whole plane (mpi_subdomain.cpp:90-111)
+__global__ void k_periodic(double* __restrict__ th, Dims g, int face) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nz1 = g.nz + 2;
+    if (face == 0) {
+        if (u >= g.ny1 || v >= nz1) return;
+        th[cell(g, 0, u, v)] = th[cell(g, g.nx, u, v)];
+        th[cell(g, g.nx + 1, u, v)] = th[cell(g, 1, u, v)];
+    } else if (face == 1) {
+        if (u >= g.nx1 || v >= nz1) return;
+        th[cell(g, u, 0, v)] = th[cell(g, u, g.ny, v)];
+        th[cell(g, u, g.ny + 1, v)] = th[cell(g, u, 1, v)];
+    } else {
+        if (u >= g.nx1 || v >= g.ny1) return;
+        th[cell(g, u, v, 0)] = th[cell(g, u, v, g.nz)];
+        th[cell(g, u, v, g.nz + 1)] = th[cell(g, u, v, 1)];
+    }
+}
+
+struct Stencil {
+    double xa, xb, xc, ya, yb, yc, za, zb, zc, c0;
+};
+
+// explicit RHS of the factored Crank-Nicolson step (solve_theta.cpp:114-134)
+__global__ void k_rhs(const double* __restrict__ th, double* __restrict__ rhs, Dims g, Stencil s,
+                      const double* __restrict__ cx, const double* __restrict__ cy,
+                      const double* __restrict__ cz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const size_t c = cell(g, i, j, k);
+    const size_t sx = 1, sy = (size_t)g.nx1, sz = (size_t)g.nx1 * g.ny1;
+    rhs[c] = (s.xc * th[c + sx] + s.xa * th[c - sx]) + (s.yc * th[c + sy] + s.ya * th[c - sy]) +
+             (s.zc * th[c + sz] + s.za * th[c - sz]) + (1.0 + s.xb + s.yb + s.zb) * th[c] +
+             s.c0 * cx[i] * cy[j] * cz[k];
+}
+
+// Coefficients of one sweep (solve_theta.cpp:146-186 for Z, 212-247 Y,
+// 274-309 X): a = -s.a, b = 1 - s.b, c = -s.c, d = rhs; on a Dirichlet wall
+// row d += s * (ghost + adjacent), b += s. Sweep dir: 0 = x (row ii, system
+// kk*ny + jj), 1 = y (row jj, system kk*nx + ii), 2 = z (row kk, system
+// jj*nx + ii). One thread per cell, indexed in the batch's own order so the
+// writes are coalesced.
+__global__ void k_build(const double* __restrict__ th, const double* __restrict__ rhs, Dims g,
+                        int dir, double sa, double sb, double sc, int wall,
+                        double* __restrict__ A, double* __restrict__ B, double* __restrict__ C,
+                        double* __restrict__ D) {
+    const size_t n_in = (size_t)g.nx * g.ny * g.nz;
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_in) return;
+    int ii, jj, kk, row, n_row;
+    if (dir == 2) {
+        const size_t ns = (size_t)g.nx * g.ny;
+        kk = (int)(o / ns);
+        const int r = (int)(o % ns);
+        jj = r / g.nx;
+        ii = r % g.nx;
+        row = kk;
+        n_row = g.nz;
+    } else if (dir == 1) {
+        const size_t ns = (size_t)g.nx * g.nz;
+        jj = (int)(o / ns);
+        const int r = (int)(o % ns);
+        kk = r / g.nx;
+        ii = r % g.nx;
+        row = jj;
+        n_row = g.ny;
+    } else {
+        const size_t ns = (size_t)g.ny * g.nz;
+        ii = (int)(o / ns);
+        const int r = (int)(o % ns);
+        kk = r / g.ny;
+        jj = r % g.ny;
+        row = ii;
+        n_row = g.nx;
+    }
+    const int i = ii + 1, j = jj + 1, k = kk + 1;
+    double a = -sa, b = 1.0 - sb, c = -sc, d = rhs[cell(g, i, j, k)];
+    if (wall) {
+        const size_t st = dir == 0 ? 1 : dir == 1 ? (size_t)g.nx1 : (size_t)g.nx1 * g.ny1;
+        const size_t here = cell(g, i, j, k);
+        if (row == 0) {
+            d += sa * (th[here - st] + th[here]);
+            b += sa;
+        }
+        if (row == n_row - 1) {
+            d += sc * (th[here + st] + th[here]);
+            b += sc;
+        }
+    }
+    A[o] = a;
+    B[o] = b;
+    C[o] = c;
+    D[o] = d;
+}
+
+// solution of one sweep back into the field (rhs for Z and Y, theta for X)
+__global__ void k_scatter(const double* __restrict__ D, double* __restrict__ dst, Dims g, int dir) {
+    const size_t n_in = (size_t)g.nx * g.ny * g.nz;
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_in) return;
+    int ii, jj, kk;
+    if (dir == 2) {
+        const size_t ns = (size_t)g.nx * g.ny;
+        kk = (int)(o / ns);
+        const int r = (int)(o % ns);
+        jj = r / g.nx;
+        ii = r % g.nx;
+    } else if (dir == 1) {
+        const size_t ns = (size_t)g.nx * g.nz;
+        jj = (int)(o / ns);
+        const int r = (int)(o % ns);
+        kk = r / g.nx;
+        ii = r % g.nx;
+    } else {
+        const size_t ns = (size_t)g.ny * g.nz;
+        ii = (int)(o / ns);
+        const int r = (int)(o % ns);
+        kk = r / g.ny;
+        jj = r % g.ny;
+    }
+    dst[cell(g, ii + 1, jj + 1, kk + 1)] = D[o];
+}
+
+thread_local std::string g_heat_err;
+
+ftd_status herr(ftd_status s, const std::string& m) {
+    g_heat_err = m;
+    return s;
+}
+
+#define HH(call)                                                                              \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return herr(FTD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define HT(expr)                                               \
+    do {                                                       \
+        ftd_status s_ = (expr);                                \
+        if (s_ != FTD_OK) return herr(s_, ftd_last_error_string()); \
+    } while (0)
+
+}  // namespace
+
+struct ftd_heat {
+    Dims g{};
+    int nz = 0;
+    int parts[3] = {1, 1, 1};
+    int periodic[3] = {0, 0, 0};
+    int kind = FTD_KIND_PASCAL;
+    double dt = 0.0;
+    Stencil s{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double *th = nullptr, *rhs = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr;
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    std::vector<double> hx, hy, hz, sx_, sy_, sz_, cx_, cy_, cz_;
+    ftd_handle* solver[3] = {nullptr, nullptr, nullptr};  // x, y, z sweeps
+    long steps_done = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+ftd_status enqueue_step(ftd_heat* h) {
+    const Dims g = h->g;
+    hipStream_t st = h->stream;
+    const dim3 b2(16, 16);
+    // Dirichlet ghosts of the non-periodic faces
+    for (int f = 0; f < 3; ++f) {
+        if (h->periodic[f]) continue;
+        const int u = f == 0 ? g.ny : g.nx, v = f == 2 ? g.ny : g.nz;
+        hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f,
+                           h->cx, h->cy, h->cz);
+    }
+    hipLaunchKernelGGL(k_rhs, dim3((g.nx + 15) / 16, (g.ny + 15) / 16, h->nz), b2, 0, st, h->th,
+                       h->rhs, g, h->s, h->cx, h->cy, h->cz);
+    HH(hipGetLastError());
+    const size_t n_in = (size_t)g.nx * g.ny * h->nz;
+    const dim3 b1(256), g1((unsigned)((n_in + 255) / 256));
+    const int order[3] = {2, 1, 0};
+    const int n_sys[3] = {g.ny * h->nz, g.nx * h->nz, g.nx * g.ny};
+    const double sa[3] = {h->s.xa, h->s.ya, h->s.za}, sb[3] = {h->s.xb, h->s.yb, h->s.zb},
+                 sc[3] = {h->s.xc, h->s.yc, h->s.zc};
+    for (int q = 0; q < 3; ++q) {
+        const int d = order[q];
+        hipLaunchKernelGGL(k_build, g1, b1, 0, st, h->th, h->rhs, g, d, sa[d], sb[d], sc[d],
+                           h->periodic[d] ? 0 : 1, h->A, h->B, h->C, h->D);
+        HH(hipGetLastError());
+        ftd_handle* sv = h->solver[d];
+        HT(ftd_set_eps_constant(sv, h->dt));
+        HT(ftd_set_rho(sv, h->A, h->B, h->C));
+        if (h->periodic[d])
+            HT(ftd_solve_cyclic(sv, h->A, h->B, h->C, h->D));
+        else
+            HT(ftd_solve(sv, h->A, h->B, h->C, h->D));
+        (void)n_sys;
+        hipLaunchKernelGGL(k_scatter, g1, b1, 0, st, h->D, d == 0 ? h->th : h->rhs, g, d);
+        HH(hipGetLastError());
+    }
+    // periodic ghosts (ghostcellUpdate), x then y then z
+    for (int f = 0; f < 3; ++f) {
+        if (!h->periodic[f]) continue;
+        const int u = f == 0 ? g.ny1 : g.nx1, v = f == 2 ? g.ny1 : h->nz + 2;
+        hipLaunchKernelGGL(k_periodic, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f);
+    }
+    HH(hipGetLastError());
+    return FTD_OK;
+}
+
+void centres(std::vector<double>& c, int n) {
+    c.resize(n + 2);
+    const double dd = 2.0 / (double)n;
+    for (int i = 0; i <= n + 1; ++i) c[i] = -1.0 + dd / 2.0 + (double)(i - 1) * dd;
+}
+
+}  // namespace
+
+extern "C" {
+
+ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* parts, int kind,
+                           const int* periodic, double dt, void* stream) {
+    if (!out) return herr(FTD_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (nx < 3 || ny < 3 || nz < 3) return herr(FTD_ERR_INVALID, "need at least 3 cells per direction");
+    if (kind < FTD_KIND_FILTERED || kind > FTD_KIND_PASCAL) return herr(FTD_ERR_INVALID, "bad kind");
+    ftd_heat* h = new (std::nothrow) ftd_heat();
+    if (!h) return herr(FTD_ERR_ALLOC, "out of host memory");
+    h->g = Dims{nx, ny, nz, nx + 2, ny + 2};
+    h->nz = nz;
+    for (int d = 0; d < 3; ++d) {
+        h->parts[d] = parts ? parts[d] : 1;
+        h->periodic[d] = periodic ? (periodic[d] != 0) : 0;
+    }
+    h->kind = kind;
+    h->dt = dt;
+    const double ddx = 2.0 / nx, ddy = 2.0 / ny, ddz = 2.0 / nz;
+    const double bx = dt / (2.0 * ddx * ddx), by = dt / (2.0 * ddy * ddy), bz = dt / (2.0 * ddz * ddz);
+    h->s = Stencil{bx, -2.0 * bx, bx, by, -2.0 * by, by, bz, -2.0 * bz, bz, dt * 3.0 * kPi * kPi};
+    centres(h->hx, nx);
+    centres(h->hy, ny);
+    centres(h->hz, nz);
+    auto tab = [](const std::vector<double>& x, std::vector<double>& s, std::vector<double>& c) {
+        s.resize(x.size());
+        c.resize(x.size());
+        for (size_t i = 0; i < x.size(); ++i) {
+            s[i] = std::sin(kPi * x[i]);
+            c[i] = std::cos(kPi * x[i]);
+        }
+    };
+    tab(h->hx, h->sx_, h->cx_);
+    tab(h->hy, h->sy_, h->cy_);
+    tab(h->hz, h->sz_, h->cz_);
+    const size_t ncell = (size_t)(nx + 2) * (ny + 2) * (nz + 2), nin = (size_t)nx * ny * nz;
+    hipError_t e = hipSuccess;
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+    } else {
+        e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        h->own_stream = true;
+    }
+    double** bufs[] = {&h->th, &h->rhs};
+    for (double** b : bufs)
+        if (e == hipSuccess) e = hipMalloc(b, ncell * sizeof(double));
+    double** ins[] = {&h->A, &h->B, &h->C, &h->D};
+    for (double** b : ins)
+        if (e == hipSuccess) e = hipMalloc(b, nin * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->cx, (nx + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->cy, (ny + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->cz, (nz + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(h->cx, h->cx_.data(), (nx + 2) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->cy, h->cy_.data(), (ny + 2) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->cz, h->cz_.data(), (nz + 2) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(h->rhs, 0, ncell * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&h->e0);
+    if (e == hipSuccess) e = hipEventCreate(&h->e1);
+    if (e != hipSuccess) {
+        ftd_heat_destroy(h);
+        return herr(FTD_ERR_HIP, std::string("ftd_heat_create: ") + hipGetErrorString(e));
+    }
+    // x, y, z sweeps: n_sys = ny*nz, nx*nz, nx*ny
+    const int ns[3] = {ny * nz, nx * nz, nx * ny}, nr[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; ++d) {
+        ftd_status st = ftd_create(&h->solver[d], kind, ns[d], nr[d], h->parts[d], dt, h->stream);
+        if (st != FTD_OK) {
+            std::string m = ftd_last_error_string();
+            ftd_heat_destroy(h);
+            return herr(st, "sweep solver: " + m);
+        }
+    }
+    *out = h;
+    return ftd_heat_init(h);
+}
+
+ftd_status ftd_heat_destroy(ftd_heat* h) {
+    if (!h) return FTD_OK;
+    if (h->exec) hipGraphExecDestroy(h->exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    for (ftd_handle* s : h->solver) ftd_destroy(s);
+    for (double* p : {h->th, h->rhs, h->A, h->B, h->C, h->D, h->cx, h->cy, h->cz})
+        if (p) hipFree(p);
+    if (h->e0) hipEventDestroy(h->e0);
+    if (h->e1) hipEventDestroy(h->e1);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return FTD_OK;
+}
+
+// manufactured initial field at every cell (mpi_subdomain.cpp:164-173),
+// periodic wrap, Dirichlet ghosts (main.cpp:45-49)
+ftd_status ftd_heat_init(ftd_heat* h) {
+    if (!h) return herr(FTD_ERR_INVALID, "null handle");
+    const int nx1 = h->g.nx1, ny1 = h->g.ny1, nz1 = h->nz + 2;
+    std::vector<double> f((size_t)nx1 * ny1 * nz1);
+    for (int k = 0; k < nz1; ++k)
+        for (int j = 0; j < ny1; ++j)
+            for (int i = 0; i < nx1; ++i)
+                f[((size_t)k * ny1 + j) * nx1 + i] =
+                    h->sx_[i] * h->sy_[j] * h->sz_[k] + h->cx_[i] * h->cy_[j] * h->cz_[k];
+    HH(hipMemcpyAsync(h->th, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const Dims g = h->g;
+    const dim3 b2(16, 16);
+    for (int d = 0; d < 3; ++d) {
+        if (!h->periodic[d]) continue;
+        const int u = d == 0 ? g.ny1 : g.nx1, v = d == 2 ? g.ny1 : h->nz + 2;
+        hipLaunchKernelGGL(k_periodic, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, h->stream, h->th, g, d);
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (h->periodic[d]) continue;
+        const int u = d == 0 ? g.ny : g.nx, v = d == 2 ? g.ny : g.nz;
+        hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, h->stream, h->th,
+                           g, d, h->cx, h->cy, h->cz);
+    }
+    HH(hipGetLastError());
+    HH(hipStreamSynchronize(h->stream));
+    h->steps_done = 0;
+    return FTD_OK;
+}
+
+// nsteps ADI steps; use_graph != 0 captures one step into a hipGraph on first
+// use and replays it (the first step always runs eagerly so that every
+// solver scratch buffer exists before capture). Waits for the last step and
+// records its device time.
+ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph) {
+    if (!h) return herr(FTD_ERR_INVALID, "null handle");
+    for (int t = 0; t < nsteps; ++t) {
+        const bool last = t == nsteps - 1;
+        if (last) HH(hipEventRecord(h->e0, h->stream));
+        if (use_graph && h->steps_done > 0) {
+            if (!h->exec) {
+                HH(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                ftd_status s = enqueue_step(h);
+                hipGraph_t graph = nullptr;
+                hipError_t e = hipStreamEndCapture(h->stream, &graph);
+                if (s != FTD_OK) {
+                    if (graph) hipGraphDestroy(graph);
+                    return s;
+                }
+                HH(e);
+                h->graph = graph;
+                HH(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+            }
+            HH(hipGraphLaunch(h->exec, h->stream));
+        } else {
+            ftd_status s = enqueue_step(h);
+            if (s != FTD_OK) return s;
+        }
+        if (last) HH(hipEventRecord(h->e1, h->stream));
+        ++h->steps_done;
+    }
+    if (nsteps > 0) {
+        HH(hipEventSynchronize(h->e1));
+        float ms = 0.f;
+        HH(hipEventElapsedTime(&ms, h->e0, h->e1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+// copies the field with ghosts ((nz+2)(ny+2)(nx+2), x fastest) to host memory
+ftd_status ftd_heat_field(ftd_heat* h, double* host_out) {
+    if (!h || !host_out) return herr(FTD_ERR_INVALID, "null argument");
+    const size_t ncell = (size_t)h->g.nx1 * h->g.ny1 * (h->nz + 2);
+    HH(hipMemcpyAsync(host_out, h->th, ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HH(hipStreamSynchronize(h->stream));
+    return FTD_OK;
+}
+
+// L2 error against the manufactured solution at t = steps*dt (main.cpp:59-81;
+// the sum is divided by (nx+1)(ny+1)(nz+1) as the reference does)
+ftd_status ftd_heat_l2(ftd_heat* h, double* l2) {
+    if (!h || !l2) return herr(FTD_ERR_INVALID, "null argument");
+    const int nx = h->g.nx, ny = h->g.ny, nz = h->nz, nx1 = h->g.nx1, ny1 = h->g.ny1;
+    std::vector<double> f((size_t)nx1 * ny1 * (nz + 2));
+    ftd_status s = ftd_heat_field(h, f.data());
+    if (s != FTD_OK) return s;
+    const double decay = std::exp(-3.0 * kPi * kPi * (double)h->steps_done * h->dt);
+    double err = 0.0;
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double ex = std::sin(kPi * h->hx[i]) * std::sin(kPi * h->hy[j]) *
+                                      std::sin(kPi * h->hz[k]) * decay +
+                                  std::cos(kPi * h->hx[i]) * std::cos(kPi * h->hy[j]) *
+                                      std::cos(kPi * h->hz[k]);
+                const double d = f[((size_t)k * ny1 + j) * nx1 + i] - ex;
+                err += d * d;
+            }
+    *l2 = std::sqrt(err / (double)(nx + 1) / (double)(ny + 1) / (double)(nz + 1));
+    return FTD_OK;
+}
+
+double ftd_heat_last_step_ms(const ftd_heat* h) { return h ? h->last_ms : 0.0; }
+
+const char* ftd_heat_last_error_string(void) { return g_heat_err.c_str(); }
+
+}  // extern "C"

@@ -144,6 +144,63 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
     }
 }
 
+// Row loads of the fused kernels: row j of the lane's partition, clamped to
+// the partition's last row so that the load never depends on a branch (the
+// compiler may then issue it PF rows ahead of its use). Rows past the count
+// reload the last row, which is in cache, and are never used.
+#define FTD_LOAD_ROW(j)                                                      \
+    do {                                                                     \
+        const int jj_ = (j) < cnt ? (j) : cnt - 1;                           \
+        const size_t o_ = base + (size_t)jj_ * ld;                           \
+        ra[(j)] = A[o_];                                                     \
+        rb[(j)] = B[o_];                                                     \
+        rc[(j)] = C[o_];                                                     \
+        rd[(j)] = D[o_];                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// plain Thomas with the whole system in registers (n_row <= M): the
+// reference's single-rank path (tdma_many_kernel, tdma_local_cuda.cu:32-64),
+// row 0 divided by b. One lane per system, D read once and written once.
+// ---------------------------------------------------------------------------
+template <int M, int PF>
+__global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__ A,
+                                                      const double* __restrict__ B,
+                                                      const double* __restrict__ C,
+                                                      double* __restrict__ D, int n_sys,
+                                                      int n_row) {
+    const int i_raw = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i_raw >= n_sys) return;
+    const int i = i_raw;
+    const int cnt = n_row;
+    const size_t ld = (size_t)n_sys;
+    const size_t base = (size_t)i;
+    double c[M], d[M];
+    double ra[M], rb[M], rc[M], rd[M];
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_LOAD_ROW(j + PF);
+        if (j < cnt) {
+            if (j == 0) {
+                d[0] = rd[0] / rb[0];
+                c[0] = rc[0] / rb[0];
+            } else {
+                const double r = 1.0 / (rb[j] - ra[j] * c[j - 1]);
+                d[j] = r * (rd[j] - ra[j] * d[j - 1]);
+                c[j] = r * rc[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = M - 2; j >= 0; --j)
+        if (j <= cnt - 2) d[j] = d[j] - c[j] * d[j + 1];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < cnt) D[base + (size_t)j * ld] = d[j];
+}
+
 // ---------------------------------------------------------------------------
 // fused PaScaL: modified Thomas per (system, sub-partition) in registers,
 // reduced 2Q-row system in LDS, interior update, single store of x.
@@ -151,8 +208,8 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
 // / update_solution_kernel (libs/pascal_tdma/gpu/tdma_local_cuda.cu:67-224)
 // for Q partitions of one GPU.
 // ---------------------------------------------------------------------------
-template <int M, int MAXT, bool CYCLIC>
-__global__ __launch_bounds__(MAXT) void k_pascal_fused(const double* __restrict__ A,
+template <int M, int MAXT, bool CYCLIC, int PF, int WPE>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_fused(const double* __restrict__ A,
                                                        const double* __restrict__ B,
                                                        const double* __restrict__ C,
                                                        double* __restrict__ D, int n_sys,
@@ -171,14 +228,18 @@ __global__ __launch_bounds__(MAXT) void k_pascal_fused(const double* __restrict_
     const size_t base = (size_t)r0 * ld + (size_t)i;
 
     double a[M], c[M], d[M];
+    double ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
     double aN = 0.0, cN = 0.0, dN = 0.0;
+
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
 
     // forward sweep: rows 0,1 divided by b, then modified elimination
 #pragma unroll
     for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_LOAD_ROW(j + PF);
         if (j < cnt) {
-            const size_t o = base + (size_t)j * ld;
-            const double aj = A[o], bj = B[o], cj = C[o], dj = D[o];
+            const double aj = ra[j], bj = rb[j], cj = rc[j], dj = rd[j];
             if (j < 2) {
                 a[j] = aj / bj;
                 c[j] = cj / bj;
@@ -287,8 +348,8 @@ __global__ __launch_bounds__(MAXT) void k_pascal_fused(const double* __restrict_
 //   PASS 2: v1 second pass -> recompute the sweeps, take x0, xN from xbuf and
 //           J_p, store x.
 // ---------------------------------------------------------------------------
-template <int M, int MAXT, int VERSION, int PASS>
-__global__ __launch_bounds__(MAXT) void k_filtered_fused(const double* __restrict__ A,
+template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_fused(const double* __restrict__ A,
                                                          const double* __restrict__ B,
                                                          const double* __restrict__ C,
                                                          double* __restrict__ D, int n_sys,
@@ -311,15 +372,20 @@ __global__ __launch_bounds__(MAXT) void k_filtered_fused(const double* __restric
     const int J = (VERSION == 2 || PASS == 2) ? Jp[p] : m;
 
     double a[M], c[M], d[M];
+    double ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
     double aN = 0.0, cN = 0.0, dN = 0.0;
     double b1 = 1.0;
+    const int cnt = m;
+
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
 
     // forward sweep (k_fwd_pass_v1 / k_fwd_pass_v2)
 #pragma unroll
     for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_LOAD_ROW(j + PF);
         if (j < m) {
-            const size_t o = base + (size_t)j * ld;
-            const double aj = A[o], bj = B[o], cj = C[o], dj = D[o];
+            const double aj = ra[j], bj = rb[j], cj = rc[j], dj = rd[j];
             if (j < 2) {
                 const double r = 1.0 / bj;
                 a[j] = aj * r;
@@ -841,16 +907,18 @@ hipError_t launch_compute_J(const double* partial, int n_tiles, int P, int n_row
 
 // -------- fused PaScaL dispatch over the compiled (M, MAXT) variants --------
 
+// (rows per lane M, max threads T, prefetch rows PF, target waves per SIMD)
+#ifndef FTD_PASCAL_VARIANTS
 #define FTD_PASCAL_VARIANTS(X) \
-    X(8, 1024)                  \
-    X(16, 512)                  \
-    X(16, 1024)                 \
-    X(32, 256)                  \
-    X(32, 512)                  \
-    X(64, 256)
+    X(8, 512, 8, 4)             \
+    X(16, 512, 8, 2)            \
+    X(32, 256, 8, 2)            \
+    X(32, 512, 4, 2)            \
+    X(64, 256, 4, 1)
+#endif
 
 int pascal_fused_supported(int M, int maxt) {
-#define X(m_, t_) \
+#define X(m_, t_, pf_, w_) \
     if (M == m_ && maxt == t_) return 1;
     FTD_PASCAL_VARIANTS(X)
 #undef X
@@ -867,31 +935,50 @@ hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * Q);
     const size_t lds = pascal_fused_lds(Q, S, cyclic);
-#define X(m_, t_)                                                                                \
-    if (M == m_ && maxt == t_) {                                                                 \
-        if (cyclic)                                                                              \
-            hipLaunchKernelGGL((k_pascal_fused<m_, t_, true>), grid, block, lds, st, A, B, C, D, \
-                               n_sys, n_row, Q, lgS);                                            \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false>), grid, block, lds, st, A, B, C,   \
-                               D, n_sys, n_row, Q, lgS);                                         \
-        return hipGetLastError();                                                                \
+#define X(m_, t_, pf_, w_)                                                                 \
+    if (M == m_ && maxt == t_) {                                                           \
+        if (cyclic)                                                                        \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, true, pf_, w_>), grid, block, lds,  \
+                               st, A, B, C, D, n_sys, n_row, Q, lgS);                      \
+        else                                                                               \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false, pf_, w_>), grid, block, lds, \
+                               st, A, B, C, D, n_sys, n_row, Q, lgS);                      \
+        return hipGetLastError();                                                          \
     }
     FTD_PASCAL_VARIANTS(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
+int thomas_fused_rows(int n_row) {
+    return n_row <= 8 ? 8 : n_row <= 16 ? 16 : n_row <= 32 ? 32 : n_row <= 64 ? 64 : 0;
+}
+
+hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
+                               int n_sys, int n_row, hipStream_t st) {
+    const dim3 grid((n_sys + 255) / 256), block(256);
+    switch (thomas_fused_rows(n_row)) {
+        case 8: hipLaunchKernelGGL((k_thomas_fused<8, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 16: hipLaunchKernelGGL((k_thomas_fused<16, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 32: hipLaunchKernelGGL((k_thomas_fused<32, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 64: hipLaunchKernelGGL((k_thomas_fused<64, 4>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // -------- fused Filtered dispatch --------
 
+#ifndef FTD_FILTERED_VARIANTS
 #define FTD_FILTERED_VARIANTS(X) \
-    X(8, 1024)                    \
-    X(16, 512)                    \
-    X(32, 256)                    \
-    X(64, 256)
+    X(8, 512, 8, 4)               \
+    X(16, 512, 8, 2)              \
+    X(32, 256, 8, 2)              \
+    X(64, 256, 4, 1)
+#endif
 
 int filtered_fused_supported(int M, int maxt) {
-#define X(m_, t_) \
+#define X(m_, t_, pf_, w_) \
     if (M == m_ && maxt == t_) return 1;
     FTD_FILTERED_VARIANTS(X)
 #undef X
@@ -907,17 +994,17 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * P);
     const size_t lds = (pass == 2) ? 0 : filtered_fused_lds(P, S);
-#define LAUNCH(m_, t_, v_, p_)                                                                   \
-    hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_>), grid, block, lds, st, A, B, C, D,     \
-                       n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial)
-#define X(m_, t_)                                         \
+#define LAUNCH(m_, t_, v_, p_, pf_, w_)                                                    \
+    hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_>), grid, block, lds, st, A, \
+                       B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial)
+#define X(m_, t_, pf_, w_)                                \
     if (M == m_ && maxt == t_) {                          \
         if (version == 2)                                 \
-            LAUNCH(m_, t_, 2, 0);                         \
+            LAUNCH(m_, t_, 2, 0, pf_, w_);                \
         else if (pass == 1)                               \
-            LAUNCH(m_, t_, 1, 1);                         \
+            LAUNCH(m_, t_, 1, 1, pf_, w_);                \
         else                                              \
-            LAUNCH(m_, t_, 1, 2);                         \
+            LAUNCH(m_, t_, 1, 2, pf_, w_);                \
         return hipGetLastError();                         \
     }
     FTD_FILTERED_VARIANTS(X)

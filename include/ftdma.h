@@ -113,7 +113,7 @@ double ftd_last_comm_ms(const ftd_handle* h);
 
 /* Extension (autotuner hook): force the fused-kernel shape of later solves.
  * rows_per_lane in {8,16,32,64} (0 = automatic), max_threads in
- * {256,512,1024}, systems_per_tile a power of two <= 64 (0 = automatic),
+ * {256,512}, systems_per_tile a power of two <= 64 (0 = automatic),
  * lanes_per_system (PaScaL sub-partitions, 0 = automatic). rows_per_lane
  * == -1 selects the general kernels (partitions walked in global memory,
  * in place) for every later solve. Returns
@@ -128,6 +128,44 @@ const char* ftd_last_path(const ftd_handle* h);
 
 /* Text of the last error reported by any ftd_* call in this thread. */
 const char* ftd_last_error_string(void);
+
+/* ------------------------------------------------------------------------
+ * Heat ADI harness: the reference's caller of the path, SolveTheta
+ * (apps/heat_gpu/solve_theta.cu:391-650; apps/heat_cpu/solve_theta.cpp:18-334)
+ * with MPISubdomain's ghost cells (apps/heat_cpu/mpi_subdomain.cpp:90-236)
+ * and main.cpp's L2 error (:59-81), for the manufactured solution
+ * sin(pi x)sin(pi y)sin(pi z)exp(-3 pi^2 t) + cos(pi x)cos(pi y)cos(pi z) on
+ * [-1,1]^3. The field lives on the device with one ghost layer per face;
+ * parts[3] = the reference's (npx, npy, npz) become partitions of the x, y
+ * and z sweeps, each one batched solve of kind `kind`.
+ * ------------------------------------------------------------------------ */
+typedef struct ftd_heat ftd_heat;
+
+/* GlobalParams + MPISubdomain set-up and the initial field (main.cpp:20-49);
+ * nx, ny, nz interior cells, periodic[3] flags (NULL = Dirichlet), step dt,
+ * stream NULL = a stream owned by the handle. */
+ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* parts, int kind,
+                           const int* periodic, double dt, void* stream);
+ftd_status ftd_heat_destroy(ftd_heat* h);
+
+/* re-initialise the field to t = 0 (mpi_subdomain.cpp:164-173) */
+ftd_status ftd_heat_init(ftd_heat* h);
+
+/* nsteps time steps (solve_theta.cpp:92-334); use_graph != 0 replays one
+ * step captured into a hipGraph. Waits for the last step. */
+ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph);
+
+/* the field with ghosts, (nz+2)(ny+2)(nx+2) doubles, x fastest, to host memory */
+ftd_status ftd_heat_field(ftd_heat* h, double* host_out);
+
+/* L2 error at t = steps * dt, as main.cpp:59-81 prints it */
+ftd_status ftd_heat_l2(ftd_heat* h, double* l2);
+
+/* device time of the last step of the most recent ftd_heat_step, in ms */
+double ftd_heat_last_step_ms(const ftd_heat* h);
+
+/* text of the last error reported by an ftd_heat_* call in this thread */
+const char* ftd_heat_last_error_string(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""The heat ADI harness on the GPU (ftd_heat_* through the C ABI) against the
+oracle's restatement of apps/heat_cpu, and the hipGraph replay against the
+eager steps.
+
+Bit-for-bit where every sweep runs the same partitioning in both (the
+filtered kinds always; PaScaL when its sweeps fit the single-lane Thomas
+kernel); the L2 text must equal what the reference heat_cpu prints for the
+same case (tests/golden/heat_l2.json)."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from filtered_tdma_amd.heat import SolveTheta, dt_policy
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "heat_l2.json")
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def golden():
+    with open(GOLDEN) as f:
+        return [c for c in json.load(f) if c["n"] <= 64]
+
+
+@pytest.mark.parametrize("case", golden(),
+                         ids=lambda c: f"{c['kind']}-n{c['n']}-r{c['rho']}-"
+                                       f"{'x'.join(map(str, c['parts']))}-per{c['periodic']}")
+def test_heat_gpu_matches_reference_l2_and_oracle(case):
+    n, parts, kind = case["n"], tuple(case["parts"]), case["kind"]
+    per = (1, 1, 1) if case["periodic"] else (0, 0, 0)
+    dt, nt = dt_policy("order", case["rho"], n)
+    note(f"heat {kind} n={n} parts={parts} periodic={per} dt={dt} Nt={nt}")
+    sim = SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt, periodic=per)
+    try:
+        sim.run(nt)
+        l2 = sim.l2()
+        field = sim.field()
+    finally:
+        sim.close()
+    assert "%g" % l2 == case["l2"]
+    l2o, ref = O.heat_run(n, n, n, parts, kind, dt, nt, periodic=per, sem=O.SEM_GPU,
+                          return_theta=True)
+    inner = (slice(1, -1),) * 3
+    if kind != "pascal" or max(parts) == 1:
+        # same partitioning in every sweep: same statements, same rounding
+        np.testing.assert_array_equal(field[inner], ref[inner])
+    rel = np.max(np.abs(field[inner] - ref[inner])) / np.max(np.abs(ref[inner]))
+    assert rel <= 1e-13
+
+
+@pytest.mark.parametrize("kind,parts,periodic", [
+    ("filtered", (1, 1, 4), (0, 0, 0)),
+    ("filtered_v2", (2, 2, 2), (0, 0, 0)),
+    ("pascal", (1, 1, 1), (1, 1, 1)),
+])
+def test_heat_graph_replay_equals_eager(kind, parts, periodic):
+    """A step captured into a hipGraph and replayed gives the eager steps'
+    field bit for bit, replay after replay, with eager work in between."""
+    n, steps = 48, 6
+    dt = dt_policy("order", 0.3, n)[0]
+    note(f"heat graph {kind} parts={parts} periodic={periodic}")
+    eager = SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt, periodic=periodic)
+    graph = SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt, periodic=periodic)
+    try:
+        for s in range(steps):
+            eager.run(1, use_graph=False)
+            graph.run(1, use_graph=True)
+            torch.cuda.synchronize()
+            _ = torch.ones(1024, device="cuda").sum().item()  # unrelated eager work
+            np.testing.assert_array_equal(graph.field(), eager.field())
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_heat_config1_filtered_vs_pascal_vs_thomas():
+    """The 64^3, 100-step, dt = eps_constant = 1e-3, 4 z-slab case:
+    filtered and filtered_v2 against pascal and single-partition Thomas,
+    and each against the oracle's restatement."""
+    n, dt, steps = 64, 1e-3, 100
+    res = {}
+    for kind, parts in (("pascal", (1, 1, 4)), ("filtered", (1, 1, 4)),
+                        ("filtered_v2", (1, 1, 4)), ("pascal", (1, 1, 1))):
+        note(f"heat config1 {kind} parts={parts}")
+        sim = SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt)
+        try:
+            sim.run(steps, use_graph=True)
+            res[(kind, parts)] = (sim.l2(), sim.field())
+        finally:
+            sim.close()
+        l2o = O.heat_run(n, n, n, parts, kind, dt, steps, sem=O.SEM_GPU)
+        assert abs(res[(kind, parts)][0] - l2o) <= 1e-9 * l2o
+    thomas = res[("pascal", (1, 1, 1))][1]
+    inner = (slice(1, -1),) * 3
+    for key, (l2, f) in res.items():
+        assert np.max(np.abs(f[inner] - thomas[inner])) <= 1e-6, key

@@ -75,7 +75,7 @@ PASCAL_SHAPES = [
     (513, 96, (16, 512, 16, 0)),
     (200, 240, (32, 512, 16, 0)),
     (130, 500, (64, 256, 32, 0)),
-    (99, 60, (8, 1024, 64, 0)),
+    (99, 60, (8, 512, 64, 0)),
 ]
 
 
@@ -87,9 +87,14 @@ def test_pascal_fused_bitwise(n_sys, n_row, tune, cyclic):
     note(f"pascal_fused n_sys={n_sys} n_row={n_row} tune={tune} cyclic={cyclic}")
     A, B, C, D = cases.general_batch(n_sys, n_row, seed=11)
     x, _, path, Bafter = run(Kind.PASCAL, A, B, C, D, P=1, cyclic=cyclic, tuning=tune)
-    assert path.startswith("pascal_fused"), path
-    Q = int(path.split("Q=")[1])
-    ref = O.pascal(A, B, C, D, Q, sem=O.SEM_GPU, cyclic=cyclic)
+    if path.startswith("thomas_fused"):
+        # a system that fits one lane: plain Thomas, the reference's P == 1 path
+        assert not cyclic and n_row <= 64
+        ref = O.thomas(A, B, C, D, sem=O.SEM_GPU)
+    else:
+        assert path.startswith("pascal_fused"), path
+        Q = int(path.split("Q=")[1])
+        ref = O.pascal(A, B, C, D, Q, sem=O.SEM_GPU, cyclic=cyclic)
     np.testing.assert_array_equal(x, ref)
     np.testing.assert_array_equal(Bafter, B)
     thomas = O.thomas(A, B, C, D, sem=O.SEM_GPU, cyclic=cyclic)
