@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kinds", default="pascal,filtered_v2,filtered")
+    ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
     return ap.parse_args()
 
 
@@ -73,6 +74,54 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     J = be.last_J() if kind != "pascal" else None
     be.close()
     return wall, ev_ms, path, J
+
+
+def time_f32(tA32, tB32, tC32, tD, tD32, copies, steps, warmup, d32, torch):
+    """FP32-storage PaScaL (C3): d32=False -> FP32 A,B,C with FP64 D in place
+    (28 B/row); d32=True -> FP32 A,B,C,D, FP64 x to a separate array (24 B/row)."""
+    from filtered_tdma_amd import TdmaBackendGPU
+
+    be = TdmaBackendGPU("pascal", N_SYS, N_ROW, P, 0.0)
+    stream = torch.cuda.current_stream()
+    for k in range(warmup):
+        if d32:
+            be.solve_f32(tA32, tB32, tC32, tD32, copies[k % len(copies)])
+        else:
+            copies[k % len(copies)].copy_(tD)
+            be.solve_f32c(tA32, tB32, tC32, copies[k % len(copies)])
+    if not d32:
+        for c in copies:
+            c.copy_(tD)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        if d32:
+            be.solve_f32(tA32, tB32, tC32, tD32, copies[k])
+        else:
+            be.solve_f32c(tA32, tB32, tC32, copies[k])
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    path = be.last_path()
+    be.close()
+    return wall, ev_ms, path
+
+
+def c3_error(x_dev, A32, B32, C32, Dw, path):
+    """max over sampled systems of |x - x_ref|_inf/|x_ref|_inf against the FP64
+    oracle of the stored (exactly widened) data."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cols = np.arange(0, N_SYS, 4099)
+    sub = [np.ascontiguousarray(a[:, cols]).astype(np.float64) for a in (A32, B32, C32)]
+    ref = O.thomas(*sub, np.ascontiguousarray(Dw[:, cols]))
+    x = x_dev[:, cols]
+    return float(np.max(np.max(np.abs(x - ref), axis=0) / np.max(np.abs(ref), axis=0)))
 
 
 def cpu_baseline(A, B, C, D):
@@ -130,6 +179,30 @@ def main():
         }
         print(f"[bench] {kind}: {ms:.4f} ms/step (events {ev_avg:.4f} ms) {path} J={J}",
               file=sys.stderr, flush=True)
+
+    # C3: same shape, r ~ U[5, 500] (rho up to 0.499), FP32 storage
+    if not args.no_c3:
+        A3, B3, C3, D3, _ = cases.toeplitz_batch(N_SYS, N_ROW, 5.0, 500.0, seed=SEED)
+        A32, B32, C32, D32 = (x.astype(np.float32) for x in (A3, B3, C3, D3))
+        tA32, tB32, tC32, tD32 = (torch.from_numpy(x).cuda() for x in (A32, B32, C32, D32))
+        tD3 = torch.from_numpy(D3).cuda()
+        for name, d32, bpr in (("pascal_f32c", False, 28), ("pascal_f32", True, 24)):
+            wall, ev_ms, path = time_f32(tA32, tB32, tC32, tD3, tD32, copies, args.steps,
+                                         args.warmup, d32, torch)
+            ms = wall * 1e3 / args.steps
+            ev_avg = statistics.mean(ev_ms)
+            err = c3_error(copies[args.steps - 1].cpu().numpy(), A32, B32, C32,
+                           D32.astype(np.float64) if d32 else D3, path)
+            results[name] = {
+                "rows_per_s": rows / (wall / args.steps), "ms_per_step": ms,
+                "event_ms_mean": ev_avg, "bytes_per_row": bpr,
+                "hbm_frac_algorithmic": bpr * rows / (ev_avg * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                "max_rel_err_vs_fp64_oracle": err, "path": path,
+                "workload": "r~U[5,500], FP32 storage, FP64 elimination",
+            }
+            print(f"[bench] {name}: {ms:.4f} ms/step (events {ev_avg:.4f} ms) err {err:.2e} {path}",
+                  file=sys.stderr, flush=True)
+        del tA32, tB32, tC32, tD32, tD3
 
     head = results["pascal"] if "pascal" in results else next(iter(results.values()))
     achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9

@@ -26,15 +26,16 @@ class Kind(enum.IntEnum):
     PASCAL = 2
 
 
-def _ptr(x, n_row=None, n_sys=None):
+def _ptr(x, n_row=None, n_sys=None, dtype="float64"):
     if isinstance(x, int):
         return ctypes.c_void_p(x)
     import torch
 
     if not isinstance(x, torch.Tensor):
         raise TypeError("arrays must be torch tensors or integer device addresses")
-    if not x.is_cuda or x.dtype != torch.float64 or not x.is_contiguous():
-        raise ValueError("arrays must be contiguous float64 device tensors")
+    want = getattr(torch, dtype)
+    if not x.is_cuda or x.dtype != want or not x.is_contiguous():
+        raise ValueError(f"arrays must be contiguous {dtype} device tensors")
     if n_row is not None and tuple(x.shape) != (n_row, n_sys):
         raise ValueError(f"expected shape ({n_row}, {n_sys}), got {tuple(x.shape)}")
     return ctypes.c_void_p(x.data_ptr())
@@ -89,6 +90,16 @@ class TdmaBackendGPU:
 
     def solve_cyclic(self, A, B, C, D) -> None:
         _lib.check(_lib.load().ftd_solve_cyclic(self._h, *self._args(A, B, C, D)))
+
+    def solve_f32c(self, A, B, C, D) -> None:
+        """FP32 A, B, C; FP64 D <- x (28 B/row)."""
+        a, b, c = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C))
+        _lib.check(_lib.load().ftd_solve_f32c(self._h, a, b, c, _ptr(D, self.n_row, self.n_sys)))
+
+    def solve_f32(self, A, B, C, D, X) -> None:
+        """FP32 A, B, C, D; x to FP64 X (24 B/row)."""
+        a, b, c, d = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C, D))
+        _lib.check(_lib.load().ftd_solve_f32(self._h, a, b, c, d, _ptr(X, self.n_row, self.n_sys)))
 
     def set_rho_device(self, A, B, C) -> None:
         a, b, c, _ = self._args(A, B, C, A)

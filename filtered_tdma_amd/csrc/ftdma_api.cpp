@@ -106,31 +106,61 @@ struct FusedShape {
     int M = 0, T = 0, S = 0, Q = 0;
 };
 
-// Fused-kernel shape for an exact solve of n_row rows: Q lanes per system
-// (sub-partitions of <= M rows, >= 3 rows each), S systems per workgroup.
+// Fused-kernel shape for an exact solve of n_row rows split into Q lanes per
+// system (sub-partitions of <= M rows, >= 3 rows each), S systems per
+// workgroup. With P > 1 partitions on the handle, Q = P whenever a compiled
+// variant holds them: the result is then the reference's P-rank PaScaL
+// statement for statement. Otherwise Q is chosen here (at least 2: one
+// partition is the plain Thomas kernel).
+bool shape_for(int M, int T, int Q, int n_row, int forced_S, FusedShape& out) {
+    if (!ftd::pascal_fused_supported(M, T) || Q < 2 || Q > T) return false;
+    int s_, m_min, m_max;
+    part_of(n_row, Q, Q - 1, s_, m_min);
+    part_of(n_row, Q, 0, s_, m_max);
+    if (m_min < 3 || m_max > M) return false;
+    const int S = forced_S ? forced_S : std::min(64, pow2_floor(T / Q));
+    if (!is_pow2(S) || S > 64 || S * Q > T) return false;
+    out.M = M;
+    out.T = T;
+    out.S = S;
+    out.Q = Q;
+    return true;
+}
+
 bool pick_pascal(const ftd_handle* h, int n_row, int cyclic, FusedShape& out) {
-    static const int Ms[] = {32, 16, 64, 8};
-    static const int Ts[] = {256, 512, 256, 512};
-    const int ncand = h->t_M ? 1 : 4;
-    for (int k = 0; k < ncand; ++k) {
-        const int M = h->t_M ? h->t_M : Ms[k];
-        const int T = h->t_M ? (h->t_T ? h->t_T : 256) : Ts[k];
-        if (!ftd::pascal_fused_supported(M, T)) continue;
-        int Q = h->t_Q ? h->t_Q : (n_row + M - 1) / M;
-        if (Q < 2) Q = 2;  // one partition is the plain Thomas kernel
-        if (Q < 1 || Q > T) continue;
-        int s_, m_min, m_max;
-        part_of(n_row, Q, Q - 1, s_, m_min);
-        part_of(n_row, Q, 0, s_, m_max);
-        if (m_min < 3 || m_max > M) continue;
-        const int S = h->t_S ? h->t_S : std::min(64, pow2_floor(T / Q));
-        if (!is_pow2(S) || S > 64 || S * Q > T) continue;
-        if (!h->t_S && !h->t_M && S < 16) continue;  // keep >= 128-byte row segments
-        out.M = M;
-        out.T = T;
-        out.S = S;
-        out.Q = Q;
-        return true;
+    if (h->t_M) {
+        int Q = h->t_Q ? h->t_Q : std::max(2, (n_row + h->t_M - 1) / h->t_M);
+        return shape_for(h->t_M, h->t_T ? h->t_T : 256, Q, n_row, h->t_S, out);
+    }
+    // (M, T) in order of preference, fastest first at 262144 x 512
+    static const int Ms[] = {64, 32, 32, 16, 8};
+    static const int Ts[] = {256, 512, 256, 512, 512};
+    const int P = h->P;
+    if (P > 1) {
+        int s_, m_max;
+        part_of(n_row, P, 0, s_, m_max);
+        for (int k = 0; k < 5; ++k) {
+            if (Ms[k] < m_max) continue;
+            FusedShape fs;
+            if (shape_for(Ms[k], Ts[k], P, n_row, 0, fs) && fs.S >= 16) {
+                // smallest M that holds the partitions
+                out = fs;
+                for (int k2 = 0; k2 < 5; ++k2)
+                    if (Ms[k2] >= m_max && Ms[k2] < out.M && shape_for(Ms[k2], Ts[k2], P, n_row, 0, fs) &&
+                        fs.S >= 16)
+                        out = fs;
+                return true;
+            }
+        }
+    }
+    (void)cyclic;
+    for (int k = 0; k < 5; ++k) {
+        const int Q = std::max(2, (n_row + Ms[k] - 1) / Ms[k]);
+        FusedShape fs;
+        if (shape_for(Ms[k], Ts[k], Q, n_row, 0, fs) && fs.S >= 16) {
+            out = fs;
+            return true;
+        }
     }
     return false;
 }
@@ -140,10 +170,17 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
     hipStream_t st = h->stream;
     if (cyclic && n_row < 3) return fail(FTD_ERR_UNSUPPORTED, "periodic solve needs n_row >= 3");
     FusedShape fs;
-    if (!h->force_general && !cyclic && !h->t_M && n_row >= 3 && ftd::thomas_fused_rows(n_row)) {
-        // short systems: plain Thomas in registers, the reference's P == 1 path
-        std::snprintf(h->path, sizeof h->path, "thomas_fused M=%d", ftd::thomas_fused_rows(n_row));
-        FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, st));
+    if (!h->force_general && !h->t_M && h->P == 1 && n_row >= 3 && ftd::thomas_fused_rows(n_row)) {
+        // one partition that fits a lane: plain (or Sherman-Morrison periodic)
+        // Thomas in registers, the reference's single-rank path
+        if (cyclic) {
+            std::snprintf(h->path, sizeof h->path, "thomas_cyclic_fused M=%d",
+                          ftd::thomas_fused_rows(n_row));
+            FTD_HIP(ftd::launch_thomas_cyclic_fused(A, B, C, D, n_sys, n_row, st));
+        } else {
+            std::snprintf(h->path, sizeof h->path, "thomas_fused M=%d", ftd::thomas_fused_rows(n_row));
+            FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, st));
+        }
         return FTD_OK;
     }
     if (!h->force_general && n_row >= 3 && pick_pascal(h, n_row, cyclic, fs)) {
@@ -436,6 +473,44 @@ ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int
     h->t_S = systems_per_tile;
     h->t_Q = lanes_per_system;
     return FTD_OK;
+}
+
+static ftd_status solve_f32_common(ftd_handle* h, const float* A, const float* B, const float* C,
+                                   const void* D, double* X, int d_is_float) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (!A || !B || !C || !D || !X) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (h->kind != FTD_KIND_PASCAL && h->P > 1)
+        return fail(FTD_ERR_UNSUPPORTED, "FP32 storage is available for the exact (PaScaL) solve");
+    if (h->n_sys == 0) return FTD_OK;
+    FusedShape fs;
+    if (h->n_row < 6 || !pick_pascal(h, h->n_row, 0, fs))
+        return fail(FTD_ERR_UNSUPPORTED, "no fused kernel holds this system for FP32 storage");
+    if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
+    std::snprintf(h->path, sizeof h->path, "pascal_fused_%s M=%d T=%d S=%d Q=%d",
+                  d_is_float ? "f32" : "f32c", fs.M, fs.T, fs.S, fs.Q);
+    if (d_is_float)
+        FTD_HIP(ftd::launch_pascal_fused_f32(fs.M, fs.T, A, B, C, (const float*)D, X, h->n_sys,
+                                             h->n_row, fs.Q, fs.S, 0, h->stream));
+    else
+        FTD_HIP(ftd::launch_pascal_fused_f32c(fs.M, fs.T, A, B, C, X, h->n_sys, h->n_row, fs.Q,
+                                              fs.S, 0, h->stream));
+    if (h->profiling) {
+        FTD_HIP(hipEventRecord(h->ev1, h->stream));
+        FTD_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0.0f;
+        FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+ftd_status ftd_solve_f32c(ftd_handle* h, const float* A, const float* B, const float* C, double* D) {
+    return solve_f32_common(h, A, B, C, D, D, 0);
+}
+
+ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const float* C,
+                         const float* D, double* X) {
+    return solve_f32_common(h, A, B, C, D, X, 1);
 }
 
 const char* ftd_last_path(const ftd_handle* h) { return h ? h->path : "none"; }

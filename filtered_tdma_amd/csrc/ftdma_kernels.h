@@ -20,9 +20,19 @@ hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B
                                double* D, int n_sys, int n_row, int Q, int S, int cyclic,
                                hipStream_t st);
 
+hipError_t launch_pascal_fused_f32c(int M, int maxt, const float* A, const float* B,
+                                    const float* C, double* D, int n_sys, int n_row, int Q, int S,
+                                    int cyclic, hipStream_t st);
+hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float* B, const float* C,
+                                   const float* D, double* X, int n_sys, int n_row, int Q, int S,
+                                   int cyclic, hipStream_t st);
+
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
                                int n_sys, int n_row, hipStream_t st);
+
+hipError_t launch_thomas_cyclic_fused(const double* A, const double* B, const double* C, double* D,
+                                      int n_sys, int n_row, hipStream_t st);
 
 int filtered_fused_supported(int M, int maxt);
 size_t filtered_fused_lds(int P, int S);

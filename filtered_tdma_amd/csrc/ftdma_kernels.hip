@@ -147,15 +147,16 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
 // Row loads of the fused kernels: row j of the lane's partition, clamped to
 // the partition's last row so that the load never depends on a branch (the
 // compiler may then issue it PF rows ahead of its use). Rows past the count
-// reload the last row, which is in cache, and are never used.
+// reload the last row, which is in cache, and are never used. Values are
+// widened to double on load (exact for float storage).
 #define FTD_LOAD_ROW(j)                                                      \
     do {                                                                     \
         const int jj_ = (j) < cnt ? (j) : cnt - 1;                           \
         const size_t o_ = base + (size_t)jj_ * ld;                           \
-        ra[(j)] = A[o_];                                                     \
-        rb[(j)] = B[o_];                                                     \
-        rc[(j)] = C[o_];                                                     \
-        rd[(j)] = D[o_];                                                     \
+        ra[(j)] = (double)A[o_];                                             \
+        rb[(j)] = (double)B[o_];                                             \
+        rc[(j)] = (double)C[o_];                                             \
+        rd[(j)] = (double)Din[o_];                                           \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__
     if (i_raw >= n_sys) return;
     const int i = i_raw;
     const int cnt = n_row;
+    const double* Din = D;
     const size_t ld = (size_t)n_sys;
     const size_t base = (size_t)i;
     double c[M], d[M];
@@ -202,18 +204,85 @@ __global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Sherman-Morrison periodic Thomas with the whole system in registers
+// (n_row <= M): the reference's single-rank periodic path
+// (tdma_cyclic_many_kernel, tdma_local_cuda.cu:67-128).
+// ---------------------------------------------------------------------------
+template <int M, int PF>
+__global__ __launch_bounds__(256) void k_thomas_cyclic_fused(const double* __restrict__ A,
+                                                             const double* __restrict__ B,
+                                                             const double* __restrict__ C,
+                                                             double* __restrict__ D, int n_sys,
+                                                             int n_row) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sys) return;
+    const int cnt = n_row;
+    const double* Din = D;
+    const size_t ld = (size_t)n_sys;
+    const size_t base = (size_t)i;
+    double c[M], d[M], e[M];
+    double ra[M], rb[M], rc[M], rd[M];
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
+    double a0 = 0.0, b0 = 0.0, c0 = 0.0, d0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_LOAD_ROW(j + PF);
+        if (j < cnt) {
+            if (j == 0) {
+                a0 = ra[0];
+                b0 = rb[0];
+                c0 = rc[0];
+                d0 = rd[0];
+            } else if (j == 1) {
+                const double inv_b = 1.0 / rb[1];
+                // E[1] = -A[1] (E[n-1] = -C[n-1] joins at the last row)
+                d[1] = rd[1] * inv_b;
+                e[1] = (cnt - 1 == 1 ? -rc[1] : -ra[1]) * inv_b;
+                c[1] = rc[1] * inv_b;
+            } else {
+                const double r = 1.0 / (rb[j] - ra[j] * c[j - 1]);
+                const double ej = (j == cnt - 1) ? -rc[j] : 0.0;
+                d[j] = r * (rd[j] - ra[j] * d[j - 1]);
+                e[j] = r * (ej - ra[j] * e[j - 1]);
+                c[j] = r * rc[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = M - 2; j >= 1; --j) {
+        if (j <= cnt - 2) {
+            d[j] = d[j] - c[j] * d[j + 1];
+            e[j] = e[j] - c[j] * e[j + 1];
+        }
+    }
+    double dN = 0.0, eN = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j == cnt - 1) {
+            dN = d[j];
+            eN = e[j];
+        }
+    const double x0 = (d0 - a0 * dN - c0 * d[1]) / (b0 + a0 * eN + c0 * e[1]);
+    D[base] = x0;
+#pragma unroll
+    for (int j = 1; j < M; ++j)
+        if (j < cnt) D[base + (size_t)j * ld] = d[j] + x0 * e[j];
+}
+
+// ---------------------------------------------------------------------------
 // fused PaScaL: modified Thomas per (system, sub-partition) in registers,
 // reduced 2Q-row system in LDS, interior update, single store of x.
 // Restates modified_thomas_kernel / tdma_many_kernel / tdma_cyclic_many_kernel
 // / update_solution_kernel (libs/pascal_tdma/gpu/tdma_local_cuda.cu:67-224)
 // for Q partitions of one GPU.
 // ---------------------------------------------------------------------------
-template <int M, int MAXT, bool CYCLIC, int PF, int WPE>
-__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_fused(const double* __restrict__ A,
-                                                       const double* __restrict__ B,
-                                                       const double* __restrict__ C,
-                                                       double* __restrict__ D, int n_sys,
-                                                       int n_row, int Q, int lgS) {
+// TC: storage type of A, B, C; TD: of the right-hand side Din. The solution
+// goes to X (float64), which may be Din itself when TD is double.
+template <int M, int MAXT, bool CYCLIC, int PF, int WPE, typename TC, typename TD>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_fused(
+    const TC* __restrict__ A, const TC* __restrict__ B, const TC* __restrict__ C, const TD* Din,
+    double* X, int n_sys, int n_row, int Q, int lgS) {
     extern __shared__ double lds[];
     const int S = 1 << lgS;
     const int t = threadIdx.x;
@@ -333,7 +402,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
                     x = xN;
                 else
                     x = d[j] - (a[j] * x0 + c[j] * xN);
-                D[base + (size_t)j * ld] = x;
+                X[base + (size_t)j * ld] = x;
             }
         }
     }
@@ -376,6 +445,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     double aN = 0.0, cN = 0.0, dN = 0.0;
     double b1 = 1.0;
     const int cnt = m;
+    const double* Din = D;
 
 #pragma unroll
     for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
@@ -929,25 +999,47 @@ size_t pascal_fused_lds(int Q, int S, int cyclic) {
     return sizeof(double) * (size_t)(cyclic ? 4 : 3) * (size_t)(2 * Q) * (size_t)S;
 }
 
-hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B, const double* C,
-                               double* D, int n_sys, int n_row, int Q, int S, int cyclic,
-                               hipStream_t st) {
+template <typename TC, typename TD>
+static hipError_t launch_pascal_fused_t(int M, int maxt, const TC* A, const TC* B, const TC* C,
+                                        const TD* Din, double* Xout, int n_sys, int n_row, int Q,
+                                        int S, int cyclic, hipStream_t st) {
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * Q);
     const size_t lds = pascal_fused_lds(Q, S, cyclic);
-#define X(m_, t_, pf_, w_)                                                                 \
-    if (M == m_ && maxt == t_) {                                                           \
-        if (cyclic)                                                                        \
-            hipLaunchKernelGGL((k_pascal_fused<m_, t_, true, pf_, w_>), grid, block, lds,  \
-                               st, A, B, C, D, n_sys, n_row, Q, lgS);                      \
-        else                                                                               \
-            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false, pf_, w_>), grid, block, lds, \
-                               st, A, B, C, D, n_sys, n_row, Q, lgS);                      \
-        return hipGetLastError();                                                          \
+#define X(m_, t_, pf_, w_)                                                                  \
+    if (M == m_ && maxt == t_) {                                                            \
+        if (cyclic)                                                                         \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, true, pf_, w_, TC, TD>), grid, block, \
+                               lds, st, A, B, C, Din, Xout, n_sys, n_row, Q, lgS);          \
+        else                                                                                \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false, pf_, w_, TC, TD>), grid,      \
+                               block, lds, st, A, B, C, Din, Xout, n_sys, n_row, Q, lgS);   \
+        return hipGetLastError();                                                           \
     }
     FTD_PASCAL_VARIANTS(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B, const double* C,
+                               double* D, int n_sys, int n_row, int Q, int S, int cyclic,
+                               hipStream_t st) {
+    return launch_pascal_fused_t<double, double>(M, maxt, A, B, C, D, D, n_sys, n_row, Q, S,
+                                                 cyclic, st);
+}
+
+hipError_t launch_pascal_fused_f32c(int M, int maxt, const float* A, const float* B,
+                                    const float* C, double* D, int n_sys, int n_row, int Q, int S,
+                                    int cyclic, hipStream_t st) {
+    return launch_pascal_fused_t<float, double>(M, maxt, A, B, C, D, D, n_sys, n_row, Q, S,
+                                                cyclic, st);
+}
+
+hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float* B, const float* C,
+                                   const float* D, double* X, int n_sys, int n_row, int Q, int S,
+                                   int cyclic, hipStream_t st) {
+    return launch_pascal_fused_t<float, float>(M, maxt, A, B, C, D, X, n_sys, n_row, Q, S, cyclic,
+                                               st);
 }
 
 int thomas_fused_rows(int n_row) {
@@ -962,6 +1054,19 @@ hipError_t launch_thomas_fused(const double* A, const double* B, const double* C
         case 16: hipLaunchKernelGGL((k_thomas_fused<16, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
         case 32: hipLaunchKernelGGL((k_thomas_fused<32, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
         case 64: hipLaunchKernelGGL((k_thomas_fused<64, 4>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_thomas_cyclic_fused(const double* A, const double* B, const double* C, double* D,
+                                      int n_sys, int n_row, hipStream_t st) {
+    const dim3 grid((n_sys + 255) / 256), block(256);
+    switch (thomas_fused_rows(n_row)) {
+        case 8: hipLaunchKernelGGL((k_thomas_cyclic_fused<8, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 16: hipLaunchKernelGGL((k_thomas_cyclic_fused<16, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 32: hipLaunchKernelGGL((k_thomas_cyclic_fused<32, 4>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 64: hipLaunchKernelGGL((k_thomas_cyclic_fused<64, 2>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -98,6 +98,19 @@ ftd_status ftd_set_stream(ftd_handle* h, void* stream);
  * (tdma_backend_gpu.cu:35-46). Off by default (no host synchronisation). */
 ftd_status ftd_set_profiling(ftd_handle* h, int enable);
 
+/* Extension without a reference counterpart (FP32 storage): A, B, C are
+ * float [n_row x n_sys], D is float64 and receives x in place. The
+ * elimination runs in FP64 on the exactly widened coefficients, so x equals
+ * the FP64 PaScaL solve of the stored system; HBM traffic is 28 B/row
+ * instead of 40. Exact kinds only (PASCAL, or one partition). */
+ftd_status ftd_solve_f32c(ftd_handle* h, const float* A, const float* B, const float* C, double* D);
+
+/* Extension (FP32 storage of coefficients and right-hand side): A, B, C, D
+ * are float, x (float64) goes to X; 24 B/row. Same numerics as
+ * ftd_solve_f32c with D widened. */
+ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const float* C,
+                         const float* D, double* X);
+
 /* FilteredTDMACUDA::last_J (filtered_tdma_cuda.cu:83-87), one value per
  * partition: J of the most recent filtered solve (blocking device->host
  * copy of n <= partitions ints). -1 where no filtered solve ran yet. */

@@ -36,7 +36,14 @@ def golden():
 @pytest.mark.parametrize("case", golden(),
                          ids=lambda c: f"{c['kind']}-n{c['n']}-r{c['rho']}-"
                                        f"{'x'.join(map(str, c['parts']))}-per{c['periodic']}")
-def test_heat_gpu_matches_reference_l2_and_oracle(case):
+def test_heat_gpu_matches_oracle_and_reference_l2(case):
+    """Field bit-for-bit equal to the oracle's GPU-semantics restatement
+    (every sweep runs the reference's partitioning). The golden L2 was
+    printed by the reference's CPU heat_cpu: PaScaL is exact, so its L2 text
+    must match too; the filtered kinds choose J differently on the CPU and
+    the GPU (filtered_tdma.cpp:63-115 vs filtered_tdma_kernels.cuh:220-261),
+    so for them the CPU-semantics oracle is what matches heat_cpu (checked
+    on the CPU) and the GPU is held to the GPU-semantics oracle."""
     n, parts, kind = case["n"], tuple(case["parts"]), case["kind"]
     per = (1, 1, 1) if case["periodic"] else (0, 0, 0)
     dt, nt = dt_policy("order", case["rho"], n)
@@ -48,15 +55,13 @@ def test_heat_gpu_matches_reference_l2_and_oracle(case):
         field = sim.field()
     finally:
         sim.close()
-    assert "%g" % l2 == case["l2"]
     l2o, ref = O.heat_run(n, n, n, parts, kind, dt, nt, periodic=per, sem=O.SEM_GPU,
                           return_theta=True)
     inner = (slice(1, -1),) * 3
-    if kind != "pascal" or max(parts) == 1:
-        # same partitioning in every sweep: same statements, same rounding
-        np.testing.assert_array_equal(field[inner], ref[inner])
-    rel = np.max(np.abs(field[inner] - ref[inner])) / np.max(np.abs(ref[inner]))
-    assert rel <= 1e-13
+    np.testing.assert_array_equal(field[inner], ref[inner])
+    assert l2 == l2o
+    if kind == "pascal":
+        assert "%g" % l2 == case["l2"]
 
 
 @pytest.mark.parametrize("kind,parts,periodic", [
@@ -100,7 +105,7 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
         finally:
             sim.close()
         l2o = O.heat_run(n, n, n, parts, kind, dt, steps, sem=O.SEM_GPU)
-        assert abs(res[(kind, parts)][0] - l2o) <= 1e-9 * l2o
+        assert res[(kind, parts)][0] == l2o
     thomas = res[("pascal", (1, 1, 1))][1]
     inner = (slice(1, -1),) * 3
     for key, (l2, f) in res.items():

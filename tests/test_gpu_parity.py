@@ -82,15 +82,16 @@ PASCAL_SHAPES = [
 @pytest.mark.parametrize("cyclic", [False, True])
 @pytest.mark.parametrize("n_sys,n_row,tune", PASCAL_SHAPES)
 def test_pascal_fused_bitwise(n_sys, n_row, tune, cyclic):
-    if cyclic and n_row < 6:
-        pytest.skip("periodic fused solve needs two sub-partitions of >= 3 rows")
+    if cyclic and n_row < 3:
+        pytest.skip("a periodic system needs at least 3 rows")
     note(f"pascal_fused n_sys={n_sys} n_row={n_row} tune={tune} cyclic={cyclic}")
     A, B, C, D = cases.general_batch(n_sys, n_row, seed=11)
     x, _, path, Bafter = run(Kind.PASCAL, A, B, C, D, P=1, cyclic=cyclic, tuning=tune)
-    if path.startswith("thomas_fused"):
-        # a system that fits one lane: plain Thomas, the reference's P == 1 path
-        assert not cyclic and n_row <= 64
-        ref = O.thomas(A, B, C, D, sem=O.SEM_GPU)
+    if path.startswith("thomas"):
+        # a system that fits one lane: plain or periodic Thomas, the
+        # reference's single-rank path
+        assert n_row <= 64
+        ref = O.thomas(A, B, C, D, sem=O.SEM_GPU, cyclic=cyclic)
     else:
         assert path.startswith("pascal_fused"), path
         Q = int(path.split("Q=")[1])
@@ -132,8 +133,12 @@ def test_solve_nr_other_dims():
         tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
         be.solve_nr(tA, tB, tC, tD, 70, 45)
         torch.cuda.synchronize()
-        Q = int(be.last_path().split("Q=")[1])
-        np.testing.assert_array_equal(tD.cpu().numpy(), O.pascal(A, B, C, D, Q, sem=O.SEM_GPU))
+        path = be.last_path()
+        if path.startswith("thomas_fused"):
+            ref = O.thomas(A, B, C, D, sem=O.SEM_GPU)
+        else:
+            ref = O.pascal(A, B, C, D, int(path.split("Q=")[1]), sem=O.SEM_GPU)
+        np.testing.assert_array_equal(tD.cpu().numpy(), ref)
     finally:
         be.close()
 
@@ -271,3 +276,38 @@ def test_full_size_columns_match_oracle(kind):
                 mb = max(np.max(np.abs(D[s])), np.max(np.abs(D[e])))
                 rh = max(np.max(rho[0][s + 1:e + 1]), np.max(rho[1][s + 1:e + 1]))
                 assert J[p] == O.cal_J(rh, mb, eps, e - s + 1)
+
+
+# ------------------------------------------------------- FP32 storage (C3)
+
+@pytest.mark.parametrize("d32", [False, True])
+@pytest.mark.parametrize("n_sys,n_row,P", [(300, 512, 8), (257, 129, 1), (64, 96, 3)])
+def test_f32_storage_bitwise_vs_oracle_on_widened_data(n_sys, n_row, P, d32):
+    """FP32 A, B, C (and D): the engine eliminates in FP64 on the exactly
+    widened values, so the result equals the FP64 PaScaL of the stored data."""
+    note(f"f32 storage n_sys={n_sys} n_row={n_row} P={P} d32={d32}")
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 5.0, 500.0, seed=12)
+    A32, B32, C32 = (x.astype(np.float32) for x in (A, B, C))
+    D32 = D.astype(np.float32)
+    be = TdmaBackendGPU("pascal", n_sys, n_row, P)
+    try:
+        tA, tB, tC = (torch.from_numpy(x).cuda() for x in (A32, B32, C32))
+        if d32:
+            tD = torch.from_numpy(D32).cuda()
+            tX = torch.empty((n_row, n_sys), dtype=torch.float64, device="cuda")
+            be.solve_f32(tA, tB, tC, tD, tX)
+        else:
+            tX = dev(D)
+            be.solve_f32c(tA, tB, tC, tX)
+        torch.cuda.synchronize()
+        path = be.last_path()
+        x = tX.cpu().numpy()
+    finally:
+        be.close()
+    Q = int(path.split("Q=")[1])
+    Dw = D32.astype(np.float64) if d32 else D
+    ref = O.pascal(A32.astype(np.float64), B32.astype(np.float64), C32.astype(np.float64), Dw, Q,
+                   sem=O.SEM_GPU)
+    np.testing.assert_array_equal(x, ref)
+    assert relerr(x, O.thomas(A32.astype(np.float64), B32.astype(np.float64),
+                              C32.astype(np.float64), Dw)) <= 1e-12
