@@ -75,6 +75,7 @@ struct ftd_handle {
     char path[96] = "none";
     int t_M = 0, t_T = 0, t_S = 0, t_Q = 0;  // forced tuning (0 = automatic)
     int force_general = 0;                   // use the general kernels only
+    int no_stream = 0;                       // automatic choice without the streamed kernels
     double* rho = nullptr;                   // A_rho[n_row+1] then C_rho[n_row+1]
     int* J = nullptr;                        // J per partition
     DevBuf partial, xbuf, R, E;
@@ -184,6 +185,17 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
         return FTD_OK;
     }
     if (!h->force_general && n_row >= 3 && pick_pascal(h, n_row, cyclic, fs)) {
+        // the streamed kernel runs the same partitioning (bit-identical
+        // results) where a variant holds it and the arrays allow its DMA
+        int s_, m_max;
+        part_of(n_row, fs.Q, 0, s_, m_max);
+        const int Ms = (h->t_M || h->no_stream) ? 0 : ftd::stream_rows(m_max, fs.Q);
+        if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
+            std::snprintf(h->path, sizeof h->path, "pascal_stream M=%d Q=%d R=%d", Ms, fs.Q,
+                          ftd::stream_ring_depth(Ms, fs.Q));
+            FTD_HIP(ftd::launch_pascal_stream(Ms, fs.Q, A, B, C, D, n_sys, n_row, cyclic, st));
+            return FTD_OK;
+        }
         std::snprintf(h->path, sizeof h->path, "pascal_fused M=%d T=%d S=%d Q=%d", fs.M, fs.T,
                       fs.S, fs.Q);
         FTD_HIP(ftd::launch_pascal_fused(fs.M, fs.T, A, B, C, D, n_sys, n_row, fs.Q, fs.S, cyclic,
@@ -249,6 +261,32 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
 
     int* Jp = h->J;
     const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
+    const int Ms = (h->force_general || h->t_M || h->no_stream) ? 0 : ftd::stream_rows(mmax, P);
+    if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
+        const int R = ftd::stream_ring_depth(Ms, P);
+        if (version == 2) {
+            int nt = 0;
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * P));
+            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+                                          h->eps_direct, Jp, st));
+            std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d", Ms, P, R);
+            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
+                                                nullptr, nullptr, st));
+        } else {
+            const int tiles = n_sys / 32;
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+            FTD_TRY(ensure(h->xbuf, xbytes));
+            std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d", Ms, P, R);
+            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
+                                                (double*)h->xbuf.p, (double*)h->partial.p, st));
+            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
+                                          h->eps_value, h->eps_direct, Jp, st));
+            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
+                                                (double*)h->xbuf.p, nullptr, st));
+        }
+        return FTD_OK;
+    }
     if (fs.M) {
         const int tiles = (n_sys + fs.S - 1) / fs.S;
         if (version == 2) {
@@ -456,12 +494,14 @@ double ftd_last_comm_ms(const ftd_handle* h) {
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int systems_per_tile,
                           int lanes_per_system) {
     if (!h) return fail(FTD_ERR_INVALID, "null handle");
-    if (rows_per_lane == 0 || rows_per_lane == -1) {
+    if (rows_per_lane == 0 || rows_per_lane == -1 || rows_per_lane == -2) {
         h->t_M = h->t_T = h->t_S = h->t_Q = 0;
         h->force_general = rows_per_lane == -1;
+        h->no_stream = rows_per_lane == -2;
         return FTD_OK;
     }
     h->force_general = 0;
+    h->no_stream = 0;
     if (!ftd::pascal_fused_supported(rows_per_lane, max_threads) &&
         !ftd::filtered_fused_supported(rows_per_lane, max_threads))
         return fail(FTD_ERR_UNSUPPORTED, "no compiled fused kernel with this rows_per_lane/max_threads");

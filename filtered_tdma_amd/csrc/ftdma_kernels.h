@@ -27,6 +27,19 @@ hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float*
                                    const float* D, double* X, int n_sys, int n_row, int Q, int S,
                                    int cyclic, hipStream_t st);
 
+// LDS-streamed persistent kernels (S = 32 systems per tile, two partitions per
+// wave): M rows per partition for Q partitions, or 0 when no variant holds
+// them; eligible when n_sys % 32 == 0 and the arrays are 16-byte aligned
+int stream_rows(int m_max, int Q);
+int stream_ring_depth(int M, int Q);
+bool stream_eligible(int n_sys, const void* A, const void* B, const void* C, const void* D);
+hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, const double* C,
+                                double* D, int n_sys, int n_row, int cyclic, hipStream_t st);
+hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,
+                                  const double* B, const double* C, double* D, int n_sys,
+                                  int n_row, int cyclic, const int* Jp, double* xbuf,
+                                  double* partial, hipStream_t st);
+
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
                                int n_sys, int n_row, hipStream_t st);

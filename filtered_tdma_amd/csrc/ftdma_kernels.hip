@@ -27,6 +27,10 @@
 
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "ftdma_kernels.h"
 
 namespace ftd {
@@ -149,14 +153,27 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
 // compiler may then issue it PF rows ahead of its use). Rows past the count
 // reload the last row, which is in cache, and are never used. Values are
 // widened to double on load (exact for float storage).
+// Streamed operands are touched once: FTD_NONTEMPORAL=1 marks the row loads
+// and the solution stores nontemporal (no reuse expected in the caches).
+#ifndef FTD_NONTEMPORAL
+#define FTD_NONTEMPORAL 1
+#endif
+#if FTD_NONTEMPORAL
+#define FTD_LD(p) __builtin_nontemporal_load(p)
+#define FTD_ST(v, p) __builtin_nontemporal_store((v), (p))
+#else
+#define FTD_LD(p) (*(p))
+#define FTD_ST(v, p) (*(p) = (v))
+#endif
+
 #define FTD_LOAD_ROW(j)                                                      \
     do {                                                                     \
         const int jj_ = (j) < cnt ? (j) : cnt - 1;                           \
         const size_t o_ = base + (size_t)jj_ * ld;                           \
-        ra[(j)] = (double)A[o_];                                             \
-        rb[(j)] = (double)B[o_];                                             \
-        rc[(j)] = (double)C[o_];                                             \
-        rd[(j)] = (double)Din[o_];                                           \
+        ra[(j)] = (double)FTD_LD(A + o_);                                    \
+        rb[(j)] = (double)FTD_LD(B + o_);                                    \
+        rc[(j)] = (double)FTD_LD(C + o_);                                    \
+        rd[(j)] = (double)FTD_LD(Din + o_);                                  \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -200,7 +217,7 @@ __global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__
         if (j <= cnt - 2) d[j] = d[j] - c[j] * d[j + 1];
 #pragma unroll
     for (int j = 0; j < M; ++j)
-        if (j < cnt) D[base + (size_t)j * ld] = d[j];
+        if (j < cnt) FTD_ST(d[j], D + base + (size_t)j * ld);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,10 +281,10 @@ __global__ __launch_bounds__(256) void k_thomas_cyclic_fused(const double* __res
             eN = e[j];
         }
     const double x0 = (d0 - a0 * dN - c0 * d[1]) / (b0 + a0 * eN + c0 * e[1]);
-    D[base] = x0;
+    FTD_ST(x0, D + base);
 #pragma unroll
     for (int j = 1; j < M; ++j)
-        if (j < cnt) D[base + (size_t)j * ld] = d[j] + x0 * e[j];
+        if (j < cnt) FTD_ST(d[j] + x0 * e[j], D + base + (size_t)j * ld);
 }
 
 // ---------------------------------------------------------------------------
@@ -402,10 +419,311 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
                     x = xN;
                 else
                     x = d[j] - (a[j] * x0 + c[j] * xN);
-                X[base + (size_t)j * ld] = x;
+                FTD_ST(x, X + base + (size_t)j * ld);
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-streamed PaScaL (persistent): the arithmetic of k_pascal_fused, with
+// the row operands brought in by direct global->LDS loads
+// (global_load_lds_dwordx4) into a per-wave ring of R row-steps instead of
+// through VGPRs. The ring keeps R rows of every lane in flight without
+// spending registers on them (the M x 3 elimination state already fills the
+// register file at one wave per SIMD), and a block walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... so that the first R rows of the next
+// tile stream in while the current tile does its backward sweep, reduced
+// solve and stores.
+//
+// Shape: S = 32 systems per tile, Q partitions, two partitions per wave
+// (lanes 0..31 -> partition 2w, lanes 32..63 -> 2w+1), Q/2 waves per block.
+// Requirements checked by the launcher: n_sys % 32 == 0 (no partial tiles),
+// 16-byte aligned arrays, every partition has 2 <= m_p <= M rows.
+//
+// Ring slot layout (one wave, one row-step, 2 KB): two 1 KB blocks, each
+// filled by one dwordx4 DMA of the wave (lane l -> bytes 16l..16l+15):
+//   block 0: [A of 2w][A of 2w+1][B of 2w][B of 2w+1]  (32 doubles each)
+//   block 1: [C of 2w][C of 2w+1][D of 2w][D of 2w+1]
+// so the reading lane l finds a, b, c, d at doubles l, 64 + l of the blocks.
+// Each wave loads exactly the rows it consumes: the ring needs no barrier,
+// only the wave's own vmcnt.
+//
+// vmcnt bookkeeping (in-order completion of a wave's vector memory ops):
+// per row-step two DMA ops; per tile exactly M stores (rows past m_p store
+// the last row's value again, so the count does not depend on m_p). At the
+// wait for row j the ops issued after that row's DMAs are
+//   j >= R                : the DMAs of the next R-1 row-steps  -> 2(R-1)
+//   j <  R, first tile    : the same                           -> 2(R-1)
+//   j <  R, later tiles   : 2(R-1) DMAs and the M stores of the previous
+//                           tile                               -> 2(R-1)+M
+// (capped at 63, the counter's range: a smaller count only waits longer).
+// The last tile re-fetches its own rows into consumed slots so that the
+// count stays uniform; the kernel drains vmcnt before it ends.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef const __attribute__((address_space(3))) double* lds_cdptr_t;
+typedef double ftd_dbl2 __attribute__((ext_vector_type(2)));
+
+// The four operands of one row-step from a ring slot (a, b at bytes 0 and
+// 512 of block 0, c, d at 1024 and 1536), waited for before returning, so the
+// slot may be refilled right after. Written as inline asm so that the
+// compiler's own wait insertion (which cannot tell ring slots apart and would
+// drain every DMA in flight) stays out of the ring; the waits the data needs
+// are the explicit vmcnt waits above.
+__device__ __forceinline__ void ring_read(lds_cdptr_t p, double& a, double& b, double& c,
+                                          double& d) {
+    ftd_dbl2 ab, cd;
+    asm volatile(
+        "ds_read2st64_b64 %0, %2 offset1:1\n\t"
+        "ds_read2st64_b64 %1, %2 offset0:2 offset1:3\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(ab), "=&v"(cd)
+        : "v"(p)
+        : "memory");
+    a = ab.x;
+    b = ab.y;
+    c = cd.x;
+    d = cd.y;
+}
+
+// x0 and xN of lane l from a 1 KB block holding four 32-double rows
+// [x0 of 2w][xN of 2w][x0 of 2w+1][xN of 2w+1] (p points at x0 of the lane)
+__device__ __forceinline__ void xin_read(lds_cdptr_t p, double& x0, double& xN) {
+    ftd_dbl2 v;
+    asm volatile(
+        "ds_read2_b64 %0, %1 offset1:32\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v)
+        : "v"(p)
+        : "memory");
+    x0 = v.x;
+    xN = v.y;
+}
+
+// workgroup barrier for LDS traffic only: the LDS writes are complete, the
+// DMAs in flight are left alone (__syncthreads would wait for them as well)
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+#ifndef FTD_DMA_AUX
+#define FTD_DMA_AUX 0
+#endif
+
+// The ring of one wave and its bookkeeping. NST: vector memory ops the tail
+// of a tile issues (all lanes together count once per instruction).
+template <int M, int Q, int R, int NST>
+struct StreamRing {
+    static constexpr int S = 32;
+    double (*slots)[2][128];  // this wave's R slots
+    const double* src0;       // DMA sources of this lane (row 0 of its sub-block)
+    const double* src1;
+    size_t ld;
+
+    __device__ __forceinline__ void init(double (*wave_slots)[2][128], const double* A,
+                                         const double* B, const double* C, const double* D,
+                                         int n_sys, int n_row, int w, int l, int& dcnt) {
+        slots = wave_slots;
+        ld = (size_t)n_sys;
+        const int g = l >> 4;
+        int dr0;
+        part_of(n_row, Q, 2 * w + (g & 1), dr0, dcnt);
+        src0 = (g < 2 ? A : B) + (size_t)dr0 * ld + 2 * (l & 15);
+        src1 = (g < 2 ? C : D) + (size_t)dr0 * ld + 2 * (l & 15);
+    }
+    // row j (clamped to the sub-block's last row) of tile T into slot
+    __device__ __forceinline__ void dma(int T, int j, int slot, int dcnt) const {
+        const int jj = j < dcnt ? j : dcnt - 1;
+        const size_t o = (size_t)jj * ld + (size_t)T * S;
+        __builtin_amdgcn_global_load_lds((const void*)(src0 + o), (lds_ptr_t)&slots[slot][0][0], 16,
+                                         0, FTD_DMA_AUX);
+        __builtin_amdgcn_global_load_lds((const void*)(src1 + o), (lds_ptr_t)&slots[slot][1][0], 16,
+                                         0, FTD_DMA_AUX);
+    }
+    __device__ __forceinline__ void prologue(int tile, int dcnt) const {
+#pragma unroll
+        for (int j = 0; j < R; ++j) dma(tile, j, j, dcnt);
+    }
+    // operands of row j of the current tile; refills the slot with row j + R
+    // (of this tile, else of tile `ntile`)
+    __device__ __forceinline__ void take(int j, bool first, int tile, int ntile, int dcnt, int l,
+                                         double& a, double& b, double& c, double& d) const {
+        constexpr int steady = 2 * (R - 1);
+        constexpr int after_tail = steady + NST < 63 ? steady + NST : 63;
+        if (j >= R || first)
+            wait_vmcnt<steady>();
+        else
+            wait_vmcnt<after_tail>();
+        ring_read((lds_cdptr_t)&slots[j % R][0][l], a, b, c, d);
+        if (j + R < M)
+            dma(tile, j + R, j % R, dcnt);
+        else
+            dma(ntile, j + R - M, j % R, dcnt);
+    }
+};
+
+template <int M, int Q, int R, bool CYCLIC, int WPE>
+__global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_stream(
+    const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
+    double* D, int n_sys, int n_row, int n_tiles) {
+    constexpr int S = 32, NW = Q / 2, NR = 2 * Q;
+    static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
+    __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
+    __shared__ double red[CYCLIC ? 4 : 3][NR][S];
+
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int l = t & 63;
+    const int q = 2 * w + (l >> 5);
+    const int sl = l & 31;
+    const size_t ld = (size_t)n_sys;
+    StreamRing<M, Q, R, M> rg;
+    int dcnt0;
+    rg.init(ring[w], A, B, C, D, n_sys, n_row, w, l, dcnt0);
+    int r0, cnt0;
+    part_of(n_row, Q, q, r0, cnt0);
+
+    int tile = blockIdx.x;
+    rg.prologue(tile, dcnt0);
+    bool first = true;
+    for (; tile < n_tiles; tile += gridDim.x) {
+        // opaque per tile: keeps the row clamps from being hoisted out of the
+        // tile loop as M live values
+        int cnt = cnt0, dcnt = dcnt0;
+        asm volatile("" : "+v"(cnt), "+v"(dcnt));
+        const int next = tile + (int)gridDim.x;
+        const int ntile = next < n_tiles ? next : tile;
+        const size_t base = (size_t)r0 * ld + (size_t)tile * S + sl;
+
+        double a[M], c[M], d[M];
+        double aN = 0.0, cN = 0.0, dN = 0.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double aj, bj, cj, dj;
+            rg.take(j, first, tile, ntile, dcnt, l, aj, bj, cj, dj);
+            if (j < cnt) {
+                if (j < 2) {
+                    a[j] = aj / bj;
+                    c[j] = cj / bj;
+                    d[j] = dj / bj;
+                } else {
+                    const double inv = 1.0 / (bj - aj * c[j - 1]);
+                    d[j] = inv * (dj - aj * d[j - 1]);
+                    c[j] = inv * cj;
+                    a[j] = -inv * aj * a[j - 1];
+                }
+                if (j == cnt - 1) {
+                    aN = a[j];
+                    cN = c[j];
+                    dN = d[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = M - 3; j >= 1; --j) {
+            if (j <= cnt - 3) {
+                const double cj = c[j];
+                d[j] = d[j] - cj * d[j + 1];
+                a[j] = a[j] - cj * a[j + 1];
+                c[j] = -cj * c[j + 1];
+            }
+        }
+        const double r = 1.0 / (1.0 - a[1] * c[0]);
+        const double d0 = r * (d[0] - c[0] * d[1]);
+        const double a0 = r * a[0];
+        const double c0 = -r * c[0] * c[1];
+
+        lds_barrier();  // the previous tile's readers of red[] are done
+        red[0][2 * q][sl] = a0;
+        red[1][2 * q][sl] = c0;
+        red[2][2 * q][sl] = d0;
+        red[0][2 * q + 1][sl] = aN;
+        red[1][2 * q + 1][sl] = cN;
+        red[2][2 * q + 1][sl] = dN;
+        lds_barrier();
+        if (q == 0) {
+            // reduced system (B == 1), one lane per system; the running
+            // c, d (and e) stay in registers, the eliminated rows go back to
+            // LDS for the backward pass
+            double* Ar = &red[0][0][sl];
+            double* Cr = &red[1][0][sl];
+            double* Dr = &red[2][0][sl];
+            if (!CYCLIC) {
+                double cp = Cr[0], dp = Dr[0];
+#pragma unroll 2
+                for (int k = 1; k < NR; ++k) {
+                    const double ak = Ar[k * S];
+                    const double inv = 1.0 / (1.0 - ak * cp);
+                    dp = inv * (Dr[k * S] - ak * dp);
+                    cp = inv * Cr[k * S];
+                    Dr[k * S] = dp;
+                    Cr[k * S] = cp;
+                }
+                double dn = dp;
+#pragma unroll 2
+                for (int k = NR - 2; k >= 0; --k) {
+                    dn = Dr[k * S] - Cr[k * S] * dn;
+                    Dr[k * S] = dn;
+                }
+            } else {
+                double* Er = &red[3][0][sl];
+                const double e_last = -Cr[(NR - 1) * S];
+                double cp = Cr[1 * S], dp = Dr[1 * S], ep = -Ar[1 * S];
+                Er[1 * S] = ep;
+#pragma unroll 2
+                for (int k = 2; k < NR; ++k) {
+                    const double ak = Ar[k * S];
+                    const double inv = 1.0 / (1.0 - ak * cp);
+                    const double ek = (k == NR - 1) ? e_last : 0.0;
+                    dp = inv * (Dr[k * S] - ak * dp);
+                    ep = inv * (ek - ak * ep);
+                    cp = inv * Cr[k * S];
+                    Dr[k * S] = dp;
+                    Er[k * S] = ep;
+                    Cr[k * S] = cp;
+                }
+                double dn = dp, en = ep;
+#pragma unroll 2
+                for (int k = NR - 2; k >= 1; --k) {
+                    const double ck = Cr[k * S];
+                    dn = Dr[k * S] - ck * dn;
+                    en = Er[k * S] - ck * en;
+                    Dr[k * S] = dn;
+                    Er[k * S] = en;
+                }
+                const double x0r = (Dr[0] - Ar[0] * dp - Cr[0] * Dr[1 * S]) /
+                                   (1.0 + Ar[0] * ep + Cr[0] * Er[1 * S]);
+                Dr[0] = x0r;
+#pragma unroll 2
+                for (int k = 1; k < NR; ++k) Dr[k * S] = Dr[k * S] + x0r * Er[k * S];
+            }
+        }
+        lds_barrier();
+        const double x0 = red[2][2 * q][sl];
+        const double xN = red[2][2 * q + 1][sl];
+        // exactly M stores: rows past m_p store row m_p - 1 (x_N) again
+        double* const xlast = D + base + (size_t)(cnt - 1) * ld;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double x;
+            if (j == 0)
+                x = x0;
+            else if (j >= cnt - 1)
+                x = xN;
+            else
+                x = d[j] - (a[j] * x0 + c[j] * xN);
+            FTD_ST(x, j < cnt - 1 ? D + base + (size_t)j * ld : xlast);
+        }
+        first = false;
+    }
+    wait_vmcnt<0>();  // no DMA into LDS outlives the wave
 }
 
 // ---------------------------------------------------------------------------
@@ -574,10 +892,174 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
                     if (j <= jL) x = x - a[j] * x0;
                     if (j >= jR) x = x - c[j] * xN;
                 }
-                D[base + (size_t)j * ld] = x;
+                FTD_ST(x, D + base + (size_t)j * ld);
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-streamed Filtered v1/v2 (persistent): the arithmetic of
+// k_filtered_fused over the StreamRing of k_pascal_stream. Same shape
+// requirements (S = 32, two partitions per wave, n_sys % 32 == 0).
+//   PASS 0: v2 -> x stored (M stores per tile).
+//   PASS 1: v1 first pass -> x0, xN to xbuf, per-tile max to partial
+//           (3 stores per tile).
+//   PASS 2: v1 second pass -> x0, xN from xbuf (one more DMA per tile into
+//           an LDS block, issued ahead of the sweep), x stored (M stores per
+//           tile).
+// ---------------------------------------------------------------------------
+template <int M, int Q, int R, int VERSION, int PASS, int WPE>
+__global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_stream(
+    const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
+    double* D, int n_sys, int n_row, int n_tiles, int cyclic, const int* __restrict__ Jp,
+    double* __restrict__ xbuf, double* __restrict__ partial) {
+    constexpr int S = 32, NW = Q / 2;
+    constexpr int NST = PASS == 1 ? 3 : M;
+    static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
+    __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
+    __shared__ double ex[4][Q][S];  // C, D of row m-1; A, D of row 0 (packed)
+    __shared__ __attribute__((aligned(16))) double xin[PASS == 2 ? NW : 1][128];
+
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int l = t & 63;
+    const int p = 2 * w + (l >> 5);
+    const int sl = l & 31;
+    const size_t ld = (size_t)n_sys;
+    int r0, m0;
+    part_of(n_row, Q, p, r0, m0);
+    int J0 = (VERSION == 2 || PASS == 2) ? Jp[p] : m0;
+    asm volatile("" : "+v"(J0));  // J is loaded before the ring starts
+    // PASS 2: rows 4w..4w+3 of xbuf (x0, xN of partitions 2w, 2w+1), two
+    // systems per lane, as the ring's sub-blocks
+    const double* xsrc = xbuf + (size_t)(4 * w + (l >> 4)) * ld + 2 * (l & 15);
+    StreamRing<M, Q, R, NST> rg;
+    int dcnt0;
+    rg.init(ring[w], A, B, C, D, n_sys, n_row, w, l, dcnt0);
+
+    int tile = blockIdx.x;
+    rg.prologue(tile, dcnt0);
+    bool first = true;
+    for (; tile < n_tiles; tile += gridDim.x) {
+        int m = m0, dcnt = dcnt0, J = J0;
+        asm volatile("" : "+v"(m), "+v"(dcnt), "+v"(J));
+        const int next = tile + (int)gridDim.x;
+        const int ntile = next < n_tiles ? next : tile;
+        const int i = tile * S + sl;
+        const size_t base = (size_t)r0 * ld + (size_t)i;
+        double x0 = 0.0, xN = 0.0;
+        if (PASS == 2)
+            __builtin_amdgcn_global_load_lds((const void*)(xsrc + (size_t)tile * S),
+                                             (lds_ptr_t)&xin[w][0], 16, 0, FTD_DMA_AUX);
+
+        double a[M], c[M], d[M];
+        double aN = 0.0, cN = 0.0, dN = 0.0;
+        double b1 = 1.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double aj, bj, cj, dj;
+            rg.take(j, first, tile, ntile, dcnt, l, aj, bj, cj, dj);
+            if (j < m) {
+                if (j < 2) {
+                    const double r = 1.0 / bj;
+                    a[j] = aj * r;
+                    c[j] = cj * r;
+                    d[j] = dj * r;
+                    if (j == 1) b1 = bj;
+                } else if (VERSION == 1 || j <= J) {
+                    const double inv = 1.0 / (bj - aj * c[j - 1]);
+                    d[j] = inv * (dj - aj * d[j - 1]);
+                    c[j] = inv * cj;
+                    a[j] = -inv * aj * a[j - 1];
+                } else {
+                    const double inv = 1.0 / (bj - aj * c[j - 1]);
+                    d[j] = inv * (dj - aj * d[j - 1]);
+                    c[j] = inv * cj;
+                    a[j] = aj;  // left raw, as the skip-A loop leaves A[j]
+                }
+                if (VERSION == 2 && j == 1 && J == 0) {
+                    // k_fwd_pass_v2 with J == 0 eliminates the normalised row 1
+                    // against itself
+                    const double inv = 1.0 / (b1 - a[1] * c[1]);
+                    d[1] = inv * (d[1] - a[1] * d[1]);
+                    c[1] = inv * c[1];
+                }
+                if (j == m - 1) {
+                    aN = a[j];
+                    cN = c[j];
+                    dN = d[j];
+                }
+            }
+        }
+        const int lo = (m - 2) - J;
+#pragma unroll
+        for (int j = M - 3; j >= 1; --j) {
+            if (j <= m - 3) {
+                const double cj = c[j];
+                d[j] = d[j] - cj * d[j + 1];
+                if (VERSION == 1) {
+                    a[j] = a[j] - cj * a[j + 1];
+                    c[j] = -cj * c[j + 1];
+                } else {
+                    if (J >= 2 && j < J) {
+                        const double an = (J == m - 1 && j == m - 3) ? aN : a[j + 1];
+                        a[j] = a[j] - cj * an;
+                    }
+                    if (j > lo) c[j] = -cj * c[j + 1];
+                }
+            }
+        }
+        const double rr = 1.0 / (1.0 - a[1] * c[0]);
+        const double d0p = rr * (d[0] - c[0] * d[1]);
+        const double a0p = rr * a[0];
+
+        if (PASS == 2) {
+            // the 2M row DMAs of this tile were issued after the xbuf DMA
+            wait_vmcnt<(2 * M < 63 ? 2 * M : 63)>();
+            xin_read((lds_cdptr_t)&xin[w][(l >> 5) * 64 + sl], x0, xN);
+        } else {
+            lds_barrier();  // the previous tile's readers of ex[] are done
+            ex[0][p][sl] = cN;
+            ex[1][p][sl] = dN;
+            ex[2][p][sl] = a0p;
+            ex[3][p][sl] = d0p;
+            lds_barrier();
+            const bool has_left = cyclic || p > 0;
+            const bool has_right = cyclic || p < Q - 1;
+            const int pl = (p + Q - 1) % Q, pr = (p + 1) % Q;
+            x0 = d0p;
+            xN = dN;
+            if (has_left) x0 = (d0p - a0p * ex[1][pl][sl]) / (1.0 - ex[0][pl][sl] * a0p);
+            if (has_right) xN = (dN - cN * ex[3][pr][sl]) / (1.0 - ex[2][pr][sl] * cN);
+        }
+        if (PASS == 1) {
+            xbuf[(size_t)(2 * p) * ld + i] = x0;
+            xbuf[(size_t)(2 * p + 1) * ld + i] = xN;
+            const double v = group_max(fmax(fabs(x0), fabs(xN)), S);
+            if (sl == 0) partial[(size_t)tile * Q + p] = v;
+        } else {
+            const int jL = J < m - 2 ? J : m - 2;
+            const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+            double* const xlast = D + base + (size_t)(m - 1) * ld;
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double x;
+                if (j == 0) {
+                    x = x0;
+                } else if (j >= m - 1) {
+                    x = xN;
+                } else {
+                    x = d[j];
+                    if (j <= jL) x = x - a[j] * x0;
+                    if (j >= jR) x = x - c[j] * xN;
+                }
+                FTD_ST(x, j < m - 1 ? D + base + (size_t)j * ld : xlast);
+            }
+        }
+        first = false;
+    }
+    wait_vmcnt<0>();  // no DMA into LDS outlives the wave
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,6 +1595,119 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
         return hipGetLastError();                         \
     }
     FTD_FILTERED_VARIANTS(X)
+#undef X
+#undef LAUNCH
+    return hipErrorInvalidValue;
+}
+
+// -------- streamed (LDS ring, persistent) dispatch --------
+
+#ifndef FTD_STREAM_VARIANTS
+// X(M rows per partition, Q partitions, R ring depth, waves per SIMD)
+#define FTD_STREAM_VARIANTS(X) \
+    X(64, 8, 16, 1)            \
+    X(32, 8, 8, 2)             \
+    X(64, 4, 16, 1)            \
+    X(32, 4, 8, 2)             \
+    X(32, 16, 8, 2)
+#endif
+
+int stream_rows(int m_max, int Q) {
+    int best = 0;
+#define X(m_, q_, r_, w_) \
+    if (Q == q_ && m_max <= m_ && (best == 0 || m_ < best)) best = m_;
+    FTD_STREAM_VARIANTS(X)
+#undef X
+    return best;
+}
+
+int stream_ring_depth(int M, int Q) {
+#define X(m_, q_, r_, w_) \
+    if (M == m_ && Q == q_) return r_;
+    FTD_STREAM_VARIANTS(X)
+#undef X
+    return 0;
+}
+
+bool stream_eligible(int n_sys, const void* A, const void* B, const void* C, const void* D) {
+    const uintptr_t mis = ((uintptr_t)A | (uintptr_t)B | (uintptr_t)C | (uintptr_t)D) & 15u;
+    return n_sys > 0 && n_sys % 32 == 0 && mis == 0;
+}
+
+// Workgroups of `kernel` resident on the whole device at once: the
+// persistent grid. Cached per (kernel, device).
+static int resident_grid(const void* kernel, int threads, int n_tiles) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int total = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({kernel, dev});
+        if (it != cache.end()) total = it->second;
+    }
+    if (!total) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus < 1)
+            cus = 1;
+        total = per_cu * cus;
+        std::lock_guard<std::mutex> lock(mu);
+        cache[{kernel, dev}] = total;
+    }
+    return total < n_tiles ? total : n_tiles;
+}
+
+hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, const double* C,
+                                double* D, int n_sys, int n_row, int cyclic, hipStream_t st) {
+    if (!stream_eligible(n_sys, A, B, C, D)) return hipErrorInvalidValue;
+    const int n_tiles = n_sys / 32;
+#define X(m_, q_, r_, w_)                                                                    \
+    if (M == m_ && Q == q_) {                                                                \
+        const void* k = cyclic ? (const void*)k_pascal_stream<m_, q_, r_, true, w_>          \
+                               : (const void*)k_pascal_stream<m_, q_, r_, false, w_>;        \
+        const dim3 grid(resident_grid(k, q_ * 32, n_tiles)), block(q_ * 32);                 \
+        if (cyclic)                                                                          \
+            hipLaunchKernelGGL((k_pascal_stream<m_, q_, r_, true, w_>), grid, block, 0, st, A, \
+                               B, C, D, n_sys, n_row, n_tiles);                              \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_pascal_stream<m_, q_, r_, false, w_>), grid, block, 0, st,   \
+                               A, B, C, D, n_sys, n_row, n_tiles);                           \
+        return hipGetLastError();                                                            \
+    }
+    FTD_STREAM_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,
+                                  const double* B, const double* C, double* D, int n_sys,
+                                  int n_row, int cyclic, const int* Jp, double* xbuf,
+                                  double* partial, hipStream_t st) {
+    if (!stream_eligible(n_sys, A, B, C, D)) return hipErrorInvalidValue;
+    const int n_tiles = n_sys / 32;
+#define LAUNCH(m_, q_, r_, w_, v_, p_)                                                        \
+    do {                                                                                     \
+        const void* k = (const void*)k_filtered_stream<m_, q_, r_, v_, p_, w_>;              \
+        const dim3 grid(resident_grid(k, q_ * 32, n_tiles)), block(q_ * 32);                 \
+        hipLaunchKernelGGL((k_filtered_stream<m_, q_, r_, v_, p_, w_>), grid, block, 0, st, A, \
+                           B, C, D, n_sys, n_row, n_tiles, cyclic, Jp, xbuf, partial);       \
+    } while (0)
+#define X(m_, q_, r_, w_)                         \
+    if (M == m_ && Q == q_) {                     \
+        if (version == 2)                         \
+            LAUNCH(m_, q_, r_, w_, 2, 0);         \
+        else if (pass == 1)                       \
+            LAUNCH(m_, q_, r_, w_, 1, 1);         \
+        else                                      \
+            LAUNCH(m_, q_, r_, w_, 1, 2);         \
+        return hipGetLastError();                 \
+    }
+    FTD_STREAM_VARIANTS(X)
 #undef X
 #undef LAUNCH
     return hipErrorInvalidValue;

@@ -124,19 +124,22 @@ double ftd_last_gpu_ms(const ftd_handle* h);
  * the partitions of one GPU exchange nothing over a network. */
 double ftd_last_comm_ms(const ftd_handle* h);
 
-/* Extension (autotuner hook): force the fused-kernel shape of later solves.
- * rows_per_lane in {8,16,32,64} (0 = automatic), max_threads in
- * {256,512}, systems_per_tile a power of two <= 64 (0 = automatic),
- * lanes_per_system (PaScaL sub-partitions, 0 = automatic). rows_per_lane
- * == -1 selects the general kernels (partitions walked in global memory,
- * in place) for every later solve. Returns
- * FTD_ERR_UNSUPPORTED for combinations that are not compiled. */
+/* Extension (autotuner hook): force the register-resident fused-kernel
+ * shape of later solves. rows_per_lane in {8,16,32,64} (0 = automatic),
+ * max_threads in {256,512}, systems_per_tile a power of two <= 64
+ * (0 = automatic), lanes_per_system (PaScaL sub-partitions, 0 = automatic).
+ * rows_per_lane == -1 selects the general kernels (partitions walked in
+ * global memory, in place) for every later solve; -2 keeps the automatic
+ * choice but without the LDS-streamed kernels (which the automatic choice
+ * takes when n_sys % 32 == 0, the arrays are 16-byte aligned and a variant
+ * holds the partitions; they give the same bits as the fused kernels).
+ * Returns FTD_ERR_UNSUPPORTED for combinations that are not compiled. */
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
                           int systems_per_tile, int lanes_per_system);
 
 /* Extension (diagnostics): the path the last solve took, as a short
- * static string, e.g. "pascal_fused M=32 T=256 S=16 Q=16" or
- * "filtered_general". */
+ * static string, e.g. "pascal_stream M=64 Q=8 R=16",
+ * "pascal_fused M=32 T=256 S=16 Q=16" or "filtered_v1_general". */
 const char* ftd_last_path(const ftd_handle* h);
 
 /* Text of the last error reported by any ftd_* call in this thread. */

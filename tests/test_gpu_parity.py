@@ -55,6 +55,11 @@ def run(kind, A, B, C, D, P=1, cyclic=False, rho=False, eps=None, eps_constant=0
         be.close()
 
 
+def path_Q(path):
+    """partition count of a PaScaL path string ("... Q=8" or "... Q=8 R=16")"""
+    return int(path.split("Q=")[1].split()[0])
+
+
 def relerr(x, ref):
     """max over systems of ||x - ref||_inf / ||ref||_inf"""
     num = np.max(np.abs(x - ref), axis=0)
@@ -93,9 +98,8 @@ def test_pascal_fused_bitwise(n_sys, n_row, tune, cyclic):
         assert n_row <= 64
         ref = O.thomas(A, B, C, D, sem=O.SEM_GPU, cyclic=cyclic)
     else:
-        assert path.startswith("pascal_fused"), path
-        Q = int(path.split("Q=")[1])
-        ref = O.pascal(A, B, C, D, Q, sem=O.SEM_GPU, cyclic=cyclic)
+        assert path.startswith(("pascal_fused", "pascal_stream")), path
+        ref = O.pascal(A, B, C, D, path_Q(path), sem=O.SEM_GPU, cyclic=cyclic)
     np.testing.assert_array_equal(x, ref)
     np.testing.assert_array_equal(Bafter, B)
     thomas = O.thomas(A, B, C, D, sem=O.SEM_GPU, cyclic=cyclic)
@@ -137,8 +141,111 @@ def test_solve_nr_other_dims():
         if path.startswith("thomas_fused"):
             ref = O.thomas(A, B, C, D, sem=O.SEM_GPU)
         else:
-            ref = O.pascal(A, B, C, D, int(path.split("Q=")[1]), sem=O.SEM_GPU)
+            ref = O.pascal(A, B, C, D, path_Q(path), sem=O.SEM_GPU)
         np.testing.assert_array_equal(tD.cpu().numpy(), ref)
+    finally:
+        be.close()
+
+
+# ------------------------------------------- LDS-streamed persistent kernels
+
+STREAM_SHAPES = [
+    # (n_sys, n_row, P): n_sys % 32 == 0; variants (M, Q) = (64, 8), (32, 8),
+    # (64, 4), (32, 4), (32, 16); several tiles per resident block in the
+    # first case, fewer tiles than blocks in others, uneven para_range splits
+    (16480, 509, 8),   # M=64, 515 tiles, m_p = 64/63
+    (96, 250, 8),      # M=32, 3 tiles, m_p = 32/31
+    (24, 24, 8),       # M=32 (m_p = 3, rows past m_p stream clamped), n_sys < 32 -> fused
+    (4128, 256, 4),    # M=64, Q=4
+    (2048, 100, 4),    # M=32, Q=4, m_p = 25
+    (3200, 500, 16),   # M=32, Q=16, m_p = 32/31
+    (8192, 512, 1),    # P=1: 8 internal partitions of 64 rows
+    (9216, 24, 8),     # M=32 with m_p = 3
+]
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("n_sys,n_row,P", STREAM_SHAPES)
+def test_pascal_stream_bitwise(n_sys, n_row, P, cyclic):
+    """The streamed kernel equals the oracle's PaScaL of the same partitioning
+    bit for bit, and so equals the register-resident fused kernel."""
+    note(f"pascal_stream n_sys={n_sys} n_row={n_row} P={P} cyclic={cyclic}")
+    A, B, C, D = cases.general_batch(n_sys, n_row, seed=21)
+    x, _, path, Bafter = run(Kind.PASCAL, A, B, C, D, P=P, cyclic=cyclic)
+    if n_sys % 32 == 0:
+        assert path.startswith("pascal_stream"), path
+    Q = path_Q(path)
+    assert Q == (P if P > 1 else 8)
+    np.testing.assert_array_equal(x, O.pascal(A, B, C, D, Q, sem=O.SEM_GPU, cyclic=cyclic))
+    np.testing.assert_array_equal(Bafter, B)
+    xf, _, pathf, _ = run(Kind.PASCAL, A, B, C, D, P=P, cyclic=cyclic, tuning=(-2, 0, 0, 0))
+    assert pathf.startswith("pascal_fused"), pathf
+    np.testing.assert_array_equal(x, xf)
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("kind", [Kind.FILTERED, Kind.FILTERED_V2])
+@pytest.mark.parametrize("n_sys,n_row,P,rr", [
+    (16480, 509, 8, (0.5, 20.0)),
+    (8448, 512, 8, (0.05, 0.3)),   # interior J
+    (96, 250, 8, (0.1, 1.0)),
+    (4128, 256, 4, (0.05, 0.3)),
+    (2048, 100, 4, (0.5, 20.0)),
+    (3200, 500, 16, (0.05, 0.3)),
+    (9216, 24, 8, (0.1, 1.0)),
+])
+def test_filtered_stream_bitwise(n_sys, n_row, P, rr, kind, cyclic):
+    version = 1 if kind == Kind.FILTERED else 2
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, rr[0], rr[1], seed=17)
+    for eps_c in (1e-3, 1e-12):
+        note(f"filtered_stream v{version} n_sys={n_sys} n_row={n_row} P={P} r={rr} "
+             f"cyclic={cyclic} eps_constant={eps_c}")
+        x, J, path, Bafter = run(kind, A, B, C, D, P=P, cyclic=cyclic, rho=True,
+                                 eps_constant=eps_c)
+        assert path.startswith(f"filtered_v{version}_stream"), path
+        ref, Jref = O.filtered(A, B, C, D, P, version, eps_c, sem=O.SEM_GPU, cyclic=cyclic,
+                               rho="auto", return_J=True)
+        assert list(J) == list(Jref)
+        np.testing.assert_array_equal(x, ref)
+        np.testing.assert_array_equal(Bafter, B)
+        xf, Jf, pathf, _ = run(kind, A, B, C, D, P=P, cyclic=cyclic, rho=True,
+                               eps_constant=eps_c, tuning=(-2, 0, 0, 0))
+        assert "fused" in pathf, pathf
+        assert list(Jf) == list(J)
+        np.testing.assert_array_equal(x, xf)
+
+
+def test_stream_J_edges():
+    """J = 0 (tiny RHS), interior J and J = m-1 through the streamed kernels."""
+    n_sys, n_row, P = 4096, 256, 8
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 0.05, 0.3, seed=19)
+    for scale, eps in ((1e-14, 1e-10), (1.0, 1e-10), (1.0, 1e-300)):
+        for kind, version in ((Kind.FILTERED_V2, 2), (Kind.FILTERED, 1)):
+            note(f"filtered_stream v{version} D scale={scale} eps={eps}")
+            Ds = D * scale
+            x, J, path, _ = run(kind, A, B, C, Ds, P=P, rho=True, eps=eps)
+            assert "stream" in path, path
+            ref, Jref = O.filtered(A, B, C, Ds, P, version, eps, eps_direct=True,
+                                   sem=O.SEM_GPU, rho="auto", return_J=True)
+            assert list(J) == list(Jref)
+            np.testing.assert_array_equal(x, ref)
+
+
+def test_stream_needs_aligned_arrays():
+    """An array that is not 16-byte aligned takes the fused kernel, same bits."""
+    n_sys, n_row, P = 2048, 256, 8
+    note("misaligned D -> fused")
+    A, B, C, D = cases.general_batch(n_sys, n_row, seed=23)
+    be = TdmaBackendGPU("pascal", n_sys, n_row, P)
+    try:
+        tA, tB, tC = dev(A), dev(B), dev(C)
+        flat = torch.zeros(n_row * n_sys + 1, dtype=torch.float64, device="cuda")
+        tD = flat[1:].view(n_row, n_sys)
+        tD.copy_(dev(D))
+        be.solve(tA, tB, tC, tD)
+        torch.cuda.synchronize()
+        assert be.last_path().startswith("pascal_fused"), be.last_path()
+        np.testing.assert_array_equal(tD.cpu().numpy(), O.pascal(A, B, C, D, P, sem=O.SEM_GPU))
     finally:
         be.close()
 
@@ -167,7 +274,7 @@ def test_filtered_fused_bitwise(n_sys, n_row, P, rr, kind, cyclic):
         A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, rr[0], rr[1], seed=7)
         x, J, path, Bafter = run(kind, A, B, C, D, P=P, cyclic=cyclic, rho=rho,
                                  eps_constant=eps_c)
-        assert "fused" in path, path
+        assert "fused" in path or "stream" in path, path
         ref, Jref = O.filtered(A, B, C, D, P, version, eps_c, sem=O.SEM_GPU, cyclic=cyclic,
                                rho="auto" if rho else None, return_J=True)
         assert list(J) == list(Jref)
@@ -256,8 +363,7 @@ def test_full_size_columns_match_oracle(kind):
     cols = np.unique(np.concatenate([np.arange(0, n_sys, 4099), [0, n_sys - 1]]))
     sub = [np.ascontiguousarray(a[:, cols]) for a in (A, B, C, D)]
     if kind == Kind.PASCAL:
-        Q = int(path.split("Q=")[1])
-        ref = O.pascal(*sub, Q, sem=O.SEM_GPU)
+        ref = O.pascal(*sub, path_Q(path), sem=O.SEM_GPU)
         np.testing.assert_array_equal(x[:, cols], ref)
         # residual of the whole batch
         res = B * x - D
@@ -304,7 +410,7 @@ def test_f32_storage_bitwise_vs_oracle_on_widened_data(n_sys, n_row, P, d32):
         x = tX.cpu().numpy()
     finally:
         be.close()
-    Q = int(path.split("Q=")[1])
+    Q = path_Q(path)
     Dw = D32.astype(np.float64) if d32 else D
     ref = O.pascal(A32.astype(np.float64), B32.astype(np.float64), C32.astype(np.float64), Dw, Q,
                    sem=O.SEM_GPU)
