@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kinds", default="pascal,filtered_v2,filtered")
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the uneven-batch and 256^3 ADI-step (C4) legs")
     return ap.parse_args()
 
 
@@ -124,6 +126,67 @@ def c3_error(x_dev, A32, B32, C32, Dw, path):
     return float(np.max(np.max(np.abs(x - ref), axis=0) / np.max(np.abs(ref), axis=0)))
 
 
+def c4_uneven(steps, warmup, torch):
+    """20000 systems, n_row = round(exp(U[ln 16, ln 4096])), a = c = -r,
+    b = 1 + 2r, r ~ U[0.5, 50], D ~ U[-1, 1] (seed 0), back to back in one
+    flat array; one ftd_solve_batched per step on its own copy of D."""
+    from filtered_tdma_amd import TdmaBackendGPU, cases
+
+    n_rows = cases.uneven_lengths(20000, 16, 4096, seed=0)
+    off, n_rows, A, B, C, D, _ = cases.uneven_batch(n_rows, 0.5, 50.0, seed=0)
+    rows = int(n_rows.sum())
+    tA, tB, tC, tD0 = (torch.from_numpy(v).cuda() for v in (A, B, C, D))
+    copies = [torch.empty_like(tD0) for _ in range(steps)]
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    stream = torch.cuda.current_stream()
+    for k in range(warmup):
+        copies[k % steps].copy_(tD0)
+        be.solve_batched(tA, tB, tC, copies[k % steps], off, n_rows)
+    for c in copies:
+        c.copy_(tD0)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        be.solve_batched(tA, tB, tC, copies[k], off, n_rows)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
+    path = be.last_path()
+    be.close()
+    return {
+        "workload": "20000 systems, n_row log-uniform [16, 4096], r~U[0.5,50], FP64, "
+                    "system-contiguous layout, one persistent launch",
+        "rows": rows, "rows_per_s": rows / (wall / steps), "ms_per_step": wall * 1e3 / steps,
+        "event_ms_mean": ms, "bytes_per_row": BYTES_PER_ROW,
+        "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "path": path,
+    }
+
+
+def c4_adi(steps):
+    """One 256^3 heat ADI step (Dirichlet ghosts, RHS, Z, Y, X sweeps: build,
+    pascal solve, scatter) captured as a hipGraph and replayed."""
+    from filtered_tdma_amd.heat import SolveTheta
+
+    n = 256
+    sim = SolveTheta(n, n, n, parts=(1, 1, 1), backend="pascal", dt=1e-4)
+    try:
+        sim.run(2, use_graph=True)  # eager step, then capture + first replay
+        t0 = time.perf_counter()
+        sim.run(steps, use_graph=True)
+        wall = time.perf_counter() - t0
+        last = sim.last_step_ms()
+    finally:
+        sim.close()
+    return {"workload": "heat ADI step 256^3, pascal, one partition per direction, hipGraph",
+            "ms_per_step": wall * 1e3 / steps, "last_step_event_ms": last,
+            "rows_per_s": 3 * n ** 3 / (wall / steps)}
+
+
 def cpu_baseline(A, B, C, D):
     """The oracle's PaScaL (P = 8) on the host cores, whole batch, in place on
     copies made outside the timed region; reps until ~10 s have passed."""
@@ -204,6 +267,16 @@ def main():
                   file=sys.stderr, flush=True)
         del tA32, tB32, tC32, tD32, tD3
 
+    c4 = None
+    if not args.no_c4:
+        del copies
+        torch.cuda.empty_cache()
+        c4 = {"uneven": c4_uneven(args.steps, args.warmup, torch), "adi_256": c4_adi(args.steps)}
+        print(f"[bench] c4 uneven: {c4['uneven']['event_ms_mean']:.4f} ms "
+              f"({c4['uneven']['hbm_frac_algorithmic']:.3f} of HBM) {c4['uneven']['path']}; "
+              f"adi 256^3 graph step {c4['adi_256']['ms_per_step']:.4f} ms",
+              file=sys.stderr, flush=True)
+
     head = results["pascal"] if "pascal" in results else next(iter(results.values()))
     achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9
     traffic = None
@@ -238,6 +311,8 @@ def main():
         },
         "backends": results,
     }
+    if c4 is not None:
+        line["c4"] = c4
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A, B, C, D)
     print(json.dumps(line), flush=True)

@@ -101,6 +101,22 @@ class TdmaBackendGPU:
         a, b, c, d = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C, D))
         _lib.check(_lib.load().ftd_solve_f32(self._h, a, b, c, d, _ptr(X, self.n_row, self.n_sys)))
 
+    def solve_batched(self, A, B, C, D, row_offsets, n_rows) -> None:
+        """Uneven batch (no reference counterpart): system s is elements
+        row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of the 1-D float64
+        device arrays A, B, C, D; D <- x. row_offsets (int64) and n_rows
+        (int32) are host arrays."""
+        import numpy as np
+
+        off = np.ascontiguousarray(row_offsets, dtype=np.int64)
+        rows = np.ascontiguousarray(n_rows, dtype=np.int32)
+        if off.shape != rows.shape or off.ndim != 1:
+            raise ValueError("row_offsets and n_rows must be 1-D of one length")
+        ptrs = [_ptr(x) for x in (A, B, C, D)]
+        _lib.check(_lib.load().ftd_solve_batched(
+            self._h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(off.size), *ptrs))
+
     def set_rho_device(self, A, B, C) -> None:
         a, b, c, _ = self._args(A, B, C, A)
         _lib.check(_lib.load().ftd_set_rho(self._h, a, b, c))

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libftdma.so")
 SOURCES = [os.path.join(CSRC, "ftdma_kernels.hip"), os.path.join(CSRC, "ftdma_api.cpp"),
-           os.path.join(CSRC, "ftdma_heat.hip")]
+           os.path.join(CSRC, "ftdma_heat.hip"), os.path.join(CSRC, "ftdma_batched.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -77,3 +77,29 @@ def general_batch(n_sys: int, n_row: int, seed: int = 0, margin: float = 0.1):
     B = sgn * (np.abs(A) + np.abs(C)) * (1.0 + margin + u)
     D = _rhs(seed, n_sys, n_row)
     return A, B, C, D
+
+
+def uneven_lengths(n_systems: int, n_lo: int, n_hi: int, seed: int = 0) -> np.ndarray:
+    """N_s = round(exp(U[ln n_lo, ln n_hi])) per system (stream 6)."""
+    u = uniform(seed, 6, np.arange(n_systems, dtype=np.uint64))
+    n = np.rint(np.exp(np.log(n_lo) + (np.log(n_hi) - np.log(n_lo)) * u)).astype(np.int32)
+    return np.clip(n, n_lo, n_hi)
+
+
+def uneven_batch(n_rows, r_lo: float, r_hi: float, seed: int = 0):
+    """Uneven batch in the system-contiguous layout of ftd_solve_batched:
+    system s is elements off[s] .. off[s] + n_rows[s] - 1 of the flat arrays
+    (systems back to back). a = c = -r_s, b = 1 + 2 r_s, r_s ~ U[r_lo, r_hi]
+    (stream 7), D ~ U[-1, 1] over the flat index (stream 8).
+    Returns (off, n_rows, A, B, C, D, r)."""
+    n_rows = np.asarray(n_rows, dtype=np.int32)
+    off = np.zeros(n_rows.size, dtype=np.int64)
+    np.cumsum(n_rows[:-1], out=off[1:])
+    total = int(n_rows.sum())
+    r = r_lo + (r_hi - r_lo) * uniform(seed, 7, np.arange(n_rows.size, dtype=np.uint64))
+    rr = np.repeat(r, n_rows)
+    A = -rr
+    C = -rr
+    B = 1.0 + 2.0 * rr
+    D = -1.0 + 2.0 * uniform(seed, 8, np.arange(total, dtype=np.uint64))
+    return off, n_rows, A, B, C, D, r

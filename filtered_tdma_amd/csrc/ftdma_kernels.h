@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
 
 namespace ftd {
 
@@ -39,6 +42,31 @@ hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const dou
                                   const double* B, const double* C, double* D, int n_sys,
                                   int n_row, int cyclic, const int* Jp, double* xbuf,
                                   double* partial, hipStream_t st);
+
+// uneven batches: one lane per chunk of <= 64 rows of a system, 64 lanes per
+// work item (a wave); kind 0 = single-chunk systems (plain Thomas), kind 1 =
+// chunked systems (PaScaL with P = Q inside the wave)
+struct alignas(16) BatchLane {
+    int64_t off;     // element offset of the chunk's first row
+    int32_t m;       // rows in the chunk (0: idle lane)
+    uint8_t q;       // chunk index within its system
+    uint8_t Q;       // chunks of its system
+    uint8_t lead;    // lane of chunk 0 of its system within the item
+    uint8_t kind;    // 1: chunked system, 0: single-chunk (uniform over the item)
+};
+static_assert(sizeof(BatchLane) == 16, "BatchLane is one dwordx4");
+
+// Plans the batch: lanes (64 per item; the n_chunked items of chunked
+// systems first, then single-chunk items, each range longest first) and the
+// systems longer than 64 * 64 rows (lane-per-system kernel). Returns the
+// number of items, or -1 when some n_rows[s] < 1.
+int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
+               std::vector<BatchLane>& lanes, std::vector<int64_t>& long_off,
+               std::vector<int>& long_n, int& n_chunked);
+hipError_t launch_batched(const double* A, const double* B, double* C, double* D,
+                          const BatchLane* lanes, int n_chunked, int n_items,
+                          const int64_t* long_off, const int* long_n, int n_long,
+                          hipStream_t st);
 
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,

@@ -21,11 +21,14 @@
  * exact for any P, which then only affects rounding.
  *
  * Every call is ordered on the handle's stream and returns without a host
- * synchronisation, except ftd_last_J and solves with profiling enabled.
+ * synchronisation, except ftd_last_J, solves with profiling enabled and
+ * ftd_solve_batched with a layout the handle has not planned yet.
  * Calls on one handle must not overlap (the handle owns scratch memory).
  */
 #ifndef FTDMA_H
 #define FTDMA_H
+
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -136,6 +139,20 @@ double ftd_last_comm_ms(const ftd_handle* h);
  * Returns FTD_ERR_UNSUPPORTED for combinations that are not compiled. */
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
                           int systems_per_tile, int lanes_per_system);
+
+/* Extension (uneven batches; no counterpart in the reference, whose batches
+ * share one n_row): n_systems independent systems, system s occupying
+ * elements row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of each of A, B,
+ * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
+ * a system of <= 64 rows is plain Thomas, a longer one PaScaL with
+ * ceil(n_rows/64) partitions inside one wave; all in one persistent launch
+ * (systems above 4096 rows in a second, lane-per-system launch that uses C
+ * as scratch). D is overwritten with x; A and B are read only. The plan is
+ * made on the host once per layout and kept on the handle; a call with a
+ * new layout waits for the handle's stream first. The handle's kind and
+ * sizes are not used. */
+ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                             int n_systems, double* A, double* B, double* C, double* D);
 
 /* Extension (diagnostics): the path the last solve took, as a short
  * static string, e.g. "pascal_stream M=64 Q=8 R=16",
