@@ -101,6 +101,30 @@ class TdmaBackendGPU:
         a, b, c, d = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C, D))
         _lib.check(_lib.load().ftd_solve_f32(self._h, a, b, c, d, _ptr(X, self.n_row, self.n_sys)))
 
+    def solve_complex(self, A, B, C, Z) -> None:
+        """Complex right-hand sides: Z complex128 [n_row, n_sys] (or float64
+        [n_row, 2 n_sys], (re, im) interleaved) <- x; A, B, C real."""
+        import torch
+
+        a, b, c = self._args(A, B, C, A)[:3]
+        if isinstance(Z, int):
+            z = ctypes.c_void_p(Z)
+        else:
+            shape = (self.n_row, self.n_sys) if Z.dtype == torch.complex128 else \
+                (self.n_row, 2 * self.n_sys)
+            if Z.dtype not in (torch.complex128, torch.float64) or not Z.is_cuda or \
+                    not Z.is_contiguous() or tuple(Z.shape) != shape:
+                raise ValueError(f"Z must be a contiguous complex128 {shape[0]}x{self.n_sys} "
+                                 "device tensor (or its float64 view)")
+            z = ctypes.c_void_p(Z.data_ptr())
+        _lib.check(_lib.load().ftd_solve_complex(self._h, a, b, c, z))
+
+    def last_J(self, n=None) -> list:
+        n = self.partitions if n is None else int(n)
+        out = (ctypes.c_int * n)()
+        _lib.check(_lib.load().ftd_last_J(self._h, out, n))
+        return list(out)
+
     def solve_batched(self, A, B, C, D, row_offsets, n_rows) -> None:
         """Uneven batch (no reference counterpart): system s is elements
         row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of the 1-D float64
@@ -138,10 +162,6 @@ class TdmaBackendGPU:
         _lib.check(_lib.load().ftd_set_tuning(self._h, rows_per_lane, max_threads,
                                               systems_per_tile, lanes_per_system))
 
-    def last_J(self) -> list:
-        out = (ctypes.c_int * self.partitions)()
-        _lib.check(_lib.load().ftd_last_J(self._h, out, self.partitions))
-        return list(out)
 
     def last_gpu_ms(self) -> float:
         return _lib.load().ftd_last_gpu_ms(self._h)

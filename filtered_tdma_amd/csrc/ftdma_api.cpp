@@ -175,8 +175,10 @@ bool pick_pascal(const ftd_handle* h, int n_row, int cyclic, FusedShape& out) {
     return false;
 }
 
+// csh = 1: D holds complex right-hand sides as interleaved (re, im) columns,
+// n_sys counts those real columns and the coefficients have n_sys / 2
 ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D, int n_sys,
-                       int n_row, int cyclic) {
+                       int n_row, int cyclic, int csh = 0) {
     hipStream_t st = h->stream;
     if (cyclic && n_row < 3) return fail(FTD_ERR_UNSUPPORTED, "periodic solve needs n_row >= 3");
     FusedShape fs;
@@ -189,7 +191,7 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
             FTD_HIP(ftd::launch_thomas_cyclic_fused(A, B, C, D, n_sys, n_row, st));
         } else {
             std::snprintf(h->path, sizeof h->path, "thomas_fused M=%d", ftd::thomas_fused_rows(n_row));
-            FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, st));
+            FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, csh, st));
         }
         return FTD_OK;
     }
@@ -198,7 +200,7 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
         // results) where a variant holds it and the arrays allow its DMA
         int s_, m_max;
         part_of(n_row, fs.Q, 0, s_, m_max);
-        const int Ms = (h->t_M || h->no_stream) ? 0 : ftd::stream_rows(m_max, fs.Q);
+        const int Ms = (h->t_M || h->no_stream || csh) ? 0 : ftd::stream_rows(m_max, fs.Q);
         if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
             std::snprintf(h->path, sizeof h->path, "pascal_stream M=%d Q=%d R=%d", Ms, fs.Q,
                           ftd::stream_ring_depth(Ms, fs.Q));
@@ -208,10 +210,11 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
         std::snprintf(h->path, sizeof h->path, "pascal_fused M=%d T=%d S=%d Q=%d", fs.M, fs.T,
                       fs.S, fs.Q);
         FTD_HIP(ftd::launch_pascal_fused(fs.M, fs.T, A, B, C, D, n_sys, n_row, fs.Q, fs.S, cyclic,
-                                         st));
+                                         csh, st));
         return FTD_OK;
     }
     if (h->t_M) return fail(FTD_ERR_UNSUPPORTED, "forced fused shape does not fit this system");
+    if (csh) return fail(FTD_ERR_UNSUPPORTED, "complex right-hand sides need a fused kernel shape");
     int P = h->P;
     if (P > 1) {
         int s_, ml;
@@ -239,8 +242,9 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
 
 // ---------------- Filtered ----------------
 
-ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double* D, int cyclic) {
-    const int n_sys = h->n_sys, n_row = h->n_row, P = h->P;
+ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double* D, int cyclic,
+                          int csh = 0) {
+    const int n_sys = h->n_sys << csh, n_row = h->n_row, P = h->P;
     const int version = (h->kind == FTD_KIND_FILTERED_V2) ? 2 : 1;
     hipStream_t st = h->stream;
     int s_, mmax, mmin;
@@ -270,77 +274,80 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
 
     int* Jp = h->J;
     const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
-    const int Ms = (h->force_general || h->t_M || h->no_stream) ? 0 : ftd::stream_rows(mmax, P);
+    const int Ms =
+        (h->force_general || h->t_M || h->no_stream || csh) ? 0 : ftd::stream_rows(mmax, P);
     if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
         const int R = ftd::stream_ring_depth(Ms, P);
         if (version == 2) {
             int nt = 0;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * P));
-            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
+            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                          h->eps_direct, Jp, st));
+                                          h->eps_direct, csh, Jp, st));
             std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d", Ms, P, R);
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 nullptr, nullptr, st));
         } else {
             const int tiles = n_sys / 32;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h->xbuf, xbytes));
             std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d", Ms, P, R);
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 (double*)h->xbuf.p, (double*)h->partial.p, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
-                                          h->eps_value, h->eps_direct, Jp, st));
+                                          h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 (double*)h->xbuf.p, nullptr, st));
         }
         return FTD_OK;
     }
+    if (fs.M && csh && fs.S < 2) fs.M = 0;  // the two parts of a system share a tile
     if (fs.M) {
         const int tiles = (n_sys + fs.S - 1) / fs.S;
         if (version == 2) {
             int nt = 0;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * P));
-            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
+            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                          h->eps_direct, Jp, st));
+                                          h->eps_direct, csh, Jp, st));
             std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d", fs.M, fs.T,
                           fs.S);
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, Jp, nullptr, nullptr, st));
+                                               cyclic, csh, Jp, nullptr, nullptr, st));
         } else {
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h->xbuf, xbytes));
             std::snprintf(h->path, sizeof h->path, "filtered_v1_fused M=%d T=%d S=%d", fs.M, fs.T,
                           fs.S);
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, Jp, (double*)h->xbuf.p,
+                                               cyclic, csh, Jp, (double*)h->xbuf.p,
                                                (double*)h->partial.p, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
-                                          h->eps_value, h->eps_direct, Jp, st));
+                                          h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, Jp, (double*)h->xbuf.p, nullptr, st));
+                                               cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, st));
         }
         return FTD_OK;
     }
 
+    if (csh) return fail(FTD_ERR_UNSUPPORTED, "complex right-hand sides need a fused kernel shape");
     // general path: sweeps in place, interface, J, final
     const int tiles = (n_sys + 255) / 256;
-    FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * P));
+    FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
     FTD_TRY(ensure(h->xbuf, xbytes));
     std::snprintf(h->path, sizeof h->path, "filtered_v%d_general", version);
     int nt = 0;
     if (version == 2) {
-        FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, (double*)h->partial.p, &nt, st));
+        FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
         FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                      h->eps_direct, Jp, st));
+                                      h->eps_direct, csh, Jp, st));
     }
     FTD_HIP(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
     FTD_HIP(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
                                            (double*)h->partial.p, &nt, st));
     if (version == 1)
         FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                      h->eps_direct, Jp, st));
+                                      h->eps_direct, csh, Jp, st));
     FTD_HIP(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
     return FTD_OK;
 }
@@ -405,10 +412,12 @@ ftd_status ftd_create(ftd_handle** out, int kind, int n_sys, int n_row, int part
     hipError_t e = hipGetDevice(&h->device);
     if (e == hipSuccess) e = hipMalloc(&h->rho, sizeof(double) * 2 * (size_t)(n_row + 1));
     if (e == hipSuccess) e = hipMemset(h->rho, 0, sizeof(double) * 2 * (size_t)(n_row + 1));
-    if (e == hipSuccess) e = hipMalloc(&h->J, sizeof(int) * (size_t)partitions);
+    // J per partition; entries P..2P-1 hold the imaginary part's J after a
+    // complex solve
+    if (e == hipSuccess) e = hipMalloc(&h->J, sizeof(int) * 2 * (size_t)partitions);
     if (e == hipSuccess) {
-        std::vector<int> neg(partitions, -1);
-        e = hipMemcpy(h->J, neg.data(), sizeof(int) * partitions, hipMemcpyHostToDevice);
+        std::vector<int> neg(2 * partitions, -1);
+        e = hipMemcpy(h->J, neg.data(), sizeof(int) * 2 * partitions, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
@@ -487,7 +496,7 @@ ftd_status ftd_set_profiling(ftd_handle* h, int enable) {
 
 ftd_status ftd_last_J(ftd_handle* h, int* J, int n) {
     if (!h || !J) return fail(FTD_ERR_INVALID, "null argument");
-    if (n < 0 || n > h->P) return fail(FTD_ERR_INVALID, "n must be in [0, partitions]");
+    if (n < 0 || n > 2 * h->P) return fail(FTD_ERR_INVALID, "n must be in [0, 2 * partitions]");
     if (n == 0) return FTD_OK;
     FTD_HIP(hipMemcpyAsync(J, h->J, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
     FTD_HIP(hipStreamSynchronize(h->stream));
@@ -613,6 +622,26 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
                                 h->b_items,
                                 (const int64_t*)h->b_dlong_off.p, (const int*)h->b_dlong_n.p, nl,
                                 h->stream));
+    if (h->profiling) {
+        FTD_HIP(hipEventRecord(h->ev1, h->stream));
+        FTD_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0.0f;
+        FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, double* Z) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (!A || !B || !C || !Z) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (h->n_sys == 0) return FTD_OK;
+    if (h->n_sys > (1 << 29)) return fail(FTD_ERR_INVALID, "too many systems for complex mode");
+    const bool filtered = h->kind != FTD_KIND_PASCAL && h->P > 1;
+    if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
+    ftd_status s = filtered ? solve_filtered(h, A, B, C, Z, 0, 1)
+                            : solve_exact(h, A, B, C, Z, 2 * h->n_sys, h->n_row, 0, 1);
+    if (s != FTD_OK) return s;
     if (h->profiling) {
         FTD_HIP(hipEventRecord(h->ev1, h->stream));
         FTD_HIP(hipEventSynchronize(h->ev1));

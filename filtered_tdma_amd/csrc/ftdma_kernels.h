@@ -12,15 +12,18 @@ namespace ftd {
 
 hipError_t launch_set_rho(const double* A, const double* B, const double* C, int n_sys, int n_row,
                           double* rho, hipStream_t st);
-hipError_t launch_boundary_maxabs(const double* D, int n_sys, int n_row, int P, double* partial,
-                                  int* n_tiles, hipStream_t st);
+// csh = 1: complex right-hand sides stored as interleaved (re, im) columns
+// of n_sys = 2 x (complex systems) real columns; the coefficients have
+// n_sys / 2 columns, and J is kept per parity (J[p], J[P + p])
+hipError_t launch_boundary_maxabs(const double* D, int n_sys, int n_row, int P, int csh,
+                                  double* partial, int* n_tiles, hipStream_t st);
 hipError_t launch_compute_J(const double* partial, int n_tiles, int P, int n_row, const double* rho,
-                            double eps_value, int eps_direct, int* J, hipStream_t st);
+                            double eps_value, int eps_direct, int csh, int* J, hipStream_t st);
 
 int pascal_fused_supported(int M, int maxt);
 size_t pascal_fused_lds(int Q, int S, int cyclic);
 hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B, const double* C,
-                               double* D, int n_sys, int n_row, int Q, int S, int cyclic,
+                               double* D, int n_sys, int n_row, int Q, int S, int cyclic, int csh,
                                hipStream_t st);
 
 hipError_t launch_pascal_fused_f32c(int M, int maxt, const float* A, const float* B,
@@ -70,7 +73,7 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
 
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
-                               int n_sys, int n_row, hipStream_t st);
+                               int n_sys, int n_row, int csh, hipStream_t st);
 
 hipError_t launch_thomas_cyclic_fused(const double* A, const double* B, const double* C, double* D,
                                       int n_sys, int n_row, hipStream_t st);
@@ -79,7 +82,7 @@ int filtered_fused_supported(int M, int maxt);
 size_t filtered_fused_lds(int P, int S);
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
-                                 int P, int S, int cyclic, const int* Jp, double* xbuf,
+                                 int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
                                  double* partial, hipStream_t st);
 
 hipError_t launch_gen_thomas(const double* A, const double* B, double* C, double* D, int n_sys,

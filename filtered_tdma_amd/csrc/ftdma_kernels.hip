@@ -93,10 +93,12 @@ __global__ void k_set_rho(const double* __restrict__ A, const double* __restrict
 
 // max over systems of |D[row 0 of p]| and |D[row m_p-1 of p]|, one block of
 // 256 systems per (tile, partition); partial[tile * P + p]
+// (csh = 1: complex right-hand sides, even and odd columns reduce separately
+// into partial[(parity * tiles + tile) * P + p])
 __global__ __launch_bounds__(256) void k_boundary_maxabs(const double* __restrict__ D, int n_sys,
-                                                         int n_row, int P,
+                                                         int n_row, int P, int csh,
                                                          double* __restrict__ partial) {
-    __shared__ double red[4];
+    __shared__ double red[2][4];
     const int p = blockIdx.y;
     int s, m;
     part_of(n_row, P, p, s, m);
@@ -107,12 +109,19 @@ __global__ __launch_bounds__(256) void k_boundary_maxabs(const double* __restric
         double b = fabs(D[(size_t)(s + m - 1) * n_sys + i]);
         v = fmax(a, b);
     }
-    v = group_max(v, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    const int odd = csh ? (i & 1) : 0;
+    double v0 = odd ? 0.0 : v, v1 = odd ? v : 0.0;  // |D| >= 0: 0 is neutral
+    v0 = group_max(v0, 64);
+    v1 = group_max(v1, 64);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = v0;
+        red[1][threadIdx.x >> 6] = v1;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double w = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        partial[(size_t)blockIdx.x * P + p] = w;
+    if (threadIdx.x < 1 + csh) {
+        const double* r = red[threadIdx.x];
+        double w = fmax(fmax(r[0], r[1]), fmax(r[2], r[3]));
+        partial[((size_t)threadIdx.x * gridDim.x + blockIdx.x) * P + p] = w;
     }
 }
 
@@ -125,11 +134,13 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
                                                    double eps_value, int eps_direct,
                                                    int* __restrict__ J_out) {
     __shared__ double red_b[4], red_r[4];
-    const int p = blockIdx.x;
+    // block p + k * P: partition p of parity k (complex right-hand sides)
+    const int p = blockIdx.x % P;
+    const double* part = partial + (size_t)(blockIdx.x / P) * n_tiles * P;
     int s, m;
     part_of(n_row, P, p, s, m);
     double vb = 0.0, vr = 0.0;
-    for (int t = threadIdx.x; t < n_tiles; t += 256) vb = fmax(vb, partial[(size_t)t * P + p]);
+    for (int t = threadIdx.x; t < n_tiles; t += 256) vb = fmax(vb, part[(size_t)t * P + p]);
     for (int k = 1 + threadIdx.x; k < m; k += 256)
         vr = fmax(vr, fmax(fabs(A_rho[s + k]), fabs(C_rho[s + k])));
     vb = group_max(vb, 64);
@@ -144,7 +155,7 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
         double rho = fmax(fmax(red_r[0], red_r[1]), fmax(red_r[2], red_r[3]));
         double Nl = (double)(m * P);
         double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
-        J_out[p] = cal_J_gpu(rho, mb, eps, m);
+        J_out[blockIdx.x] = cal_J_gpu(rho, mb, eps, m);
     }
 }
 
@@ -152,7 +163,11 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
 // the partition's last row so that the load never depends on a branch (the
 // compiler may then issue it PF rows ahead of its use). Rows past the count
 // reload the last row, which is in cache, and are never used. Values are
-// widened to double on load (exact for float storage).
+// widened to double on load (exact for float storage). The coefficients are
+// read at cbase / cld, the right-hand side at base / ld: the same place for
+// real systems; for complex right-hand sides stored as interleaved (re, im)
+// pairs, lanes 2k and 2k+1 solve the real and imaginary parts of system k
+// and read the same coefficient column k (CSH = 1).
 // Streamed operands are touched once: FTD_NONTEMPORAL=1 marks the row loads
 // and the solution stores nontemporal (no reuse expected in the caches).
 #ifndef FTD_NONTEMPORAL
@@ -170,9 +185,10 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
     do {                                                                     \
         const int jj_ = (j) < cnt ? (j) : cnt - 1;                           \
         const size_t o_ = base + (size_t)jj_ * ld;                           \
-        ra[(j)] = (double)FTD_LD(A + o_);                                    \
-        rb[(j)] = (double)FTD_LD(B + o_);                                    \
-        rc[(j)] = (double)FTD_LD(C + o_);                                    \
+        const size_t oc_ = cbase + (size_t)jj_ * cld;                        \
+        ra[(j)] = (double)FTD_LD(A + oc_);                                   \
+        rb[(j)] = (double)FTD_LD(B + oc_);                                   \
+        rc[(j)] = (double)FTD_LD(C + oc_);                                   \
         rd[(j)] = (double)FTD_LD(Din + o_);                                  \
     } while (0)
 
@@ -181,7 +197,7 @@ __global__ __launch_bounds__(256) void k_compute_J(const double* __restrict__ pa
 // reference's single-rank path (tdma_many_kernel, tdma_local_cuda.cu:32-64),
 // row 0 divided by b. One lane per system, D read once and written once.
 // ---------------------------------------------------------------------------
-template <int M, int PF>
+template <int M, int PF, int CSH>
 __global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__ A,
                                                       const double* __restrict__ B,
                                                       const double* __restrict__ C,
@@ -194,6 +210,8 @@ __global__ __launch_bounds__(256) void k_thomas_fused(const double* __restrict__
     const double* Din = D;
     const size_t ld = (size_t)n_sys;
     const size_t base = (size_t)i;
+    const size_t cld = ld >> CSH;
+    const size_t cbase = (size_t)(i >> CSH);
     double c[M], d[M];
     double ra[M], rb[M], rc[M], rd[M];
 #pragma unroll
@@ -237,6 +255,7 @@ __global__ __launch_bounds__(256) void k_thomas_cyclic_fused(const double* __res
     const double* Din = D;
     const size_t ld = (size_t)n_sys;
     const size_t base = (size_t)i;
+    const size_t cld = ld, cbase = base;
     double c[M], d[M], e[M];
     double ra[M], rb[M], rc[M], rd[M];
 #pragma unroll
@@ -296,7 +315,7 @@ __global__ __launch_bounds__(256) void k_thomas_cyclic_fused(const double* __res
 // ---------------------------------------------------------------------------
 // TC: storage type of A, B, C; TD: of the right-hand side Din. The solution
 // goes to X (float64), which may be Din itself when TD is double.
-template <int M, int MAXT, bool CYCLIC, int PF, int WPE, typename TC, typename TD>
+template <int M, int MAXT, bool CYCLIC, int PF, int WPE, typename TC, typename TD, int CSH = 0>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_fused(
     const TC* __restrict__ A, const TC* __restrict__ B, const TC* __restrict__ C, const TD* Din,
     double* X, int n_sys, int n_row, int Q, int lgS) {
@@ -307,11 +326,14 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     const int q = t >> lgS;
     const int i_raw = blockIdx.x * S + sl;
     const bool live = i_raw < n_sys;
-    const int i = live ? i_raw : n_sys - 1;  // dead lanes shadow the last system
+    // dead lanes shadow the last system (of their parity when CSH = 1)
+    const int i = live ? i_raw : n_sys - (1 << CSH) + (i_raw & ((1 << CSH) - 1));
     int r0, cnt;
     part_of(n_row, Q, q, r0, cnt);
     const size_t ld = (size_t)n_sys;
     const size_t base = (size_t)r0 * ld + (size_t)i;
+    const size_t cld = ld >> CSH;
+    const size_t cbase = (size_t)r0 * cld + (size_t)(i >> CSH);
 
     double a[M], c[M], d[M];
     double ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
@@ -514,7 +536,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 #ifndef FTD_DMA_AUX
-#define FTD_DMA_AUX 0
+#define FTD_DMA_AUX 2  // nontemporal: the rows are read once
 #endif
 
 // The ring of one wave and its bookkeeping. NST: vector memory ops the tail
@@ -735,7 +757,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
 //   PASS 2: v1 second pass -> recompute the sweeps, take x0, xN from xbuf and
 //           J_p, store x.
 // ---------------------------------------------------------------------------
-template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE>
+template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE, int CSH>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_fused(const double* __restrict__ A,
                                                          const double* __restrict__ B,
                                                          const double* __restrict__ C,
@@ -751,12 +773,18 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     const int p = t >> lgS;
     const int i_raw = blockIdx.x * S + sl;
     const bool live = i_raw < n_sys;
-    const int i = live ? i_raw : n_sys - 1;
+    // dead lanes shadow the last system (of their parity when CSH = 1)
+    const int i = live ? i_raw : n_sys - (1 << CSH) + (i_raw & ((1 << CSH) - 1));
     int r0, m;
     part_of(n_row, P, p, r0, m);
     const size_t ld = (size_t)n_sys;
     const size_t base = (size_t)r0 * ld + (size_t)i;
-    const int J = (VERSION == 2 || PASS == 2) ? Jp[p] : m;
+    const size_t cld = ld >> CSH;
+    const size_t cbase = (size_t)r0 * cld + (size_t)(i >> CSH);
+    // complex right-hand sides: the real and imaginary parts keep their own J
+    // (the reference solves them as two real systems)
+    const int par = CSH ? (i & 1) : 0;
+    const int J = (VERSION == 2 || PASS == 2) ? Jp[par * P + p] : m;
 
     double a[M], c[M], d[M];
     double ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
@@ -867,11 +895,13 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
             xbuf[(size_t)(2 * p) * ld + i] = x0;
             xbuf[(size_t)(2 * p + 1) * ld + i] = xN;
         }
-        // dead lanes shadow a live system, so they do not change the max
-        double v = group_max(fmax(fabs(x0), fabs(xN)), S < 64 ? S : 64);
-        if (S <= 64) {
-            if (sl == 0) partial[(size_t)blockIdx.x * P + p] = v;
-        }
+        // dead lanes shadow a live system, so they do not change the max;
+        // with CSH = 1 the two parities reduce separately (xor offsets >= 2)
+        double v = fmax(fabs(x0), fabs(xN));
+        const int width = S < 64 ? S : 64;
+        for (int off = width >> 1; off >= (1 << CSH); off >>= 1)
+            v = fmax(v, __shfl_xor(v, off, 64));
+        if (sl < (1 << CSH)) partial[((size_t)sl * gridDim.x + blockIdx.x) * P + p] = v;
         return;
     }
 
@@ -1441,19 +1471,19 @@ hipError_t launch_set_rho(const double* A, const double* B, const double* C, int
     return hipGetLastError();
 }
 
-hipError_t launch_boundary_maxabs(const double* D, int n_sys, int n_row, int P, double* partial,
-                                  int* n_tiles, hipStream_t st) {
+hipError_t launch_boundary_maxabs(const double* D, int n_sys, int n_row, int P, int csh,
+                                  double* partial, int* n_tiles, hipStream_t st) {
     const int nt = (n_sys + 255) / 256;
     *n_tiles = nt;
-    hipLaunchKernelGGL(k_boundary_maxabs, dim3(nt, P), dim3(256), 0, st, D, n_sys, n_row, P,
+    hipLaunchKernelGGL(k_boundary_maxabs, dim3(nt, P), dim3(256), 0, st, D, n_sys, n_row, P, csh,
                        partial);
     return hipGetLastError();
 }
 
 hipError_t launch_compute_J(const double* partial, int n_tiles, int P, int n_row, const double* rho,
-                            double eps_value, int eps_direct, int* J, hipStream_t st) {
-    hipLaunchKernelGGL(k_compute_J, dim3(P), dim3(256), 0, st, partial, n_tiles, P, n_row, rho,
-                       rho + (n_row + 1), eps_value, eps_direct, J);
+                            double eps_value, int eps_direct, int csh, int* J, hipStream_t st) {
+    hipLaunchKernelGGL(k_compute_J, dim3(P << csh), dim3(256), 0, st, partial, n_tiles, P, n_row,
+                       rho, rho + (n_row + 1), eps_value, eps_direct, J);
     return hipGetLastError();
 }
 
@@ -1484,7 +1514,7 @@ size_t pascal_fused_lds(int Q, int S, int cyclic) {
 template <typename TC, typename TD>
 static hipError_t launch_pascal_fused_t(int M, int maxt, const TC* A, const TC* B, const TC* C,
                                         const TD* Din, double* Xout, int n_sys, int n_row, int Q,
-                                        int S, int cyclic, hipStream_t st) {
+                                        int S, int cyclic, int csh, hipStream_t st) {
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * Q);
     const size_t lds = pascal_fused_lds(Q, S, cyclic);
@@ -1493,6 +1523,9 @@ static hipError_t launch_pascal_fused_t(int M, int maxt, const TC* A, const TC* 
         if (cyclic)                                                                         \
             hipLaunchKernelGGL((k_pascal_fused<m_, t_, true, pf_, w_, TC, TD>), grid, block, \
                                lds, st, A, B, C, Din, Xout, n_sys, n_row, Q, lgS);          \
+        else if (csh)                                                                       \
+            hipLaunchKernelGGL((k_pascal_fused<m_, t_, false, pf_, w_, TC, TD, 1>), grid,   \
+                               block, lds, st, A, B, C, Din, Xout, n_sys, n_row, Q, lgS);   \
         else                                                                                \
             hipLaunchKernelGGL((k_pascal_fused<m_, t_, false, pf_, w_, TC, TD>), grid,      \
                                block, lds, st, A, B, C, Din, Xout, n_sys, n_row, Q, lgS);   \
@@ -1504,41 +1537,49 @@ static hipError_t launch_pascal_fused_t(int M, int maxt, const TC* A, const TC* 
 }
 
 hipError_t launch_pascal_fused(int M, int maxt, const double* A, const double* B, const double* C,
-                               double* D, int n_sys, int n_row, int Q, int S, int cyclic,
+                               double* D, int n_sys, int n_row, int Q, int S, int cyclic, int csh,
                                hipStream_t st) {
+    if (csh && cyclic) return hipErrorInvalidValue;
     return launch_pascal_fused_t<double, double>(M, maxt, A, B, C, D, D, n_sys, n_row, Q, S,
-                                                 cyclic, st);
+                                                 cyclic, csh, st);
 }
 
 hipError_t launch_pascal_fused_f32c(int M, int maxt, const float* A, const float* B,
                                     const float* C, double* D, int n_sys, int n_row, int Q, int S,
                                     int cyclic, hipStream_t st) {
     return launch_pascal_fused_t<float, double>(M, maxt, A, B, C, D, D, n_sys, n_row, Q, S,
-                                                cyclic, st);
+                                                cyclic, 0, st);
 }
 
 hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float* B, const float* C,
                                    const float* D, double* X, int n_sys, int n_row, int Q, int S,
                                    int cyclic, hipStream_t st) {
     return launch_pascal_fused_t<float, float>(M, maxt, A, B, C, D, X, n_sys, n_row, Q, S, cyclic,
-                                               st);
+                                               0, st);
 }
 
 int thomas_fused_rows(int n_row) {
     return n_row <= 8 ? 8 : n_row <= 16 ? 16 : n_row <= 32 ? 32 : n_row <= 64 ? 64 : 0;
 }
 
-hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
-                               int n_sys, int n_row, hipStream_t st) {
+template <int CSH>
+static hipError_t launch_thomas_fused_t(const double* A, const double* B, const double* C,
+                                        double* D, int n_sys, int n_row, hipStream_t st) {
     const dim3 grid((n_sys + 255) / 256), block(256);
     switch (thomas_fused_rows(n_row)) {
-        case 8: hipLaunchKernelGGL((k_thomas_fused<8, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
-        case 16: hipLaunchKernelGGL((k_thomas_fused<16, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
-        case 32: hipLaunchKernelGGL((k_thomas_fused<32, 8>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
-        case 64: hipLaunchKernelGGL((k_thomas_fused<64, 4>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 8: hipLaunchKernelGGL((k_thomas_fused<8, 8, CSH>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 16: hipLaunchKernelGGL((k_thomas_fused<16, 8, CSH>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 32: hipLaunchKernelGGL((k_thomas_fused<32, 8, CSH>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
+        case 64: hipLaunchKernelGGL((k_thomas_fused<64, 4, CSH>), grid, block, 0, st, A, B, C, D, n_sys, n_row); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,
+                               int n_sys, int n_row, int csh, hipStream_t st) {
+    return csh ? launch_thomas_fused_t<1>(A, B, C, D, n_sys, n_row, st)
+               : launch_thomas_fused_t<0>(A, B, C, D, n_sys, n_row, st);
 }
 
 hipError_t launch_thomas_cyclic_fused(const double* A, const double* B, const double* C, double* D,
@@ -1576,14 +1617,20 @@ size_t filtered_fused_lds(int P, int S) { return sizeof(double) * 4 * (size_t)P 
 
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
-                                 int P, int S, int cyclic, const int* Jp, double* xbuf,
+                                 int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
                                  double* partial, hipStream_t st) {
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * P);
     const size_t lds = (pass == 2) ? 0 : filtered_fused_lds(P, S);
-#define LAUNCH(m_, t_, v_, p_, pf_, w_)                                                    \
-    hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_>), grid, block, lds, st, A, \
-                       B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial)
+#define LAUNCH(m_, t_, v_, p_, pf_, w_)                                                        \
+    do {                                                                                     \
+        if (csh)                                                                             \
+            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 1>), grid, block, lds, \
+                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial); \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 0>), grid, block, lds, \
+                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial); \
+    } while (0)
 #define X(m_, t_, pf_, w_)                                \
     if (M == m_ && maxt == t_) {                          \
         if (version == 2)                                 \

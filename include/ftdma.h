@@ -116,7 +116,8 @@ ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const fl
 
 /* FilteredTDMACUDA::last_J (filtered_tdma_cuda.cu:83-87), one value per
  * partition: J of the most recent filtered solve (blocking device->host
- * copy of n <= partitions ints). -1 where no filtered solve ran yet. */
+ * copy of n <= 2 * partitions ints; entries from P on are the imaginary
+ * part's J of a complex solve). -1 where no filtered solve ran yet. */
 ftd_status ftd_last_J(ftd_handle* h, int* J, int n);
 
 /* TdmaBackendGPU::last_gpu_ms (tdma_backend_gpu.hpp:60-63): device time of
@@ -139,6 +140,18 @@ double ftd_last_comm_ms(const ftd_handle* h);
  * Returns FTD_ERR_UNSUPPORTED for combinations that are not compiled. */
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
                           int systems_per_tile, int lanes_per_system);
+
+/* Extension (complex right-hand sides; the reference's channel pressure
+ * solve, apps/channel_cpu/PressureSolver.cpp:412-470, solves the real and
+ * imaginary parts as two real systems with copies of one coefficient set):
+ * Z is a complex128 [n_row x n_sys] array, (re, im) interleaved, overwritten
+ * with x; A, B, C are the shared real coefficients [n_row x n_sys]. Both parts
+ * run in one pass that reads A, B, C once (56 B per complex row instead of
+ * 80 B), with results identical to two ftd_solve calls on the parts. For the
+ * filtered kinds each part keeps its own J: ftd_last_J entries 0..P-1 (real)
+ * and P..2P-1 (imaginary). Non-periodic; register-resident kernels only
+ * (FTD_ERR_UNSUPPORTED where a partition exceeds 64 rows). */
+ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, double* Z);
 
 /* Extension (uneven batches; no counterpart in the reference, whose batches
  * share one n_row): n_systems independent systems, system s occupying
