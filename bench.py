@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the uneven-batch and 256^3 ADI-step (C4) legs")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the complex channel-Poisson (C5) leg")
     return ap.parse_args()
 
 
@@ -187,6 +189,57 @@ def c4_adi(steps):
             "rows_per_s": 3 * n ** 3 / (wall / steps)}
 
 
+def c5_channel(steps, warmup, torch):
+    """Channel pressure systems: 128 x 128 (kx, ky) modes, n_row = 129 on a
+    tanh grid (gamma = 2), pinned zero mode, complex128 RHS ~ N(0, 1).
+    One ftd_solve_complex (A, B, C read once: 56 B per complex row) against
+    the reference's two real solves of the same coefficients (80 B)."""
+    from filtered_tdma_amd import TdmaBackendGPU, cases
+
+    A, B, C, Z = cases.channel_poisson(128, 128, 129, seed=0)
+    n_row, n_sys = Z.shape
+    tA, tB, tC = (torch.from_numpy(v).cuda() for v in (A, B, C))
+    tZ0 = torch.from_numpy(Z).cuda()
+    tR0 = torch.from_numpy(np.ascontiguousarray(Z.real)).cuda()
+    tI0 = torch.from_numpy(np.ascontiguousarray(Z.imag)).cuda()
+    zc = [torch.empty_like(tZ0) for _ in range(steps)]
+    rc = [torch.empty_like(tR0) for _ in range(steps)]
+    ic = [torch.empty_like(tI0) for _ in range(steps)]
+    be = TdmaBackendGPU("pascal", n_sys, n_row, 1)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, copies_src):
+        for k in range(warmup):
+            for dst, src in copies_src:
+                dst[k % steps].copy_(src)
+            fn(k % steps)
+        for dst, src in copies_src:
+            for c in dst:
+                c.copy_(src)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        for k in range(steps):
+            ev[k][0].record(stream)
+            fn(k)
+            ev[k][1].record(stream)
+        torch.cuda.synchronize()
+        return statistics.mean(a.elapsed_time(b) for a, b in ev)
+
+    ms_c = timed(lambda k: be.solve_complex(tA, tB, tC, zc[k]), [(zc, tZ0)])
+    path = be.last_path()
+    ms_2 = timed(lambda k: (be.solve(tA, tB, tC, rc[k]), be.solve(tA, tB, tC, ic[k])),
+                 [(rc, tR0), (ic, tI0)])
+    be.close()
+    rows = n_row * n_sys
+    return {"workload": "channel pressure Poisson, 128x128 modes, n_row=129, tanh grid "
+                        "gamma=2, pinned zero mode, complex128 RHS",
+            "complex_rows": rows, "complex_ms": ms_c, "two_real_ms": ms_2,
+            "complex_rows_per_s": rows / (ms_c * 1e-3),
+            "hbm_frac_algorithmic": 56 * rows / (ms_c * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "path": path}
+
+
 def cpu_baseline(A, B, C, D):
     """The oracle's PaScaL (P = 8) on the host cores, whole batch, in place on
     copies made outside the timed region; reps until ~10 s have passed."""
@@ -277,6 +330,12 @@ def main():
               f"adi 256^3 graph step {c4['adi_256']['ms_per_step']:.4f} ms",
               file=sys.stderr, flush=True)
 
+    c5 = None
+    if not args.no_c5:
+        c5 = c5_channel(args.steps, args.warmup, torch)
+        print(f"[bench] c5 complex: {c5['complex_ms']:.4f} ms vs two real solves "
+              f"{c5['two_real_ms']:.4f} ms ({c5['path']})", file=sys.stderr, flush=True)
+
     head = results["pascal"] if "pascal" in results else next(iter(results.values()))
     achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9
     traffic = None
@@ -313,6 +372,8 @@ def main():
     }
     if c4 is not None:
         line["c4"] = c4
+    if c5 is not None:
+        line["c5"] = c5
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A, B, C, D)
     print(json.dumps(line), flush=True)

@@ -103,3 +103,60 @@ def uneven_batch(n_rows, r_lo: float, r_hi: float, seed: int = 0):
     B = 1.0 + 2.0 * rr
     D = -1.0 + 2.0 * uniform(seed, 8, np.arange(total, dtype=np.uint64))
     return off, n_rows, A, B, C, D, r
+
+
+def tanh_faces(n: int, gamma: float, L: float) -> np.ndarray:
+    """Face positions x_f[0..n] of a tanh-stretched wall-normal grid
+    (apps/channel_cpu/Grid.cpp:16-30)."""
+    zeta = np.arange(n + 1, dtype=np.float64) / n
+    return 0.5 * L * (1.0 - np.tanh(gamma * (1.0 - 2.0 * zeta)) / np.tanh(gamma))
+
+
+def channel_poisson(n1: int = 128, n2: int = 128, n_row: int = 129, gamma: float = 2.0,
+                    Lx: float = 2.0 * np.pi, Ly: float = np.pi, Lz: float = 2.0, seed: int = 0):
+    """Wall-normal systems of the channel pressure Poisson solve, one per
+    (kx, ky) mode, as PressureSolver::solve_tdma_z_ builds them
+    (apps/channel_cpu/PressureSolver.cpp:425-456; wavenumbers :72-78; grid
+    spacings Grid.cpp:47-95 with mirrored ghost centres):
+      a_k = 1/(dz_k dmz_k) (0 at the lower wall), c_k = 1/(dz_k dmz_{k+1})
+      (0 at the upper wall), b_k = -(a_k + c_k) - (kx^2 + ky^2),
+    mode s = i + n1 * j, the (0, 0) mode pinned (row 0: a = c = 0, b = 1,
+    rhs 0). Right-hand sides are complex, re and im ~ N(0, 1) (Box-Muller on
+    streams 9 and 10). Returns (A, B, C, Z) with A, B, C [n_row, n1*n2]
+    float64 and Z [n_row, n1*n2] complex128."""
+    xf = tanh_faces(n_row, gamma, Lz)
+    xc = np.empty(n_row + 2)
+    xc[1:n_row + 1] = 0.5 * (xf[:-1] + xf[1:])
+    xc[0] = xf[0] - (xc[1] - xf[0])
+    xc[n_row + 1] = xf[n_row] + (xf[n_row] - xc[n_row])
+    dz = np.empty(n_row + 2)
+    dz[1:n_row + 1] = xf[1:] - xf[:-1]
+    dz[0], dz[n_row + 1] = dz[1], dz[n_row]
+    dmz = np.empty(n_row + 2)
+    dmz[1:] = xc[1:] - xc[:-1]
+    dmz[0] = dmz[1]
+    k = np.arange(n_row)
+    a = 1.0 / (dz[k + 1] * dmz[k + 1])
+    c = 1.0 / (dz[k + 1] * dmz[k + 2])
+    a[0] = 0.0
+    c[n_row - 1] = 0.0
+    dx, dy = Lx / n1, Ly / n2
+    kx2 = 2.0 * (1.0 - np.cos(2.0 * np.pi * np.arange(n1) / n1)) / (dx * dx)
+    jj = np.arange(n2)
+    jj = np.where(jj <= n2 // 2, jj, jj - n2)
+    ky2 = 2.0 * (1.0 - np.cos(2.0 * np.pi * jj / n2)) / (dy * dy)
+    lam = (kx2[None, :] + ky2[:, None]).reshape(-1)  # s = i + n1 * j
+    n_sys = n1 * n2
+    A = np.broadcast_to(a[:, None], (n_row, n_sys)).copy()
+    C = np.broadcast_to(c[:, None], (n_row, n_sys)).copy()
+    B = -(a + c)[:, None] - lam[None, :]
+    idx = np.arange(n_row * n_sys, dtype=np.uint64)
+    u1 = np.maximum(uniform(seed, 9, idx), 1e-300)
+    u2 = uniform(seed, 10, idx)
+    rad = np.sqrt(-2.0 * np.log(u1))
+    Z = (rad * np.cos(2.0 * np.pi * u2) + 1j * rad * np.sin(2.0 * np.pi * u2)).reshape(n_row, n_sys)
+    A[0, 0] = 0.0
+    B[0, 0] = 1.0
+    C[0, 0] = 0.0
+    Z[0, 0] = 0.0
+    return A, np.ascontiguousarray(B), C, np.ascontiguousarray(Z)

@@ -3,24 +3,27 @@
 //
 // Layout: system s occupies elements off_s .. off_s + N_s - 1 of each of A,
 // B, C, D (rows of one system contiguous, systems anywhere in the arrays).
-// D is overwritten with x; A, B, C are read only.
+// D is overwritten with x; A and B are read only.
 //
 // Work split (planned on the host once per batch layout):
-//   * a system of N_s <= 64 rows is one lane: plain Thomas with row 0
+//   * a system of N_s <= 32 rows is one lane: plain Thomas with row 0
 //     divided by b (the reference's single-rank path, tdma_local_cuda.cu:32-64);
-//   * a longer system is cut by para_range into Q_s = ceil(N_s / 64) chunks,
-//     one lane each, on consecutive lanes of ONE wave: modified Thomas per
-//     chunk, the 2Q_s-row reduced system (B == 1) solved in the wave's LDS by
-//     the lane of chunk 0, then the interior update -- PaScaL with P = Q_s
-//     (tdma_local_cuda.cu:134-224) for that system.
-// A work item is one wave of 64 lane descriptors, of one kind only (chunked
-// systems, or single-chunk systems), so the two code paths never diverge
-// inside a wave. The chunked items come first, then the single-chunk ones,
-// each range ordered by estimated cost, longest first, and dealt round-robin
-// to the waves of a persistent grid. Waves are independent: no workgroup
-// barrier anywhere.
+//   * a longer system is cut by para_range into Q_s = ceil(N_s / 32) chunks,
+//     one lane each, on consecutive lanes of ONE work item: modified Thomas
+//     per chunk, the 2Q_s-row reduced system (B == 1) solved in the item's
+//     LDS by the lane of chunk 0, then the interior update -- PaScaL with
+//     P = Q_s (tdma_local_cuda.cu:134-224) for that system.
+// A work item is one workgroup of 128 lanes (two waves) of one kind only
+// (chunked systems, or single-chunk systems). The chunked items come first,
+// then the single-chunk ones, each range ordered by estimated cost, longest
+// first, and dealt round-robin to the workgroups of a persistent grid.
 //
-// Systems longer than 64 * 64 rows take a lane-per-system Thomas in global
+// Rows of a chunk are contiguous in memory, so the lanes of a wave reading
+// row j of their chunks would touch 64 scattered cache lines per load. The
+// rows are therefore staged through LDS in coalesced blocks (below), and the
+// elimination reads them from there.
+//
+// Systems longer than 128 * 32 rows take a lane-per-system Thomas in global
 // memory (C is used as scratch for c', which the reference contract allows).
 #include <hip/hip_runtime.h>
 
@@ -41,83 +44,203 @@ namespace ftd {
 
 namespace {
 
-constexpr int kChunk = 64;  // rows per lane
+constexpr int kChunk = 32;   // rows per lane
+constexpr int kLanes = 128;  // lanes per work item (two waves)
 
 __device__ __forceinline__ void wave_lds_sync() {
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
 }
 
-#define FTD_BLOAD(j)                                  \
-    do {                                              \
-        const int jj_ = (j) < cl ? (j) : cl - 1;      \
-        const size_t o_ = base + (size_t)jj_;         \
-        ra[(j)] = FTD_LD(A + o_);                     \
-        rb[(j)] = FTD_LD(B + o_);                     \
-        rc[(j)] = FTD_LD(C + o_);                     \
-        rd[(j)] = FTD_LD(D + o_);                     \
-    } while (0)
+// workgroup barrier for LDS traffic only (DMAs in flight are left alone)
+__device__ __forceinline__ void group_lds_sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// ---------------------------------------------------------------------------
+// Row staging. Blocks of RB = 8 rows of a wave's 64 chunks are brought into
+// LDS by direct global->LDS dword loads: one instruction covers 4 chunks x 8
+// rows (64 contiguous bytes per chunk), 16 instructions per array and block.
+// Instruction k of array X lands at
+//   stage + X * ARR + k * SEG  (SEG = 264: 256 bytes + 8 of padding),
+// chunk c = 4k + i at + i * 64, row r of the block at + r * 8; the padding
+// spreads the 64 chunks over the LDS banks (at most 2-way conflicts).
+// Two buffers per wave: block b + 1 is in flight while block b is used, so
+// the wait for block b is vmcnt(63) (block b + 1's 64 DMAs were issued after
+// it), or vmcnt(0) for the last block of an item.
+// ---------------------------------------------------------------------------
+constexpr int kRB = 8;           // rows per staged block
+constexpr int kSEG = 264;        // bytes per instruction slot
+constexpr int kARR = 16 * kSEG;  // bytes per array per buffer
+constexpr int kBUF = 4 * kARR;   // bytes per buffer
+static_assert(kBUF + (kRB - 1) * 8 + 3 * kARR < 65536, "ds_read immediate offsets");
+
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef const __attribute__((address_space(3))) double* lds_cdptr;
+typedef int ftd_int4 __attribute__((ext_vector_type(4)));
+
+struct Stage {
+    unsigned char* buf0;    // this wave's buffer 0 (buffer 1 follows at + kBUF)
+    const BatchLane* desc;  // this wave's 64 lane descriptors (LDS)
+    lds_cdptr lane;         // this lane's chunk slot in buffer 0
+};
+
+// DMA of block b (rows 8b .. 8b+7 of every chunk of the wave, clamped to the
+// chunk) into buffer t
+__device__ __forceinline__ void stage_issue(const Stage& st, int t, int b, int l,
+                                            const double* A, const double* B, const double* C,
+                                            const double* D) {
+    const int r = (l & 15) >> 1, h = l & 1;
+    // rolled: a few registers at a time next to the elimination state
+#pragma unroll 1
+    for (int k = 0; k < 16; ++k) {
+        // descriptor of chunk 4k + l/16, read in place (opaque, so that the
+        // 16 descriptors are not hoisted into registers for the whole sweep)
+        ftd_int4 cd;
+        asm volatile(
+            "ds_read_b128 %0, %1\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(cd)
+            : "v"((const __attribute__((address_space(3))) BatchLane*)(st.desc + 4 * k + (l >> 4)))
+            : "memory");
+        const int64_t off = (int64_t)(((uint64_t)(uint32_t)cd.y << 32) | (uint32_t)cd.x);
+        const int mm = cd.z > 0 ? cd.z : 1;
+        const int row = kRB * b + r < mm ? kRB * b + r : mm - 1;
+        const size_t e = (size_t)off + (size_t)row;
+        unsigned char* dst = st.buf0 + t * kBUF + k * kSEG;
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)(A + e) + 4 * h),
+                                         (lds_vptr)(dst + 0 * kARR), 4, 0, 2);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)(B + e) + 4 * h),
+                                         (lds_vptr)(dst + 1 * kARR), 4, 0, 2);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)(C + e) + 4 * h),
+                                         (lds_vptr)(dst + 2 * kARR), 4, 0, 2);
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)(D + e) + 4 * h),
+                                         (lds_vptr)(dst + 3 * kARR), 4, 0, 2);
+    }
+}
+
+// a, b, c, d of row r of the block in buffer t, for this lane's chunk: one
+// base address per lane and the buffer, row and array as immediate offsets,
+// so that no per-row address lives in a register; inline asm keeps the
+// compiler's own (draining) waits out of the staging
+template <int OFF>
+__device__ __forceinline__ void stage_read_at(lds_cdptr lane, double& a, double& b, double& c,
+                                              double& d) {
+    asm volatile(
+        "ds_read_b64 %0, %4 offset:%5\n\t"
+        "ds_read_b64 %1, %4 offset:%6\n\t"
+        "ds_read_b64 %2, %4 offset:%7\n\t"
+        "ds_read_b64 %3, %4 offset:%8\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+        : "v"(lane), "n"(OFF), "n"(OFF + kARR), "n"(OFF + 2 * kARR), "n"(OFF + 3 * kARR)
+        : "memory");
+}
+
+template <int T, int R>
+__device__ __forceinline__ void stage_read(lds_cdptr lane, double& a, double& b, double& c,
+                                           double& d) {
+    stage_read_at<T * kBUF + R * 8>(lane, a, b, c, d);
+}
+
+__device__ __forceinline__ void vm_wait_block(bool more) {
+    if (more)
+        asm volatile("s_waitcnt vmcnt(63)" ::: "memory");
+    else
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Forward sweep over the staged blocks, block by block: Block<B>::run calls
+// step(j, a, b, c, d) for the 8 rows of block B and recurses, so that every
+// row index, buffer and LDS offset is a compile-time constant.
+template <int B, int NB>
+struct Sweep {
+    template <typename F>
+    __device__ __forceinline__ static void run(const Stage& st, int l, int nblk, const double* A,
+                                               const double* Bm, const double* C,
+                                               const double* D, F& step) {
+        if (B < nblk) {
+            vm_wait_block(B + 1 < nblk);
+            double a, b, c, d;
+            stage_read<B & 1, 0>(st.lane, a, b, c, d); step(B * kRB + 0, a, b, c, d);
+            stage_read<B & 1, 1>(st.lane, a, b, c, d); step(B * kRB + 1, a, b, c, d);
+            stage_read<B & 1, 2>(st.lane, a, b, c, d); step(B * kRB + 2, a, b, c, d);
+            stage_read<B & 1, 3>(st.lane, a, b, c, d); step(B * kRB + 3, a, b, c, d);
+            stage_read<B & 1, 4>(st.lane, a, b, c, d); step(B * kRB + 4, a, b, c, d);
+            stage_read<B & 1, 5>(st.lane, a, b, c, d); step(B * kRB + 5, a, b, c, d);
+            stage_read<B & 1, 6>(st.lane, a, b, c, d); step(B * kRB + 6, a, b, c, d);
+            stage_read<B & 1, 7>(st.lane, a, b, c, d); step(B * kRB + 7, a, b, c, d);
+            if (B + 2 < nblk) stage_issue(st, B & 1, B + 2, l, A, Bm, C, D);
+            Sweep<B + 1, NB>::run(st, l, nblk, A, Bm, C, D, step);
+        }
+    }
+};
+template <int NB>
+struct Sweep<NB, NB> {
+    template <typename F>
+    __device__ __forceinline__ static void run(const Stage&, int, int, const double*,
+                                               const double*, const double*, const double*, F&) {}
+};
+
+template <int M, typename F>
+__device__ __forceinline__ void staged_sweep(const Stage& st, int l, int nblk, const double* A,
+                                             const double* B, const double* C, const double* D,
+                                             F& step) {
+    stage_issue(st, 0, 0, l, A, B, C, D);
+    if (nblk > 1) stage_issue(st, 1, 1, l, A, B, C, D);
+    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, step);
+}
 
 // single-chunk systems: plain Thomas, the whole system in one lane
-template <int M, int PF>
-__device__ __forceinline__ void thomas_item(const double* __restrict__ A,
-                                            const double* __restrict__ B,
-                                            const double* __restrict__ C, double* __restrict__ D,
-                                            const BatchLane& L) {
-    int cnt = L.m;
-    asm volatile("" : "+v"(cnt));  // keeps the row clamps inside the item loop
-    const int cl = cnt > 0 ? cnt : 1;
-    const size_t base = (size_t)L.off;
-    double ra[M], rb[M], rc[M], rd[M];
+template <int M>
+struct ThomasStep {
     double c[M], d[M];
-#pragma unroll
-    for (int j = 0; j < PF && j < M; ++j) FTD_BLOAD(j);
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        if (j + PF < M) FTD_BLOAD(j + PF);
+    int cnt;
+    __device__ __forceinline__ void operator()(int j, double aj, double bj, double cj, double dj) {
         if (j < cnt) {
             if (j == 0) {
-                d[0] = rd[0] / rb[0];
-                c[0] = rc[0] / rb[0];
+                d[0] = dj / bj;
+                c[0] = cj / bj;
             } else {
-                const double r = 1.0 / (rb[j] - ra[j] * c[j - 1]);
-                d[j] = r * (rd[j] - ra[j] * d[j - 1]);
-                c[j] = r * rc[j];
+                const double r = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = r * (dj - aj * d[j - 1]);
+                c[j] = r * cj;
             }
         }
     }
-#pragma unroll
-    for (int j = M - 2; j >= 0; --j)
-        if (j <= cnt - 2) d[j] = d[j] - c[j] * d[j + 1];
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < cnt) FTD_ST(d[j], D + base + (size_t)j);
-}
+};
 
-// one chunk of a chunked system: modified Thomas (rows 0, 1 divided by b),
-// the system's reduced rows solved in this wave's LDS by its chunk-0 lane,
-// interior update
-template <int M, int PF>
-__device__ __forceinline__ void pascal_item(const double* __restrict__ A,
+template <int M>
+__device__ __forceinline__ void thomas_item(const Stage& st, const double* __restrict__ A,
                                             const double* __restrict__ B,
                                             const double* __restrict__ C, double* __restrict__ D,
-                                            const BatchLane& L, int l, double* rA, double* rC,
-                                            double* rD) {
-    int cnt = L.m;
-    asm volatile("" : "+v"(cnt));
-    const int cl = cnt > 0 ? cnt : 1;
+                                            const BatchLane& L, int l, int nblk) {
+    ThomasStep<M> s;
+    s.cnt = L.m;
+    asm volatile("" : "+v"(s.cnt));  // keeps the row tests inside the item loop
     const size_t base = (size_t)L.off;
-    const int q = L.q, Q = L.Q, lead = L.lead;
-    double ra[M], rb[M], rc[M], rd[M];
+    staged_sweep<M>(st, l, nblk, A, B, C, D, s);
+#pragma unroll
+    for (int j = M - 2; j >= 0; --j)
+        if (j <= s.cnt - 2) s.d[j] = s.d[j] - s.c[j] * s.d[j + 1];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < s.cnt) FTD_ST(s.d[j], D + base + (size_t)j);
+}
+
+// one chunk of a chunked system: modified Thomas (rows 0, 1 divided by b)
+template <int M>
+struct PascalStep {
     double a[M], c[M], d[M];
-    double aN = 0.0, cN = 0.0, dN = 0.0;
-#pragma unroll
-    for (int j = 0; j < PF && j < M; ++j) FTD_BLOAD(j);
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        if (j + PF < M) FTD_BLOAD(j + PF);
+    double aN, cN, dN;
+    int cnt;
+    __device__ __forceinline__ void operator()(int j, double aj, double bj, double cj, double dj) {
         if (j < cnt) {
-            const double aj = ra[j], bj = rb[j], cj = rc[j], dj = rd[j];
             if (j < 2) {
                 a[j] = aj / bj;
                 c[j] = cj / bj;
@@ -135,26 +258,44 @@ __device__ __forceinline__ void pascal_item(const double* __restrict__ A,
             }
         }
     }
+};
+
+// t: lane within the item (0..127); the system's reduced rows are solved in
+// the item's LDS by its chunk-0 lane, then every chunk updates its interior
+template <int M>
+__device__ __forceinline__ void pascal_item(const Stage& st, const double* __restrict__ A,
+                                            const double* __restrict__ B,
+                                            const double* __restrict__ C, double* __restrict__ D,
+                                            const BatchLane& L, int l, int t, int nblk, double* rA,
+                                            double* rC, double* rD) {
+    PascalStep<M> s;
+    s.aN = s.cN = s.dN = 0.0;
+    s.cnt = L.m;
+    asm volatile("" : "+v"(s.cnt));
+    const size_t base = (size_t)L.off;
+    const int q = L.q, Q = L.Q, lead = L.lead;
+    staged_sweep<M>(st, l, nblk, A, B, C, D, s);
+    const int cnt = s.cnt;
 #pragma unroll
     for (int j = M - 3; j >= 1; --j) {
         if (j <= cnt - 3) {
-            const double cj = c[j];
-            d[j] = d[j] - cj * d[j + 1];
-            a[j] = a[j] - cj * a[j + 1];
-            c[j] = -cj * c[j + 1];
+            const double cj = s.c[j];
+            s.d[j] = s.d[j] - cj * s.d[j + 1];
+            s.a[j] = s.a[j] - cj * s.a[j + 1];
+            s.c[j] = -cj * s.c[j + 1];
         }
     }
     const bool active = cnt > 0;
     if (active) {
-        const double r = 1.0 / (1.0 - a[1] * c[0]);
-        rA[2 * l] = r * a[0];
-        rC[2 * l] = -r * c[0] * c[1];
-        rD[2 * l] = r * (d[0] - c[0] * d[1]);
-        rA[2 * l + 1] = aN;
-        rC[2 * l + 1] = cN;
-        rD[2 * l + 1] = dN;
+        const double r = 1.0 / (1.0 - s.a[1] * s.c[0]);
+        rA[2 * t] = r * s.a[0];
+        rC[2 * t] = -r * s.c[0] * s.c[1];
+        rD[2 * t] = r * (s.d[0] - s.c[0] * s.d[1]);
+        rA[2 * t + 1] = s.aN;
+        rC[2 * t + 1] = s.cN;
+        rD[2 * t + 1] = s.dN;
     }
-    wave_lds_sync();
+    group_lds_sync();
     if (active && q == 0) {
         // reduced system of this system: rows 2*lead .. 2*(lead+Q)-1
         const int r0 = 2 * lead, NR = 2 * Q;
@@ -173,9 +314,9 @@ __device__ __forceinline__ void pascal_item(const double* __restrict__ A,
             rD[r0 + k] = dn;
         }
     }
-    wave_lds_sync();
+    group_lds_sync();
     if (active) {
-        const double x0 = rD[2 * l], xN = rD[2 * l + 1];
+        const double x0 = rD[2 * t], xN = rD[2 * t + 1];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             if (j < cnt) {
@@ -185,34 +326,49 @@ __device__ __forceinline__ void pascal_item(const double* __restrict__ A,
                 else if (j == cnt - 1)
                     x = xN;
                 else
-                    x = d[j] - (a[j] * x0 + c[j] * xN);
+                    x = s.d[j] - (s.a[j] * x0 + s.c[j] * xN);
                 FTD_ST(x, D + base + (size_t)j);
             }
         }
     }
-    wave_lds_sync();  // the reduced rows are read before the next item writes them
+    group_lds_sync();  // the reduced rows are read before the next item writes them
 }
-#undef FTD_BLOAD
 
 // Items [0, n_chunked) hold chunked systems, [n_chunked, n_items) single-
-// chunk ones; wave g of the grid takes items g, g + G, g + 2G, ... of each
-// range in turn.
-template <int M, int PF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void k_batched(
+// chunk ones; workgroup g takes items g, g + G, g + 2G, ... of each range.
+template <int M>
+__global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) void k_batched(
     const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
     double* __restrict__ D, const BatchLane* __restrict__ lanes, int n_chunked, int n_items) {
-    __shared__ double red[4][3][2 * 64];
-    const int w = threadIdx.x >> 6;
-    const int l = threadIdx.x & 63;
-    const int G = gridDim.x * 4;
-    const int g = blockIdx.x * 4 + w;
-    for (int item = g; item < n_chunked; item += G)
-        pascal_item<M, PF>(A, B, C, D, lanes[(size_t)item * 64 + l], l, red[w][0], red[w][1],
-                           red[w][2]);
+    __shared__ __attribute__((aligned(16))) unsigned char stage[2][2 * kBUF];
+    __shared__ __attribute__((aligned(16))) BatchLane desc[kLanes];
+    __shared__ double red[3][2 * kLanes];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int l = t & 63;
+    const int G = gridDim.x;
+    const int g = blockIdx.x;
+    Stage st;
+    st.buf0 = stage[w];
+    st.desc = desc + 64 * w;
+    st.lane = (lds_cdptr)(stage[w] + (l >> 2) * kSEG + (l & 3) * 64);
+    for (int item = g; item < n_chunked; item += G) {
+        const BatchLane L = lanes[(size_t)item * kLanes + t];
+        desc[t] = L;
+        const int nblk = __builtin_amdgcn_readfirstlane((wave_max(L.m) + kRB - 1) / kRB);
+        wave_lds_sync();
+        pascal_item<M>(st, A, B, C, D, L, l, t, nblk, red[0], red[1], red[2]);
+    }
     // the single-chunk items continue the deal where the chunked ones ended
     const int start = n_chunked + ((g - n_chunked % G) + G) % G;
-    for (int item = start; item < n_items; item += G)
-        thomas_item<M, PF>(A, B, C, D, lanes[(size_t)item * 64 + l]);
+    for (int item = start; item < n_items; item += G) {
+        const BatchLane L = lanes[(size_t)item * kLanes + t];
+        wave_lds_sync();  // the previous item's descriptors are no longer read
+        desc[t] = L;
+        const int nblk = __builtin_amdgcn_readfirstlane((wave_max(L.m) + kRB - 1) / kRB);
+        wave_lds_sync();
+        thomas_item<M>(st, A, B, C, D, L, l, nblk);
+    }
 }
 
 // one lane per long system, c' kept in C
@@ -266,7 +422,7 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
         const int N = n_rows[s];
         if (N < 1) return -1;
         const int Q = (N + kChunk - 1) / kChunk;
-        if (Q > 64) {
+        if (Q > kLanes) {
             long_off.push_back(row_offsets[s]);
             long_n.push_back(N);
         } else if (Q >= 2) {
@@ -317,18 +473,18 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
             maxQ = std::max(maxQ, Q);
         };
         put(multi[lo++]);
-        while (lo < hi && used + chunks(multi[lo]) <= 64) put(multi[lo++]);
-        while (lo < hi && used + chunks(multi[hi - 1]) <= 64) put(multi[--hi]);
+        while (lo < hi && used + chunks(multi[lo]) <= kLanes) put(multi[lo++]);
+        while (lo < hi && used + chunks(multi[hi - 1]) <= kLanes) put(multi[--hi]);
         const int64_t pad = it.l.front().off;
-        while ((int)it.l.size() < 64) it.l.push_back(idle(pad));
+        while ((int)it.l.size() < kLanes) it.l.push_back(idle(pad));
         for (auto& b : it.l) b.kind = 1;
         it.cost = maxm + 6.0 * maxQ;  // streamed rows + serial reduced rows
         items.push_back(std::move(it));
     }
-    for (size_t k = 0; k < single.size(); k += 64) {
+    for (size_t k = 0; k < single.size(); k += kLanes) {
         Item it;
         int maxm = 0;
-        for (size_t u = k; u < std::min(single.size(), k + 64); ++u) {
+        for (size_t u = k; u < std::min(single.size(), k + kLanes); ++u) {
             const int s = single[u];
             BatchLane b{};
             b.off = row_offsets[s];
@@ -341,7 +497,7 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
             maxm = std::max(maxm, n_rows[s]);
         }
         const int64_t pad = it.l.front().off;
-        while ((int)it.l.size() < 64) {
+        while ((int)it.l.size() < kLanes) {
             BatchLane b = idle(pad);
             b.kind = 0;
             it.l.push_back(b);
@@ -350,12 +506,12 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
         items.push_back(std::move(it));
     }
     // chunked items first, each range longest first
-    const size_t nch = items.size() - (single.size() + 63) / 64;
+    const size_t nch = items.size() - (single.size() + kLanes - 1) / kLanes;
     auto by_cost = [](const Item& x, const Item& y) { return x.cost > y.cost; };
     std::stable_sort(items.begin(), items.begin() + nch, by_cost);
     std::stable_sort(items.begin() + nch, items.end(), by_cost);
     n_chunked = (int)nch;
-    lanes.reserve(items.size() * 64);
+    lanes.reserve(items.size() * kLanes);
     for (auto& it : items) lanes.insert(lanes.end(), it.l.begin(), it.l.end());
     return (int)items.size();
 }
@@ -365,8 +521,8 @@ int batched_resident_blocks() {
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
     if (cached[dev]) return cached[dev];
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batched<kChunk, 4>,
-                                                     256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batched<kChunk>,
+                                                     kLanes, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -381,8 +537,8 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st) {
     if (n_items > 0) {
-        const int blocks = std::min(batched_resident_blocks(), (n_items + 3) / 4);
-        hipLaunchKernelGGL((k_batched<kChunk, 4>), dim3(blocks), dim3(256), 0, st, A, B, C, D,
+        const int blocks = std::min(batched_resident_blocks(), n_items);
+        hipLaunchKernelGGL((k_batched<kChunk>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D,
                            lanes, n_chunked, n_items);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -157,10 +157,10 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
  * share one n_row): n_systems independent systems, system s occupying
  * elements row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of each of A, B,
  * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
- * a system of <= 64 rows is plain Thomas, a longer one PaScaL with
- * ceil(n_rows/64) partitions inside one wave; all in one persistent launch
- * (systems above 4096 rows in a second, lane-per-system launch that uses C
- * as scratch). D is overwritten with x; A and B are read only. The plan is
+ * a system of <= 32 rows is plain Thomas, a longer one PaScaL with
+ * ceil(n_rows/32) partitions inside one 128-lane workgroup; all in one
+ * persistent launch (systems above 4096 rows in a second, lane-per-system
+ * launch that uses C as scratch). D is overwritten with x; A and B are read only. The plan is
  * made on the host once per layout and kept on the handle; a call with a
  * new layout waits for the handle's stream first. The handle's kind and
  * sizes are not used. */

@@ -1,8 +1,8 @@
 """Uneven batches (ftd_solve_batched) against the oracle, system by system.
 
-A system of N <= 64 rows is plain Thomas in one lane, a longer one (up to
-4096 rows) PaScaL with Q = ceil(N/64) partitions inside one wave, a longer
-one still the lane-per-system Thomas kernel. Each is compared bit for bit
+A system of N <= 32 rows is plain Thomas in one lane, a longer one (up to
+4096 rows) PaScaL with Q = ceil(N/32) partitions inside one 128-lane work
+item, a longer one still the lane-per-system Thomas kernel. Each is compared bit for bit
 with the oracle's solve of the same partitioning (GPU semantics), and with
 serial Thomas within 1e-13 relative."""
 import sys
@@ -49,8 +49,8 @@ def oracle_solve(off, n_rows, A, B, C, D):
         o, n = int(off[s]), int(n_rows[s])
         sl = slice(o, o + n)
         a, b, c, d = (v[sl].reshape(n, 1).copy() for v in (A, B, C, D))
-        Q = (n + 63) // 64
-        if Q == 1 or Q > 64:
+        Q = (n + 31) // 32
+        if Q == 1 or Q > 128:
             x = O.thomas(a, b, c, d, sem=O.SEM_GPU)
         else:
             x = O.pascal(a, b, c, d, Q, sem=O.SEM_GPU)
@@ -74,7 +74,8 @@ def run_batched(be, off, n_rows, A, B, C, D):
     return tA.cpu().numpy(), tB.cpu().numpy(), tD.cpu().numpy(), be.last_path()
 
 
-EDGE = [1, 2, 3, 4, 16, 63, 64, 65, 66, 127, 128, 129, 191, 1000, 4095, 4096, 4097, 6000]
+EDGE = [1, 2, 3, 4, 16, 31, 32, 33, 34, 63, 64, 65, 66, 127, 128, 129, 191, 1000, 2048, 2049,
+        4095, 4096, 4097, 6000]
 
 
 def test_batched_uneven_bitwise():
