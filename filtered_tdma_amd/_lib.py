@@ -19,7 +19,7 @@ EXPORTS = (
     "ftd_last_path", "ftd_last_error_string", "ftd_solve_f32c", "ftd_solve_f32",
     "ftd_solve_batched", "ftd_solve_complex",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
-    "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string",
+    "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
 )
 
 _lib = None
@@ -76,11 +76,12 @@ def load() -> ctypes.CDLL:
     L.ftd_heat_step.argtypes = [vp, c_int, c_int]
     L.ftd_heat_field.argtypes = [vp, vp]
     L.ftd_heat_l2.argtypes = [vp, ctypes.POINTER(c_double)]
+    L.ftd_heat_set_fused.argtypes = [vp, c_int]
     L.ftd_heat_last_step_ms.argtypes = [vp]
     L.ftd_heat_last_step_ms.restype = c_double
     L.ftd_heat_last_error_string.restype = ctypes.c_char_p
     for name in ("ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step",
-                 "ftd_heat_field", "ftd_heat_l2"):
+                 "ftd_heat_field", "ftd_heat_l2", "ftd_heat_set_fused"):
         getattr(L, name).restype = c_int
     for name in ("ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr", "ftd_solve_cyclic",
                  "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",

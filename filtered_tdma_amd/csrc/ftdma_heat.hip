@@ -159,6 +159,266 @@ __global__ void k_build(const double* __restrict__ th, const double* __restrict_
     D[o] = d;
 }
 
+// ---------------------------------------------------------------------------
+// Fused sweeps (PaScaL kind, Dirichlet walls): the coefficients of a sweep
+// are the stencil constants, so they are generated in the kernel instead of
+// being written to A, B, C and read back; the right-hand side is read from
+// the field and x is written straight into it. A sweep then moves 16 B per
+// row (read d, write x) instead of build + solve + scatter (~96 B per row).
+// The arithmetic is k_build's followed by the solve's, operation for
+// operation, with the partition count Q the solver would pick for the same
+// sweep (Q = 1: plain Thomas), so the field is bit-identical.
+//
+// Row j of partition q of a system: a = -sa, b = 1 - sb, c = -sc,
+// d = rhs; on the system's first row b += sa, d += sa (ghost + cell); on its
+// last row b += sc, d += sc (ghost + cell) (solve_theta.cpp:146-186).
+// ---------------------------------------------------------------------------
+
+// the generated row; first / last: it is the system's first / last row,
+// lo / hi the cells of those rows, rs the row stride in the field, th the
+// field holding the ghost values
+__device__ __forceinline__ void adi_row(bool first, bool last, double sa, double sb, double sc,
+                                        double d_in, const double* __restrict__ th, size_t lo,
+                                        size_t hi, size_t rs, double& a, double& b, double& c,
+                                        double& d) {
+    a = -sa;
+    b = 1.0 - sb;
+    c = -sc;
+    d = d_in;
+    if (first) {
+        d += sa * (th[lo - rs] + th[lo]);
+        b += sa;
+    }
+    if (last) {
+        d += sc * (th[hi + rs] + th[hi]);
+        b += sc;
+    }
+}
+
+// d of row j of one partition (rows past the partition repeat its last
+// row, so the load carries no branch and can be issued ahead of its use)
+struct RowAt {
+    const double* p;
+    size_t rs;
+    int cnt;
+    __device__ __forceinline__ double operator()(int j) const {
+        const int jj = j < cnt ? j : cnt - 1;
+        return p[(size_t)jj * rs];
+    }
+};
+
+// One lane per (system, partition) with the partition's rows in registers.
+// rows(j) returns d of local row j (clamped to the partition), Store(j, x)
+// writes x of local row j. The d loads run PF rows ahead of their use. Restates k_thomas_fused (Q == 1) and
+// k_pascal_fused (Q >= 2, non-cyclic): rows 0, 1 of a partition divided by
+// b, modified elimination, 2Q-row reduced system in LDS, interior update.
+template <int M, int PF, bool THOMAS, typename Rows, typename Store>
+__device__ __forceinline__ void adi_solve_lane(int N, int Q, int q, int r0, int cnt, int S, int sl,
+                                               double sa, double sb, double sc,
+                                               const double* __restrict__ th, size_t lo,
+                                               size_t hi, size_t rs, bool live, double* lds,
+                                               Rows& rows, Store store) {
+    double rd[M];
+#define FTD_ADI_LOAD(j) rd[(j)] = rows(j)
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_ADI_LOAD(j);
+    if (THOMAS) {
+        double c[M], d[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j + PF < M) FTD_ADI_LOAD(j + PF);
+            if (j < cnt) {
+                double aj, bj, cj, dj;
+                adi_row(j == 0, j == cnt - 1, sa, sb, sc, rd[j], th, lo, hi, rs, aj, bj, cj, dj);
+                if (j == 0) {
+                    d[0] = dj / bj;
+                    c[0] = cj / bj;
+                } else {
+                    const double r = 1.0 / (bj - aj * c[j - 1]);
+                    d[j] = r * (dj - aj * d[j - 1]);
+                    c[j] = r * cj;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = M - 2; j >= 0; --j)
+            if (j <= cnt - 2) d[j] = d[j] - c[j] * d[j + 1];
+        if (live) {
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < cnt) store(j, d[j]);
+        }
+        return;
+    }
+    double a[M], c[M], d[M];
+    double aN = 0.0, cN = 0.0, dN = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_ADI_LOAD(j + PF);
+        if (j < cnt) {
+            double aj, bj, cj, dj;
+            // the system's first row is row 0 of partition 0, its last row the
+            // last row of partition Q-1
+            adi_row(j == 0 && q == 0, j == cnt - 1 && q == Q - 1, sa, sb, sc, rd[j], th, lo, hi,
+                    rs, aj, bj, cj, dj);
+            if (j < 2) {
+                a[j] = aj / bj;
+                c[j] = cj / bj;
+                d[j] = dj / bj;
+            } else {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = -inv * aj * a[j - 1];
+            }
+            if (j == cnt - 1) {
+                aN = a[j];
+                cN = c[j];
+                dN = d[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = M - 3; j >= 1; --j) {
+        if (j <= cnt - 3) {
+            const double cj = c[j];
+            d[j] = d[j] - cj * d[j + 1];
+            a[j] = a[j] - cj * a[j + 1];
+            c[j] = -cj * c[j + 1];
+        }
+    }
+    const double r = 1.0 / (1.0 - a[1] * c[0]);
+    const double d0 = r * (d[0] - c[0] * d[1]);
+    const double a0 = r * a[0];
+    const double c0 = -r * c[0] * c[1];
+    const int NR = 2 * Q;
+    double* rA = lds;
+    double* rC = rA + NR * S;
+    double* rD = rC + NR * S;
+    rA[(2 * q) * S + sl] = a0;
+    rC[(2 * q) * S + sl] = c0;
+    rD[(2 * q) * S + sl] = d0;
+    rA[(2 * q + 1) * S + sl] = aN;
+    rC[(2 * q + 1) * S + sl] = cN;
+    rD[(2 * q + 1) * S + sl] = dN;
+    __syncthreads();
+    if (q == 0) {
+        double* Ar = rA + sl;
+        double* Cr = rC + sl;
+        double* Dr = rD + sl;
+        for (int k = 1; k < NR; ++k) {
+            const double inv = 1.0 / (1.0 - Ar[k * S] * Cr[(k - 1) * S]);
+            Dr[k * S] = inv * (Dr[k * S] - Ar[k * S] * Dr[(k - 1) * S]);
+            Cr[k * S] = inv * Cr[k * S];
+        }
+        for (int k = NR - 2; k >= 0; --k) Dr[k * S] = Dr[k * S] - Cr[k * S] * Dr[(k + 1) * S];
+    }
+    __syncthreads();
+    const double x0 = rD[(2 * q) * S + sl];
+    const double xN = rD[(2 * q + 1) * S + sl];
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < cnt) {
+                double x;
+                if (j == 0)
+                    x = x0;
+                else if (j == cnt - 1)
+                    x = xN;
+                else
+                    x = d[j] - (a[j] * x0 + c[j] * xN);
+                store(j, x);
+            }
+        }
+    }
+#undef FTD_ADI_LOAD
+}
+
+// Z (dir 2) and Y (dir 1) sweeps: systems consecutive in x, so a wave's row
+// loads are contiguous; rows strided by a plane (Z) or a line (Y). The
+// solution goes back into rhs.
+template <int M, bool THOMAS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_yz(const double* __restrict__ th, double* rhs, Dims g,
+                                                int nz, int dir, double sa, double sb, double sc,
+                                                int Q, int lgS) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int n_sys = g.nx * (dir == 2 ? g.ny : nz);
+    const int N = dir == 2 ? nz : g.ny;
+    const int s_raw = blockIdx.x * S + sl;
+    const bool live = s_raw < n_sys;
+    const int s = live ? s_raw : n_sys - 1;
+    const int ii = s % g.nx + 1, oo = s / g.nx + 1;
+    const size_t rs = dir == 2 ? (size_t)g.nx1 * g.ny1 : (size_t)g.nx1;
+    const size_t here0 = dir == 2 ? cell(g, ii, oo, 1) : cell(g, ii, 1, oo);
+    int r0, cnt;
+    {
+        const int base = N / Q, rem = N % Q;
+        r0 = q * base + (q < rem ? q : rem);
+        cnt = base + (q < rem ? 1 : 0);
+    }
+    double* const pbase = rhs + here0 + (size_t)r0 * rs;
+    RowAt rows{pbase, rs, cnt};
+    adi_solve_lane<M, 4, THOMAS>(
+        N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1) * rs, rs, live,
+        lds, rows, [&](int j, double x) { pbase[(size_t)j * rs] = x; });
+}
+
+// X sweep (dir 0): the rows of a system are contiguous in x, so the
+// workgroup first copies its S lines of rhs into an LDS tile (coalesced,
+// line stride N + pad with an odd stride in doubles to spread the banks),
+// solves from the tile, writes x back into the tile and copies the tile to
+// theta (coalesced).
+template <int M, bool THOMAS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_x(double* th, const double* __restrict__ rhs, Dims g,
+                                               int nz, double sa, double sb, double sc, int Q,
+                                               int lgS) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int T = blockDim.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int n_sys = g.ny * nz;
+    const int N = g.nx;
+    const int LS = N + ((N & 1) ? 2 : 1);  // odd line stride
+    double* tile = lds;
+    double* red = tile + (size_t)S * LS;
+    const int first = blockIdx.x * S;
+    const int nl = n_sys - first < S ? n_sys - first : S;
+    for (int e = t; e < nl * N; e += T) {
+        const int l = e / N, i = e - l * N;
+        const int s = first + l;
+        tile[l * LS + i] = rhs[cell(g, i + 1, s % g.ny + 1, s / g.ny + 1)];
+    }
+    __syncthreads();
+    const int s_raw = first + sl;
+    const bool live = s_raw < n_sys;
+    const int s = live ? s_raw : n_sys - 1;
+    const size_t here0 = cell(g, 1, s % g.ny + 1, s / g.ny + 1);
+    int r0, cnt;
+    {
+        const int base = N / Q, rem = N % Q;
+        r0 = q * base + (q < rem ? q : rem);
+        cnt = base + (q < rem ? 1 : 0);
+    }
+    const int lt = live ? sl : 0;  // dead lanes read a loaded line
+    double* myline = tile + (size_t)lt * LS + r0;
+    RowAt rows{myline, 1, cnt};
+    adi_solve_lane<M, 2, THOMAS>(
+        N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1), 1, live, red,
+        rows, [&](int j, double x) { myline[j] = x; });
+    __syncthreads();
+    for (int e = t; e < nl * N; e += T) {
+        const int l = e / N, i = e - l * N;
+        const int s2 = first + l;
+        th[cell(g, i + 1, s2 % g.ny + 1, s2 / g.ny + 1)] = tile[l * LS + i];
+    }
+}
+
 // solution of one sweep back into the field (rhs for Z and Y, theta for X)
 __global__ void k_scatter(const double* __restrict__ D, double* __restrict__ dst, Dims g, int dir) {
     const size_t n_in = (size_t)g.nx * g.ny * g.nz;
@@ -227,9 +487,107 @@ struct ftd_heat {
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
+    // fused sweep per direction (x, y, z): enabled, rows per lane, Thomas,
+    // partitions, log2 systems per workgroup, dynamic LDS bytes
+    struct Fused {
+        int on = 0, M = 0, thomas = 0, Q = 0, lgS = 0;
+        size_t lds = 0;
+    } fused[3];
+    int use_fused = 1;
 };
 
 namespace {
+
+// The partition count the sweep's solver picks for (N rows, P partitions):
+// P == 1 -> plain Thomas when N <= 64, else ceil(N / 64) internal
+// partitions (up to 16); P > 1 -> P. Then the fused sweep's shape, or off.
+void plan_fused(ftd_heat* h, int d, int N, int n_sys) {
+    ftd_heat::Fused f;
+    const int P = h->parts[d];
+    if (h->kind != FTD_KIND_PASCAL || h->periodic[d] || N < 3 || n_sys < 1) {
+        h->fused[d] = f;
+        return;
+    }
+    int Q;
+    if (P == 1)
+        Q = N <= 64 ? 1 : (N + 63) / 64;
+    else
+        Q = P;
+    const int m_max = (N + Q - 1) / Q, m_min = N / Q;
+    if (Q > 16 || m_max > 64 || (Q > 1 && m_min < 3)) {
+        h->fused[d] = f;
+        return;
+    }
+    f.Q = Q;
+    f.thomas = Q == 1;
+    f.M = m_max <= 16 ? 16 : m_max <= 32 ? 32 : 64;
+    int S = 64;
+    while (S * Q > 256) S >>= 1;
+    auto lds_of = [&](int S_) {
+        const size_t red = f.thomas ? 0 : (size_t)3 * 2 * Q * S_;
+        const size_t tile = d == 0 ? (size_t)S_ * (N + ((N & 1) ? 2 : 1)) : 0;
+        return sizeof(double) * (red + tile);
+    };
+    while (S > 1 && lds_of(S) > 160 * 1024) S >>= 1;
+    if (lds_of(S) > 160 * 1024) {
+        h->fused[d] = f;
+        return;
+    }
+    int lg = 0;
+    while ((1 << lg) < S) ++lg;
+    f.lgS = lg;
+    f.lds = lds_of(S);
+    f.on = 1;
+    h->fused[d] = f;
+}
+
+template <int M, bool TH>
+hipError_t launch_adi_t(ftd_heat* h, int d, hipStream_t st) {
+    const ftd_heat::Fused& f = h->fused[d];
+    const Dims g = h->g;
+    const int S = 1 << f.lgS;
+    const int n_sys = d == 0 ? g.ny * h->nz : d == 1 ? g.nx * h->nz : g.nx * g.ny;
+    const dim3 grid((n_sys + S - 1) / S), block(S * f.Q);
+    const double sa[3] = {h->s.xa, h->s.ya, h->s.za}, sb[3] = {h->s.xb, h->s.yb, h->s.zb},
+                 sc[3] = {h->s.xc, h->s.yc, h->s.zc};
+    if (d == 0) {
+        hipLaunchKernelGGL((k_adi_x<M, TH>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz,
+                           sa[0], sb[0], sc[0], f.Q, f.lgS);
+    } else {
+        hipLaunchKernelGGL((k_adi_yz<M, TH>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz, d,
+                           sa[d], sb[d], sc[d], f.Q, f.lgS);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adi(ftd_heat* h, int d, hipStream_t st) {
+    const ftd_heat::Fused& f = h->fused[d];
+#define FTD_ADI_CASE(m_)                                                              \
+    if (f.M == m_) return f.thomas ? launch_adi_t<m_, true>(h, d, st)                   \
+                                   : launch_adi_t<m_, false>(h, d, st);
+    FTD_ADI_CASE(16)
+    FTD_ADI_CASE(32)
+    FTD_ADI_CASE(64)
+#undef FTD_ADI_CASE
+    return hipErrorInvalidValue;
+}
+
+// dynamic LDS above the default limit must be declared for the kernel
+template <int M, bool TH>
+hipError_t allow_lds_t(int d, size_t bytes) {
+    const void* k = d == 0 ? (const void*)k_adi_x<M, TH> : (const void*)k_adi_yz<M, TH>;
+    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t allow_lds(const ftd_heat::Fused& f, int d) {
+#define FTD_ADI_CASE(m_)                                                                     \
+    if (f.M == m_) return f.thomas ? allow_lds_t<m_, true>(d, f.lds) : allow_lds_t<m_, false>(d, f.lds);
+    FTD_ADI_CASE(16)
+    FTD_ADI_CASE(32)
+    FTD_ADI_CASE(64)
+#undef FTD_ADI_CASE
+    return hipErrorInvalidValue;
+}
 
 ftd_status enqueue_step(ftd_heat* h) {
     const Dims g = h->g;
@@ -253,6 +611,11 @@ ftd_status enqueue_step(ftd_heat* h) {
                  sc[3] = {h->s.xc, h->s.yc, h->s.zc};
     for (int q = 0; q < 3; ++q) {
         const int d = order[q];
+        if (h->use_fused && h->fused[d].on) {
+            // coefficients generated, rhs read and x written in place
+            HH(launch_adi(h, d, st));
+            continue;
+        }
         hipLaunchKernelGGL(k_build, g1, b1, 0, st, h->th, h->rhs, g, d, sa[d], sb[d], sc[d],
                            h->periodic[d] ? 0 : 1, h->A, h->B, h->C, h->D);
         HH(hipGetLastError());
@@ -350,6 +713,13 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     // x, y, z sweeps: n_sys = ny*nz, nx*nz, nx*ny
     const int ns[3] = {ny * nz, nx * nz, nx * ny}, nr[3] = {nx, ny, nz};
     for (int d = 0; d < 3; ++d) {
+        plan_fused(h, d, nr[d], ns[d]);
+        if (h->fused[d].on && h->fused[d].lds > 64 * 1024) {
+            hipError_t ea = allow_lds(h->fused[d], d);
+            if (ea != hipSuccess) h->fused[d].on = 0;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
         ftd_status st = ftd_create(&h->solver[d], kind, ns[d], nr[d], h->parts[d], dt, h->stream);
         if (st != FTD_OK) {
             std::string m = ftd_last_error_string();
@@ -442,6 +812,23 @@ ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph) {
         float ms = 0.f;
         HH(hipEventElapsedTime(&ms, h->e0, h->e1));
         h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+// fused sweeps on (default) or off; a captured step is dropped so that the
+// next graph step captures the new sequence
+ftd_status ftd_heat_set_fused(ftd_heat* h, int enable) {
+    if (!h) return herr(FTD_ERR_INVALID, "null handle");
+    h->use_fused = enable != 0;
+    if (h->exec) {
+        HH(hipStreamSynchronize(h->stream));
+        hipGraphExecDestroy(h->exec);
+        h->exec = nullptr;
+    }
+    if (h->graph) {
+        hipGraphDestroy(h->graph);
+        h->graph = nullptr;
     }
     return FTD_OK;
 }

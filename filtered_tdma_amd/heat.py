@@ -47,6 +47,10 @@ class SolveTheta:
     def run(self, nsteps: int, use_graph: bool = False) -> None:
         _lib.check_heat(_lib.load().ftd_heat_step(self._h, int(nsteps), int(bool(use_graph))))
 
+    def set_fused(self, enable: bool) -> None:
+        """Fused sweeps (coefficients generated in the kernel) on or off."""
+        _lib.check_heat(_lib.load().ftd_heat_set_fused(self._h, int(bool(enable))))
+
     def reset(self) -> None:
         _lib.check_heat(_lib.load().ftd_heat_init(self._h))
 

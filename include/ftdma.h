@@ -201,6 +201,13 @@ ftd_status ftd_heat_init(ftd_heat* h);
  * step captured into a hipGraph. Waits for the last step. */
 ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph);
 
+/* Extension: fused sweeps on (default) or off. Where a sweep's coefficients
+ * are the stencil's (PaScaL kind, Dirichlet walls), the fused sweep generates
+ * them in the kernel and reads and writes the field in place (16 B per row
+ * instead of build + solve + scatter); the field is bit-identical either
+ * way. A captured step is dropped and re-captured by the next graph step. */
+ftd_status ftd_heat_set_fused(ftd_heat* h, int enable);
+
 /* the field with ghosts, (nz+2)(ny+2)(nx+2) doubles, x fastest, to host memory */
 ftd_status ftd_heat_field(ftd_heat* h, double* host_out);
 

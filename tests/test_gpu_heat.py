@@ -110,3 +110,29 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
     inner = (slice(1, -1),) * 3
     for key, (l2, f) in res.items():
         assert np.max(np.abs(f[inner] - thomas[inner])) <= 1e-6, key
+
+
+@pytest.mark.parametrize("n,parts,periodic", [
+    (48, (1, 1, 1), (0, 0, 0)),     # Thomas sweeps (N <= 64)
+    (100, (1, 1, 1), (0, 0, 0)),    # internal PaScaL partitions (N > 64)
+    (64, (2, 4, 8), (0, 0, 0)),     # PaScaL with the caller's partitions
+    (40, (1, 2, 1), (1, 0, 0)),     # periodic x: materialized there, fused in y, z
+    (37, (1, 1, 3), (0, 0, 0)),     # odd sizes, uneven partitions
+])
+def test_heat_fused_sweeps_equal_materialized(n, parts, periodic):
+    """Fused sweeps (coefficients generated in the kernel, rhs and x in the
+    field) give the materialized build + solve + scatter field bit for bit,
+    eagerly and from a graph."""
+    dt = dt_policy("order", 0.3, n)[0]
+    note(f"heat fused vs materialized n={n} parts={parts} periodic={periodic}")
+    sims = [SolveTheta(n, n, n, parts=parts, backend="pascal", dt=dt, periodic=periodic)
+            for _ in range(2)]
+    try:
+        sims[1].set_fused(False)
+        for use_graph in (False, True):
+            for sim in sims:
+                sim.run(3, use_graph=use_graph)
+            np.testing.assert_array_equal(sims[0].field(), sims[1].field())
+    finally:
+        for sim in sims:
+            sim.close()
