@@ -299,10 +299,31 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
     }
     group_lds_sync();
     if (active && q == 0) {
-        // reduced system of this system: rows 2*lead .. 2*(lead+Q)-1
+        // reduced system of this system: rows 2*lead .. 2*(lead+Q)-1 (up to
+        // 256), taken 8 rows at a time: the rows' values are read into
+        // registers first, so the LDS latency is paid once per 8 rows and
+        // the serial chain is the arithmetic only
         const int r0 = 2 * lead, NR = 2 * Q;
         double cp = rC[r0], dp = rD[r0];
-        for (int k = 1; k < NR; ++k) {
+        int k = 1;
+        for (; k + 8 <= NR; k += 8) {
+            double ak[8], ck[8], dk[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ak[u] = rA[r0 + k + u];
+                ck[u] = rC[r0 + k + u];
+                dk[u] = rD[r0 + k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double inv = 1.0 / (1.0 - ak[u] * cp);
+                dp = inv * (dk[u] - ak[u] * dp);
+                cp = inv * ck[u];
+                rD[r0 + k + u] = dp;
+                rC[r0 + k + u] = cp;
+            }
+        }
+        for (; k < NR; ++k) {
             const double ak = rA[r0 + k];
             const double inv = 1.0 / (1.0 - ak * cp);
             dp = inv * (rD[r0 + k] - ak * dp);
@@ -311,7 +332,21 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
             rC[r0 + k] = cp;
         }
         double dn = dp;
-        for (int k = NR - 2; k >= 0; --k) {
+        k = NR - 2;
+        for (; k >= 7; k -= 8) {
+            double ck[8], dk[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ck[u] = rC[r0 + k - u];
+                dk[u] = rD[r0 + k - u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                dn = dk[u] - ck[u] * dn;
+                rD[r0 + k - u] = dn;
+            }
+        }
+        for (; k >= 0; --k) {
             dn = rD[r0 + k] - rC[r0 + k] * dn;
             rD[r0 + k] = dn;
         }
