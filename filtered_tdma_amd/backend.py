@@ -119,6 +119,23 @@ class TdmaBackendGPU:
             z = ctypes.c_void_p(Z.data_ptr())
         _lib.check(_lib.load().ftd_solve_complex(self._h, a, b, c, z))
 
+    def set_j_mode(self, mode: int) -> None:
+        """0: the reference's cal_J per partition; 1: learned J per system
+        (FILTERED kind), checked against eps_."""
+        _lib.check(_lib.load().ftd_set_j_mode(self._h, int(mode)))
+
+    def last_J_systems(self, complex_rhs: bool = False):
+        """(J, fallback) arrays of shape [P, n_sys] (n_sys doubled for a
+        complex solve) from the last learned-mode solve."""
+        import numpy as np
+
+        n = self.partitions * self.n_sys * (2 if complex_rhs else 1)
+        out = np.empty(n, dtype=np.int32)
+        _lib.check(_lib.load().ftd_last_J_systems(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
+        out = out.reshape(self.partitions, -1)
+        return out & 0xFFFF, (out >> 16) & 1
+
     def last_J(self, n=None) -> list:
         n = self.partitions if n is None else int(n)
         out = (ctypes.c_int * n)()

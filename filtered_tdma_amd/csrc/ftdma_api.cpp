@@ -76,6 +76,9 @@ struct ftd_handle {
     int t_M = 0, t_T = 0, t_S = 0, t_Q = 0;  // forced tuning (0 = automatic)
     int force_general = 0;                   // use the general kernels only
     int no_stream = 0;                       // automatic choice without the streamed kernels
+    int j_mode = 0;                          // 0: cal_J (reference), 1: learned J per system
+    DevBuf jsys;                             // learned mode: J (| fallback << 16) per lane
+    size_t jsys_count = 0;
     double* rho = nullptr;                   // A_rho[n_row+1] then C_rho[n_row+1]
     int* J = nullptr;                        // J per partition
     DevBuf partial, xbuf, R, E;
@@ -274,8 +277,10 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
 
     int* Jp = h->J;
     const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
-    const int Ms =
-        (h->force_general || h->t_M || h->no_stream || csh) ? 0 : ftd::stream_rows(mmax, P);
+    const int Ms = (h->force_general || h->t_M || h->no_stream || csh ||
+                    (h->j_mode && version == 1))
+                       ? 0
+                       : ftd::stream_rows(mmax, P);
     if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
         const int R = ftd::stream_ring_depth(Ms, P);
         if (version == 2) {
@@ -302,6 +307,20 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         return FTD_OK;
     }
     if (fs.M && csh && fs.S < 2) fs.M = 0;  // the two parts of a system share a tile
+    if (h->j_mode == 1 && version == 1) {
+        // learned J: one pass (v1 sweeps, interface, J per system from the
+        // MLP, checked against eps_, truncated correction)
+        if (!fs.M) return fail(FTD_ERR_UNSUPPORTED, "learned J needs a fused kernel shape");
+        const size_t nj = (size_t)P * n_sys;
+        FTD_TRY(ensure(h->jsys, sizeof(int) * nj));
+        h->jsys_count = nj;
+        std::snprintf(h->path, sizeof h->path, "filtered_learned_fused M=%d T=%d S=%d", fs.M,
+                      fs.T, fs.S);
+        FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 3, A, B, C, D, n_sys, n_row, P, fs.S,
+                                           cyclic, csh, Jp, nullptr, nullptr, h->eps_value,
+                                           h->eps_direct, (int*)h->jsys.p, st));
+        return FTD_OK;
+    }
     if (fs.M) {
         const int tiles = (n_sys + fs.S - 1) / fs.S;
         if (version == 2) {
@@ -313,7 +332,7 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
             std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d", fs.M, fs.T,
                           fs.S);
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, csh, Jp, nullptr, nullptr, st));
+                                               cyclic, csh, Jp, nullptr, nullptr, 0.0, 0, nullptr, st));
         } else {
             FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h->xbuf, xbytes));
@@ -321,11 +340,12 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
                           fs.S);
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
                                                cyclic, csh, Jp, (double*)h->xbuf.p,
-                                               (double*)h->partial.p, st));
+                                               (double*)h->partial.p, 0.0, 0, nullptr, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
                                           h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, st));
+                                               cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, 0.0, 0,
+                                               nullptr, st));
         }
         return FTD_OK;
     }
@@ -434,7 +454,7 @@ ftd_status ftd_destroy(ftd_handle* h) {
     if (h->rho) hipFree(h->rho);
     if (h->J) hipFree(h->J);
     for (DevBuf* b : {&h->partial, &h->xbuf, &h->R, &h->E, &h->b_dlanes, &h->b_dlong_off,
-                      &h->b_dlong_n})
+                      &h->b_dlong_n, &h->jsys})
         if (b->p) hipFree(b->p);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
@@ -649,6 +669,24 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
         FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->last_ms = ms;
     }
+    return FTD_OK;
+}
+
+ftd_status ftd_set_j_mode(ftd_handle* h, int mode) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (mode != 0 && mode != 1) return fail(FTD_ERR_INVALID, "J mode must be 0 or 1");
+    h->j_mode = mode;
+    return FTD_OK;
+}
+
+ftd_status ftd_last_J_systems(ftd_handle* h, int* out, int64_t n) {
+    if (!h || !out) return fail(FTD_ERR_INVALID, "null argument");
+    if (n < 0 || (size_t)n > h->jsys_count)
+        return fail(FTD_ERR_INVALID, "n exceeds the J values of the last learned solve");
+    if (n == 0) return FTD_OK;
+    FTD_HIP(hipMemcpyAsync(out, h->jsys.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost,
+                           h->stream));
+    FTD_HIP(hipStreamSynchronize(h->stream));
     return FTD_OK;
 }
 

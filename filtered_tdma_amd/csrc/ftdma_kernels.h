@@ -83,7 +83,8 @@ size_t filtered_fused_lds(int P, int S);
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
                                  int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
-                                 double* partial, hipStream_t st);
+                                 double* partial, double eps_value, int eps_direct, int* Jsys,
+                                 hipStream_t st);
 
 hipError_t launch_gen_thomas(const double* A, const double* B, double* C, double* D, int n_sys,
                              int n_row, hipStream_t st);

@@ -32,6 +32,7 @@
 #include <utility>
 
 #include "ftdma_kernels.h"
+#include "ftdma_mlpj_weights.h"
 
 namespace ftd {
 
@@ -757,6 +758,59 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
 //   PASS 2: v1 second pass -> recompute the sweeps, take x0, xN from xbuf and
 //           J_p, store x.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Learned J (PASS 3 of the filtered kernel): one J per (system, partition)
+// from a 5 -> 32 -> 32 -> 1 MLP (weights from learned_j/train.py), with the
+// first dropped correction term checked against eps_ afterwards.
+// ---------------------------------------------------------------------------
+
+// piecewise-linear log2 of |x| from its float64 bit pattern: (e - 1) +
+// (2 f - 1) for |x| = f 2^e, f in [0.5, 1); -1100 for 0 (exact, so every
+// machine computes the same feature)
+__device__ inline float lg_feature(double x) {
+    x = fabs(x);
+    if (x == 0.0) return -1100.0f;
+    int e;
+    const double f = frexp(x, &e);
+    return (float)((double)(e - 1) + (2.0 * f - 1.0));
+}
+
+// the predicted J (before clamping); the operation order is fixed (the CPU
+// restatement follows it), products and sums rounded separately
+__device__ inline int mlp_J(float f0, float f1, float f2, float f3, float f4) {
+    // second layer in two halves of 16 outputs (the first layer is
+    // recomputed per half), so 16 accumulators are live instead of 32;
+    // every sum keeps its order: acc[l] over k = 0..31, out over l = 0..31
+    float out = kMlpB3[0];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        float acc[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc[l] = kMlpB2[half * 16 + l];
+        for (int k = 0; k < 32; ++k) {
+            float h = kMlpB1[k];
+            h = h + kMlpW1[0 * 32 + k] * f0;
+            h = h + kMlpW1[1 * 32 + k] * f1;
+            h = h + kMlpW1[2 * 32 + k] * f2;
+            h = h + kMlpW1[3 * 32 + k] * f3;
+            h = h + kMlpW1[4 * 32 + k] * f4;
+            h = h > 0.0f ? h : 0.0f;
+#pragma unroll
+            for (int l = 0; l < 16; ++l) acc[l] = acc[l] + kMlpW2[k * 32 + half * 16 + l] * h;
+        }
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            const float v = acc[l] > 0.0f ? acc[l] : 0.0f;
+            out = out + kMlpW3[half * 16 + l] * v;
+        }
+    }
+    const float cj = ceilf(out);
+    // saturate before converting (the prediction of a wild input is still
+    // clamped to [0, m-1] by the caller)
+    const float lim = 1.0e6f;
+    return (int)(cj > lim ? lim : (cj < -lim ? -lim : cj)) + kMlpMargin;
+}
+
 template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE, int CSH>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_fused(const double* __restrict__ A,
                                                          const double* __restrict__ B,
@@ -765,7 +819,9 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
                                                          int n_row, int P, int lgS, int cyclic,
                                                          const int* __restrict__ Jp,
                                                          double* __restrict__ xbuf,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial,
+                                                         double eps_value, int eps_direct,
+                                                         int* __restrict__ Jsys) {
     extern __shared__ double lds[];
     const int S = 1 << lgS;
     const int t = threadIdx.x;
@@ -792,6 +848,9 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     double b1 = 1.0;
     const int cnt = m;
     const double* Din = D;
+    // PASS 3 features: max |a/b|, |c/b| over rows 1..m-1, raw d of rows 0
+    // and m-1, max |d|
+    double f_rho = 0.0, f_d0 = 0.0, f_dN = 0.0, f_dmax = 0.0;
 
 #pragma unroll
     for (int j = 0; j < PF && j < M; ++j) FTD_LOAD_ROW(j);
@@ -802,6 +861,12 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         if (j + PF < M) FTD_LOAD_ROW(j + PF);
         if (j < m) {
             const double aj = ra[j], bj = rb[j], cj = rc[j], dj = rd[j];
+            if (PASS == 3) {
+                if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj / bj), fabs(cj / bj)));
+                if (j == 0) f_d0 = dj;
+                if (j == m - 1) f_dN = dj;
+                f_dmax = fmax(f_dmax, fabs(dj));
+            }
             if (j < 2) {
                 const double r = 1.0 / bj;
                 a[j] = aj * r;
@@ -905,9 +970,32 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         return;
     }
 
+    int Jf = J;
+    if (PASS == 3) {
+        // learned J of this (system, partition), then the first dropped
+        // term of each side against eps_; a larger one falls back to the
+        // full correction (J = m-1)
+        const double Nl = (double)(m * P);
+        const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        int Jl = mlp_J((float)fmin(f_rho, 1.0), lg_feature(f_d0), lg_feature(f_dN),
+                       lg_feature(f_dmax), lg_feature(eps));
+        Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
+        const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;                   // first a-term dropped
+        const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;       // first c-term dropped
+        double ta = 0.0, tc = 0.0;
+#pragma unroll
+        for (int j = 1; j < M; ++j) {
+            if (j == kL && j <= m - 2) ta = fabs(a[j] * x0);
+            if (j == kR && j >= 1) tc = fabs(c[j] * xN);
+        }
+        const bool fallback = ta > eps || tc > eps;
+        Jf = fallback ? m - 1 : Jl;
+        if (Jsys && live) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+    }
+
     // k_final_pass: truncated correction
-    const int jL = J < m - 2 ? J : m - 2;
-    const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+    const int jL = Jf < m - 2 ? Jf : m - 2;
+    const int jR = (m - 1) - Jf > 1 ? (m - 1) - Jf : 1;
     if (live) {
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -1618,7 +1706,8 @@ size_t filtered_fused_lds(int P, int S) { return sizeof(double) * 4 * (size_t)P 
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
                                  int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
-                                 double* partial, hipStream_t st) {
+                                 double* partial, double eps_value, int eps_direct, int* Jsys,
+                                 hipStream_t st) {
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * P);
     const size_t lds = (pass == 2) ? 0 : filtered_fused_lds(P, S);
@@ -1626,10 +1715,12 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
     do {                                                                                     \
         if (csh)                                                                             \
             hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 1>), grid, block, lds, \
-                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial); \
+                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial,  \
+                               eps_value, eps_direct, Jsys);                                 \
         else                                                                                 \
             hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 0>), grid, block, lds, \
-                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial); \
+                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial,  \
+                               eps_value, eps_direct, Jsys);                                 \
     } while (0)
 #define X(m_, t_, pf_, w_)                                \
     if (M == m_ && maxt == t_) {                          \
@@ -1637,6 +1728,8 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
             LAUNCH(m_, t_, 2, 0, pf_, w_);                \
         else if (pass == 1)                               \
             LAUNCH(m_, t_, 1, 1, pf_, w_);                \
+        else if (pass == 3)                               \
+            LAUNCH(m_, t_, 1, 3, (m_ == 32 ? 4 : pf_), w_); \
         else                                              \
             LAUNCH(m_, t_, 1, 2, pf_, w_);                \
         return hipGetLastError();                         \

@@ -141,6 +141,22 @@ double ftd_last_comm_ms(const ftd_handle* h);
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
                           int systems_per_tile, int lanes_per_system);
 
+/* Extension (learned truncation depth; no counterpart in the reference):
+ * mode 1 makes the FILTERED kind (v1 sweeps) choose J per (system,
+ * partition) with a small MLP (filtered_tdma_amd/learned_j/train.py) from
+ * the partition's coupling bound, right-hand-side magnitudes and eps_,
+ * instead of one cal_J per partition; the first correction term it drops is
+ * checked against eps_ and the full correction (J = m-1) is used where it
+ * exceeds it. One pass instead of v1's two (no global J). Mode 0 (default)
+ * is the reference's cal_J. Register-resident kernels only. */
+ftd_status ftd_set_j_mode(ftd_handle* h, int mode);
+
+/* Extension: the J used per (partition, system) by the last learned-mode
+ * solve, index p * n_sys + i (n_sys doubled for complex solves), with bit 16
+ * set where the eps_ check fell back to the full correction. Blocking copy
+ * of n values. */
+ftd_status ftd_last_J_systems(ftd_handle* h, int* out, int64_t n);
+
 /* Extension (complex right-hand sides; the reference's channel pressure
  * solve, apps/channel_cpu/PressureSolver.cpp:412-470, solves the real and
  * imaginary parts as two real systems with copies of one coefficient set):

@@ -574,6 +574,59 @@ static void ffinal_cols(const double* A, const double* C, double* D, size_t ld,
     }
 }
 
+/* Sweeps of every partition, the exchange rows and the interface 2x2
+ * solve (rows 0 and m-1 of each partition hold x_0, x_{m-1} afterwards).
+ * Jp: the P values of J (v2 only). */
+static void sweeps_iface(double* A, const double* B, double* C, double* D, size_t ld,
+                         int n_sys, int P, const int* sv, const int* mv, int version,
+                         const int* Jp, int sem, int cyclic, double* xbuf) {
+    const int nc = nchunks(n_sys);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            int Jfull = (version == 1) ? m : Jp[p];
+            ffwd_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, Jfull,
+                      version == 2 && sem == FTDO_SEM_GPU);
+            fbwd_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, version,
+                      version == 2 ? Jp[p] : 0, sem);
+            double* sr = xbuf + (size_t)(4 * p) * ld;
+            const double* CN = ROW(C, s + m - 1);
+            const double* DN = ROW(D, s + m - 1);
+            const double* A0 = ROW(A, s);
+            const double* D0 = ROW(D, s);
+            for (int i = c0; i < c1; ++i) {
+                sr[0 * ld + i] = CN[i]; /* send right: C */
+                sr[1 * ld + i] = DN[i]; /* send right: D */
+                sr[2 * ld + i] = A0[i]; /* send left : A */
+                sr[3 * ld + i] = D0[i]; /* send left : D */
+            }
+        }
+        /* interface 2x2 blocks: filtered_tdma_kernels.hpp:197-218, .cuh:284-303 */
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            int has_left = cyclic || p > 0;
+            int has_right = cyclic || p < P - 1;
+            int pl = (p + P - 1) % P, pr = (p + 1) % P;
+            const double* C_left = xbuf + (size_t)(4 * pl + 0) * ld;
+            const double* D_left = xbuf + (size_t)(4 * pl + 1) * ld;
+            const double* A_right = xbuf + (size_t)(4 * pr + 2) * ld;
+            const double* D0_right = xbuf + (size_t)(4 * pr + 3) * ld;
+            double* A0 = ROW(A, s);
+            double* D0 = ROW(D, s);
+            const double* CN = ROW(C, s + m - 1);
+            double* DN = ROW(D, s + m - 1);
+            if (has_left)
+                for (int i = c0; i < c1; ++i)
+                    D0[i] = (D0[i] - A0[i] * D_left[i]) / (1.0 - C_left[i] * A0[i]);
+            if (has_right)
+                for (int i = c0; i < c1; ++i)
+                    DN[i] = (DN[i] - CN[i] * D0_right[i]) / (1.0 - A_right[i] * CN[i]);
+        }
+    }
+}
+
 int ftdo_filtered(double* A, const double* B, double* C, double* D,
                   int n_sys, int n_row, int P, int version, int cyclic, int sem,
                   const double* A_rho, const double* C_rho,
@@ -651,50 +704,7 @@ int ftdo_filtered(double* A, const double* B, double* C, double* D,
         }
     }
 
-#pragma omp parallel for schedule(static)
-    for (int k = 0; k < nc; ++k) {
-        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
-        for (int p = 0; p < P; ++p) {
-            const int s = sv[p], m = mv[p];
-            int Jfull = (version == 1) ? m : Jp[p];
-            ffwd_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, Jfull,
-                      version == 2 && sem == FTDO_SEM_GPU);
-            fbwd_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, version,
-                      version == 2 ? Jp[p] : 0, sem);
-            double* sr = xbuf + (size_t)(4 * p) * ld;
-            const double* CN = ROW(C, s + m - 1);
-            const double* DN = ROW(D, s + m - 1);
-            const double* A0 = ROW(A, s);
-            const double* D0 = ROW(D, s);
-            for (int i = c0; i < c1; ++i) {
-                sr[0 * ld + i] = CN[i]; /* send right: C */
-                sr[1 * ld + i] = DN[i]; /* send right: D */
-                sr[2 * ld + i] = A0[i]; /* send left : A */
-                sr[3 * ld + i] = D0[i]; /* send left : D */
-            }
-        }
-        /* interface 2x2 blocks: filtered_tdma_kernels.hpp:197-218, .cuh:284-303 */
-        for (int p = 0; p < P; ++p) {
-            const int s = sv[p], m = mv[p];
-            int has_left = cyclic || p > 0;
-            int has_right = cyclic || p < P - 1;
-            int pl = (p + P - 1) % P, pr = (p + 1) % P;
-            const double* C_left = xbuf + (size_t)(4 * pl + 0) * ld;
-            const double* D_left = xbuf + (size_t)(4 * pl + 1) * ld;
-            const double* A_right = xbuf + (size_t)(4 * pr + 2) * ld;
-            const double* D0_right = xbuf + (size_t)(4 * pr + 3) * ld;
-            double* A0 = ROW(A, s);
-            double* D0 = ROW(D, s);
-            const double* CN = ROW(C, s + m - 1);
-            double* DN = ROW(D, s + m - 1);
-            if (has_left)
-                for (int i = c0; i < c1; ++i)
-                    D0[i] = (D0[i] - A0[i] * D_left[i]) / (1.0 - C_left[i] * A0[i]);
-            if (has_right)
-                for (int i = c0; i < c1; ++i)
-                    DN[i] = (DN[i] - CN[i] * D0_right[i]) / (1.0 - A_right[i] * CN[i]);
-        }
-    }
+    sweeps_iface(A, B, C, D, ld, n_sys, P, sv, mv, version, Jp, sem, cyclic, xbuf);
 
     /* v1: J after the interface solve */
     if (version == 1) {
@@ -743,5 +753,140 @@ int ftdo_filtered(double* A, const double* B, double* C, double* D,
     free(xbuf);
     free(Jp);
     free(maxb);
+    return 0;
+}
+
+/* ------------------------- learned-J filtered mode ------------------------
+ * No counterpart in the reference: the FILTERED solve (v1 sweeps and
+ * interface, filtered_tdma_kernels.cuh:69-118,284-303) with J chosen per
+ * (system, partition) by a 5 -> 32 -> 32 -> 1 ReLU network in FP32 instead
+ * of one cal_J per partition, the first dropped correction term of each side
+ * checked against eps_ (J = m-1 where it exceeds it), then the truncated
+ * correction of k_final_pass (.cuh:196-218) with that J. The network's
+ * weights come in as one flat array (W1[5][32], b1[32], W2[32][32], b2[32],
+ * W3[32], b3) and every sum runs in the order written here. */
+
+/* lg|x| = e + f for |x| = (1 + f) 2^e, read off frexp; lg 0 = -1100. */
+static float lg_feature(double x) {
+    x = fabs(x);
+    if (x == 0.0) return -1100.0f;
+    int e;
+    const double f = frexp(x, &e);
+    return (float)((double)(e - 1) + (2.0 * f - 1.0));
+}
+
+static int mlp_J(const float* w, int margin, const float f[5]) {
+    const float* W1 = w;
+    const float* b1 = W1 + 160;
+    const float* W2 = b1 + 32;
+    const float* b2 = W2 + 1024;
+    const float* W3 = b2 + 32;
+    const float* b3 = W3 + 32;
+    float hid[32], acc[32];
+    for (int k = 0; k < 32; ++k) {
+        float h = b1[k];
+        for (int q = 0; q < 5; ++q) h = h + W1[q * 32 + k] * f[q];
+        hid[k] = h > 0.0f ? h : 0.0f;
+    }
+    for (int l = 0; l < 32; ++l) {
+        float a = b2[l];
+        for (int k = 0; k < 32; ++k) a = a + W2[k * 32 + l] * hid[k];
+        acc[l] = a;
+    }
+    float out = b3[0];
+    for (int l = 0; l < 32; ++l) {
+        const float v = acc[l] > 0.0f ? acc[l] : 0.0f;
+        out = out + W3[l] * v;
+    }
+    const float cj = ceilf(out);
+    const float lim = 1.0e6f;
+    return (int)(cj > lim ? lim : (cj < -lim ? -lim : cj)) + margin;
+}
+
+int ftdo_mlp_J(const float* weights, int margin, const float* features) {
+    return mlp_J(weights, margin, features);
+}
+
+float ftdo_lg(double x) { return lg_feature(x); }
+
+int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
+                          int n_sys, int n_row, int P, int cyclic,
+                          double eps_value, int eps_direct,
+                          const float* weights, int margin, int* J_out, int nthreads) {
+    int rc = check_parts(n_row, P);
+    if (rc) return rc;
+    if (!weights) return fail(-1, "filtered_learned: weights needed");
+    if (P == 1) {
+        if (J_out)
+            for (int i = 0; i < n_sys; ++i) J_out[i] = -1;
+        return cyclic ? ftdo_thomas_cyclic(A, B, C, D, n_sys, n_row, FTDO_SEM_GPU, nthreads)
+                      : ftdo_thomas(A, B, C, D, n_sys, n_row, FTDO_SEM_GPU, nthreads);
+    }
+    if (P > 4096) return fail(-1, "filtered_learned: at most 4096 partitions");
+    const size_t ld = (size_t)n_sys;
+    int sv[4096], mv[4096];
+    for (int p = 0; p < P; ++p) part_rows(n_row, P, p, &sv[p], &mv[p]);
+    double* xbuf = (double*)malloc(sizeof(double) * ld * (size_t)(4 * P) + 8);
+    float* feat = (float*)malloc(sizeof(float) * ld * (size_t)P * 4 + 8);
+    if (!xbuf || !feat) {
+        free(xbuf); free(feat);
+        return fail(-3, "filtered_learned: out of memory");
+    }
+    set_threads(nthreads);
+    /* features of the raw rows: max over rows 1..m-1 of |a/b|, |c/b|
+     * (clamped to 1), lg|d_0|, lg|d_m-1|, lg max|d| */
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < P; ++p) {
+        const int s = sv[p], m = mv[p];
+        for (size_t i = 0; i < ld; ++i) {
+            double rho = 0.0, dmax = 0.0;
+            for (int j = 0; j < m; ++j) {
+                const double aj = ROW(A, s + j)[i], bj = ROW(B, s + j)[i];
+                const double cj = ROW(C, s + j)[i], dj = ROW(D, s + j)[i];
+                if (j >= 1) rho = fmax(rho, fmax(fabs(aj / bj), fabs(cj / bj)));
+                dmax = fmax(dmax, fabs(dj));
+            }
+            float* f = feat + ((size_t)p * ld + i) * 4;
+            f[0] = (float)fmin(rho, 1.0);
+            f[1] = lg_feature(ROW(D, s)[i]);
+            f[2] = lg_feature(ROW(D, s + m - 1)[i]);
+            f[3] = lg_feature(dmax);
+        }
+    }
+    sweeps_iface(A, B, C, D, ld, n_sys, P, sv, mv, 1, NULL, FTDO_SEM_GPU, cyclic, xbuf);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < P; ++p) {
+        const int s = sv[p], m = mv[p];
+        const double Nl = (double)(m * P);
+        const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        const float leps = lg_feature(eps);
+        const double* D0 = ROW(D, s);
+        const double* DN = ROW(D, s + m - 1);
+        for (size_t i = 0; i < ld; ++i) {
+            const float* f = feat + ((size_t)p * ld + i) * 4;
+            const float fv[5] = {f[0], f[1], f[2], f[3], leps};
+            int Jl = mlp_J(weights, margin, fv);
+            Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
+            const double x0 = D0[i], xN = DN[i];
+            /* first a-term and first c-term the truncation drops */
+            const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;
+            const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;
+            const double ta = kL <= m - 2 ? fabs(ROW(A, s + kL)[i] * x0) : 0.0;
+            const double tc = kR >= 1 ? fabs(ROW(C, s + kR)[i] * xN) : 0.0;
+            const int fallback = ta > eps || tc > eps;
+            const int J = fallback ? m - 1 : Jl;
+            if (J_out) J_out[(size_t)p * ld + i] = J | (fallback << 16);
+            const int left_end = J < m - 2 ? J : m - 2;
+            const int right_beg = (m - 1) - J > 1 ? (m - 1) - J : 1;
+            for (int j = 1; j < m - 1; ++j) {
+                double x = ROW(D, s + j)[i];
+                if (j <= left_end) x = x - ROW(A, s + j)[i] * x0;
+                if (j >= right_beg) x = x - ROW(C, s + j)[i] * xN;
+                ROW(D, s + j)[i] = x;
+            }
+        }
+    }
+    free(xbuf);
+    free(feat);
     return 0;
 }
