@@ -228,16 +228,35 @@ def c5_channel(steps, warmup, torch):
 
     ms_c = timed(lambda k: be.solve_complex(tA, tB, tC, zc[k]), [(zc, tZ0)])
     path = be.last_path()
+    x_exact = zc[0].cpu().numpy()  # one partition: Thomas
     ms_2 = timed(lambda k: (be.solve(tA, tB, tC, rc[k]), be.solve(tA, tB, tC, ic[k])),
                  [(rc, tR0), (ic, tI0)])
     be.close()
     rows = n_row * n_sys
+
+    # learned J per system against the reference's cal_J (rho from column 0,
+    # the pinned mode, so cal_J keeps the full correction everywhere)
+    P_f, eps_c = 8, 1e-6
+    learned = {"partitions": P_f, "eps_constant": eps_c}
+    for mode, name in ((0, "cal_J"), (1, "learned")):
+        bf = TdmaBackendGPU("filtered", n_sys, n_row, P_f, eps_c)
+        bf.set_rho_device(tA, tB, tC)
+        bf.set_j_mode(mode)
+        ms = timed(lambda k: bf.solve_complex(tA, tB, tC, zc[k]), [(zc, tZ0)])
+        err = float(np.max(np.abs(zc[0].cpu().numpy() - x_exact)))
+        if mode:
+            J, fb = bf.last_J_systems(True)
+            stats = {"mean_J": float(J.mean()), "fallback_fraction": float(fb.mean())}
+        else:
+            stats = {"mean_J": float(np.mean(bf.last_J(2 * P_f)))}
+        learned[name] = {"ms": ms, "max_abs_err_vs_thomas": err, "path": bf.last_path(), **stats}
+        bf.close()
     return {"workload": "channel pressure Poisson, 128x128 modes, n_row=129, tanh grid "
                         "gamma=2, pinned zero mode, complex128 RHS",
             "complex_rows": rows, "complex_ms": ms_c, "two_real_ms": ms_2,
             "complex_rows_per_s": rows / (ms_c * 1e-3),
             "hbm_frac_algorithmic": 56 * rows / (ms_c * 1e-3) / (HBM_PEAK_GBS * 1e9),
-            "path": path}
+            "path": path, "filtered_J": learned}
 
 
 def cpu_baseline(A, B, C, D):
