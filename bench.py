@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kinds", default="pascal,filtered_v2,filtered")
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 heat (C1) leg")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the uneven-batch and 256^3 ADI-step (C4) legs")
     ap.add_argument("--no-c5", action="store_true",
@@ -167,6 +168,38 @@ def c4_uneven(steps, warmup, torch):
         "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "path": path,
     }
+
+
+def c1_heat():
+    """The heat case of the reference's order runs at 64^3: dt = 1e-3 (also
+    eps_constant), 100 steps, Dirichlet, manufactured initial field; the
+    z-direction split into 4 partitions (the reference's 4 z-slab ranks),
+    x and y whole. Each backend runs the 100 steps from a hipGraph; L2 of
+    main.cpp:59-81 and the largest difference from the single-partition
+    (Thomas) field."""
+    from filtered_tdma_amd.heat import SolveTheta
+
+    n, dt, nsteps = 64, 1e-3, 100
+    out, fields = {}, {}
+    for name, kind, parts in (("thomas", "pascal", (1, 1, 1)), ("pascal", "pascal", (1, 1, 4)),
+                              ("filtered", "filtered", (1, 1, 4)),
+                              ("filtered_v2", "filtered_v2", (1, 1, 4))):
+        sim = SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt)
+        try:
+            sim.run(nsteps, use_graph=True)  # capture + replays, compiled and warm
+            sim.reset()
+            t0 = time.perf_counter()
+            sim.run(nsteps, use_graph=True)
+            wall = time.perf_counter() - t0
+            out[name] = {"l2": sim.l2(), "ms_per_step": wall * 1e3 / nsteps,
+                         "parts_xyz": list(parts)}
+            fields[name] = sim.field()[1:-1, 1:-1, 1:-1]
+        finally:
+            sim.close()
+    for name in out:
+        out[name]["max_diff_vs_thomas"] = float(np.max(np.abs(fields[name] - fields["thomas"])))
+    return {"workload": "heat ADI 64^3, dt=eps_constant=1e-3, 100 steps, Dirichlet, "
+                        "z in 4 partitions, hipGraph", "backends": out}
 
 
 def c4_adi(steps):
@@ -339,6 +372,13 @@ def main():
                   file=sys.stderr, flush=True)
         del tA32, tB32, tC32, tD32, tD3
 
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_heat()
+        print("[bench] c1 heat 64^3: " + ", ".join(
+            f"{k} L2 {v['l2']:.6e} {v['ms_per_step']:.4f} ms/step"
+            for k, v in c1["backends"].items()), file=sys.stderr, flush=True)
+
     c4 = None
     if not args.no_c4:
         del copies
@@ -389,6 +429,8 @@ def main():
         },
         "backends": results,
     }
+    if c1 is not None:
+        line["c1"] = c1
     if c4 is not None:
         line["c4"] = c4
     if c5 is not None:

@@ -778,32 +778,26 @@ __device__ inline float lg_feature(double x) {
 // the predicted J (before clamping); the operation order is fixed (the CPU
 // restatement follows it), products and sums rounded separately
 __device__ inline int mlp_J(float f0, float f1, float f2, float f3, float f4) {
-    // second layer in two halves of 16 outputs (the first layer is
-    // recomputed per half), so 16 accumulators are live instead of 32;
-    // every sum keeps its order: acc[l] over k = 0..31, out over l = 0..31
+    // 5 -> H -> H -> 1, ReLU, FP32; every multiply-add is one explicit fma
+    // (correctly rounded), and every sum runs in index order
+    constexpr int H = kMlpH;
+    float acc[H];
+#pragma unroll
+    for (int l = 0; l < H; ++l) acc[l] = kMlpB2[l];
+    for (int k = 0; k < H; ++k) {
+        float h = kMlpB1[k];
+        h = fmaf(kMlpW1[0 * H + k], f0, h);
+        h = fmaf(kMlpW1[1 * H + k], f1, h);
+        h = fmaf(kMlpW1[2 * H + k], f2, h);
+        h = fmaf(kMlpW1[3 * H + k], f3, h);
+        h = fmaf(kMlpW1[4 * H + k], f4, h);
+        h = h > 0.0f ? h : 0.0f;
+#pragma unroll
+        for (int l = 0; l < H; ++l) acc[l] = fmaf(kMlpW2[k * H + l], h, acc[l]);
+    }
     float out = kMlpB3[0];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        float acc[16];
-#pragma unroll
-        for (int l = 0; l < 16; ++l) acc[l] = kMlpB2[half * 16 + l];
-        for (int k = 0; k < 32; ++k) {
-            float h = kMlpB1[k];
-            h = h + kMlpW1[0 * 32 + k] * f0;
-            h = h + kMlpW1[1 * 32 + k] * f1;
-            h = h + kMlpW1[2 * 32 + k] * f2;
-            h = h + kMlpW1[3 * 32 + k] * f3;
-            h = h + kMlpW1[4 * 32 + k] * f4;
-            h = h > 0.0f ? h : 0.0f;
-#pragma unroll
-            for (int l = 0; l < 16; ++l) acc[l] = acc[l] + kMlpW2[k * 32 + half * 16 + l] * h;
-        }
-#pragma unroll
-        for (int l = 0; l < 16; ++l) {
-            const float v = acc[l] > 0.0f ? acc[l] : 0.0f;
-            out = out + kMlpW3[half * 16 + l] * v;
-        }
-    }
+    for (int l = 0; l < H; ++l) out = fmaf(kMlpW3[l], acc[l] > 0.0f ? acc[l] : 0.0f, out);
     const float cj = ceilf(out);
     // saturate before converting (the prediction of a wild input is still
     // clamped to [0, m-1] by the caller)
@@ -1729,7 +1723,7 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
         else if (pass == 1)                               \
             LAUNCH(m_, t_, 1, 1, pf_, w_);                \
         else if (pass == 3)                               \
-            LAUNCH(m_, t_, 1, 3, (m_ == 32 ? 4 : pf_), w_); \
+            LAUNCH(m_, t_, 1, 3, pf_, w_);                \
         else                                              \
             LAUNCH(m_, t_, 1, 2, pf_, w_);                \
         return hipGetLastError();                         \

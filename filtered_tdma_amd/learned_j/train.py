@@ -4,7 +4,8 @@ The filtered solve's final pass corrects x_j = d'_j - a'_j x_0 - c'_j x_N on
 every interior row of a partition; the reference truncates the a-term past
 row J and the c-term before row m-1-J, with one J per partition from the
 analytic bound cal_J (filtered_tdma.cpp:63-115, filtered_tdma_kernels.cuh:
-220-261). Here a small MLP predicts J per (system, partition) from
+220-261). Here a small MLP (5 -> 16 -> 16 -> 1, ReLU) predicts J per (system,
+partition) from
 
     rho          max over the partition's rows 1..m-1 of |a/b|, |c/b| (<= 1)
     lg|d_0|      first right-hand-side row of the partition
@@ -42,7 +43,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
-H = 32
+H = 16
 
 
 def lg(x):
@@ -235,7 +236,7 @@ def train(seed=0, count=120000, epochs=60):
 def write(params, stats):
     W1, b1, W2, b2, W3, b3 = (np.asarray(p, dtype=np.float32) for p in params)
     with open(os.path.join(HERE, "mlp_j.json"), "w") as f:
-        json.dump({"layers": [5, H, H, 1], "W1": W1.tolist(), "b1": b1.tolist(),
+        json.dump({"layers": [5, H, H, 1], "hidden": H, "W1": W1.tolist(), "b1": b1.tolist(),
                    "W2": W2.tolist(), "b2": b2.tolist(), "W3": W3[:, 0].tolist(),
                    "b3": float(b3[0]), "margin": MARGIN, "train": stats}, f)
 
@@ -245,13 +246,14 @@ def write(params, stats):
 
     with open(os.path.join(CSRC, "ftdma_mlpj_weights.h"), "w") as f:
         f.write("// Generated by filtered_tdma_amd/learned_j/train.py: weights of the J\n"
-                "// predictor (5 -> 32 -> 32 -> 1, ReLU, FP32). Features: rho, lg|d_0|,\n"
+                f"// predictor (5 -> {H} -> {H} -> 1, ReLU, FP32). Features: rho, lg|d_0|,\n"
                 "// lg|d_m-1|, lg max|d|, lg eps_ with lg the piecewise-linear log2 of the\n"
                 "// float64 bit pattern (normalisation folded into W1, b1; output in rows).\n"
                 "#pragma once\n")
-        f.write(arr("kMlpW1", W1))  # [5][32], row-major
+        f.write(f"constexpr int kMlpH = {H};\n")
+        f.write(arr("kMlpW1", W1))  # [5][H], row-major
         f.write(arr("kMlpB1", b1))
-        f.write(arr("kMlpW2", W2))  # [32][32]
+        f.write(arr("kMlpW2", W2))  # [H][H]
         f.write(arr("kMlpB2", b2))
         f.write(arr("kMlpW3", W3[:, 0]))
         f.write(f"__constant__ float kMlpB3[1] = {{{float(b3[0]):.9e}f}};\n")

@@ -759,12 +759,12 @@ int ftdo_filtered(double* A, const double* B, double* C, double* D,
 /* ------------------------- learned-J filtered mode ------------------------
  * No counterpart in the reference: the FILTERED solve (v1 sweeps and
  * interface, filtered_tdma_kernels.cuh:69-118,284-303) with J chosen per
- * (system, partition) by a 5 -> 32 -> 32 -> 1 ReLU network in FP32 instead
+ * (system, partition) by a 5 -> H -> H -> 1 ReLU network in FP32 instead
  * of one cal_J per partition, the first dropped correction term of each side
  * checked against eps_ (J = m-1 where it exceeds it), then the truncated
  * correction of k_final_pass (.cuh:196-218) with that J. The network's
- * weights come in as one flat array (W1[5][32], b1[32], W2[32][32], b2[32],
- * W3[32], b3) and every sum runs in the order written here. */
+ * weights come in as one flat array (W1[5][H], b1[H], W2[H][H], b2[H],
+ * W3[H], b3) and every multiply-add is one fmaf, each sum in index order. */
 
 /* lg|x| = e + f for |x| = (1 + f) 2^e, read off frexp; lg 0 = -1100. */
 static float lg_feature(double x) {
@@ -775,36 +775,31 @@ static float lg_feature(double x) {
     return (float)((double)(e - 1) + (2.0 * f - 1.0));
 }
 
-static int mlp_J(const float* w, int margin, const float f[5]) {
+static int mlp_J(const float* w, int H, int margin, const float f[5]) {
     const float* W1 = w;
-    const float* b1 = W1 + 160;
-    const float* W2 = b1 + 32;
-    const float* b2 = W2 + 1024;
-    const float* W3 = b2 + 32;
-    const float* b3 = W3 + 32;
-    float hid[32], acc[32];
-    for (int k = 0; k < 32; ++k) {
+    const float* b1 = W1 + 5 * H;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H * H;
+    const float* W3 = b2 + H;
+    const float* b3 = W3 + H;
+    float acc[64];
+    for (int l = 0; l < H; ++l) acc[l] = b2[l];
+    for (int k = 0; k < H; ++k) {
         float h = b1[k];
-        for (int q = 0; q < 5; ++q) h = h + W1[q * 32 + k] * f[q];
-        hid[k] = h > 0.0f ? h : 0.0f;
-    }
-    for (int l = 0; l < 32; ++l) {
-        float a = b2[l];
-        for (int k = 0; k < 32; ++k) a = a + W2[k * 32 + l] * hid[k];
-        acc[l] = a;
+        for (int q = 0; q < 5; ++q) h = fmaf(W1[q * H + k], f[q], h);
+        h = h > 0.0f ? h : 0.0f;
+        for (int l = 0; l < H; ++l) acc[l] = fmaf(W2[k * H + l], h, acc[l]);
     }
     float out = b3[0];
-    for (int l = 0; l < 32; ++l) {
-        const float v = acc[l] > 0.0f ? acc[l] : 0.0f;
-        out = out + W3[l] * v;
-    }
+    for (int l = 0; l < H; ++l) out = fmaf(W3[l], acc[l] > 0.0f ? acc[l] : 0.0f, out);
     const float cj = ceilf(out);
     const float lim = 1.0e6f;
     return (int)(cj > lim ? lim : (cj < -lim ? -lim : cj)) + margin;
 }
 
-int ftdo_mlp_J(const float* weights, int margin, const float* features) {
-    return mlp_J(weights, margin, features);
+int ftdo_mlp_J(const float* weights, int hidden, int margin, const float* features) {
+    if (hidden < 1 || hidden > 64) return INT_MIN;
+    return mlp_J(weights, hidden, margin, features);
 }
 
 float ftdo_lg(double x) { return lg_feature(x); }
@@ -812,10 +807,12 @@ float ftdo_lg(double x) { return lg_feature(x); }
 int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
                           int n_sys, int n_row, int P, int cyclic,
                           double eps_value, int eps_direct,
-                          const float* weights, int margin, int* J_out, int nthreads) {
+                          const float* weights, int hidden, int margin, int* J_out,
+                          int nthreads) {
     int rc = check_parts(n_row, P);
     if (rc) return rc;
     if (!weights) return fail(-1, "filtered_learned: weights needed");
+    if (hidden < 1 || hidden > 64) return fail(-1, "filtered_learned: hidden width in [1, 64]");
     if (P == 1) {
         if (J_out)
             for (int i = 0; i < n_sys; ++i) J_out[i] = -1;
@@ -865,7 +862,7 @@ int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
         for (size_t i = 0; i < ld; ++i) {
             const float* f = feat + ((size_t)p * ld + i) * 4;
             const float fv[5] = {f[0], f[1], f[2], f[3], leps};
-            int Jl = mlp_J(weights, margin, fv);
+            int Jl = mlp_J(weights, hidden, margin, fv);
             Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
             const double x0 = D0[i], xN = DN[i];
             /* first a-term and first c-term the truncation drops */

@@ -70,18 +70,19 @@ int ftdo_filtered(double* A, const double* B, double* C, double* D,
 
 /* Learned-J filtered mode (no counterpart in the reference): v1 sweeps and
  * interface, J per (system, partition) from the FP32 network `weights`
- * (W1[5][32], b1[32], W2[32][32], b2[32], W3[32], b3: 1281 floats) plus
- * `margin`, checked against eps_ (J = m-1 where the first dropped term
+ * of hidden width H (W1[5][H], b1[H], W2[H][H], b2[H], W3[H], b3: H^2 + 8H + 1
+ * floats, H <= 64) plus `margin`, checked against eps_ (J = m-1 where the first dropped term
  * exceeds it). J_out (optional) receives P*n_sys values, index p*n_sys+i,
  * J | fallback << 16. GPU semantics. */
 int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
                           int n_sys, int n_row, int P, int cyclic,
                           double eps_value, int eps_direct,
-                          const float* weights, int margin, int* J_out, int nthreads);
+                          const float* weights, int hidden, int margin, int* J_out,
+                          int nthreads);
 
 /* The network's J (before the clamp to [0, m-1]) for 5 features, and the
  * piecewise-linear log2 feature, exposed for tests. */
-int ftdo_mlp_J(const float* weights, int margin, const float* features);
+int ftdo_mlp_J(const float* weights, int hidden, int margin, const float* features);
 float ftdo_lg(double x);
 
 /* Heat ADI harness (apps/heat_cpu on a parts[0] x parts[1] x parts[2] process

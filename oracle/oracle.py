@@ -57,8 +57,8 @@ def lib() -> ctypes.CDLL:
         L.ftdo_last_error.restype = ctypes.c_char_p
         fp = ctypes.POINTER(ctypes.c_float)
         L.ftdo_filtered_learned.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int,
-                                            c_double, c_int, fp, c_int, ip, c_int]
-        L.ftdo_mlp_J.argtypes = [fp, c_int, fp]
+                                            c_double, c_int, fp, c_int, c_int, ip, c_int]
+        L.ftdo_mlp_J.argtypes = [fp, c_int, c_int, fp]
         L.ftdo_lg.argtypes = [c_double]
         L.ftdo_lg.restype = ctypes.c_float
         for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_cal_J,
@@ -143,7 +143,8 @@ MLP_JSON = os.path.join(os.path.dirname(HERE), "filtered_tdma_amd", "learned_j",
 
 
 def mlp_weights(path=MLP_JSON):
-    """(flat float32 weights W1, b1, W2, b2, W3, b3; margin) of the J network."""
+    """(flat float32 weights W1, b1, W2, b2, W3, b3; hidden width H; margin)
+    of the J network."""
     import json
 
     with open(path) as f:
@@ -152,8 +153,9 @@ def mlp_weights(path=MLP_JSON):
                            np.asarray(w["b1"], np.float32), np.asarray(w["W2"], np.float32).reshape(-1),
                            np.asarray(w["b2"], np.float32), np.asarray(w["W3"], np.float32),
                            np.asarray([w["b3"]], np.float32)])
-    assert flat.size == 1281
-    return np.ascontiguousarray(flat), int(w["margin"])
+    H = int(w["hidden"])
+    assert flat.size == H * H + 8 * H + 1
+    return np.ascontiguousarray(flat), H, int(w["margin"])
 
 
 def _fp(a):
@@ -162,9 +164,9 @@ def _fp(a):
 
 def mlp_J(features, weights=None):
     """The network's J (before the clamp to [0, m-1]) for 5 features."""
-    w, margin = weights if weights is not None else mlp_weights()
+    w, H, margin = weights if weights is not None else mlp_weights()
     f = np.ascontiguousarray(features, dtype=np.float32)
-    return int(lib().ftdo_mlp_J(_fp(w), margin, _fp(f)))
+    return int(lib().ftdo_mlp_J(_fp(w), H, margin, _fp(f)))
 
 
 def lg(x):
@@ -175,13 +177,13 @@ def filtered_learned(A, B, C, D, P, eps, eps_direct=False, cyclic=False, weights
                      nthreads=0, inplace=False):
     """Learned-J filtered solve (v1 sweeps, GPU semantics): returns x and the
     J per (partition, system) with bit 16 = eps_ fallback, shape [P, n_sys]."""
-    w, margin = weights if weights is not None else mlp_weights()
+    w, H, margin = weights if weights is not None else mlp_weights()
     (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
     Jout = np.zeros(P * n_sys, dtype=np.int32)
     ip = ctypes.POINTER(ctypes.c_int)
     _check(lib().ftdo_filtered_learned(
         _p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, int(cyclic), float(eps),
-        int(bool(eps_direct)), _fp(w), int(margin), Jout.ctypes.data_as(ip), nthreads))
+        int(bool(eps_direct)), _fp(w), int(H), int(margin), Jout.ctypes.data_as(ip), nthreads))
     return D, Jout.reshape(P, n_sys)
 
 

@@ -26,11 +26,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "filtered_tdma_amd", "csrc", "ftdma_mlpj_weights.h")
 
 
-def const_weights(J0, margin=1):
+def const_weights(J0, H=16, margin=1):
     """All weights zero, b3 = J0 - margin - 0.5: the network answers J0."""
-    w = np.zeros(1281, dtype=np.float32)
+    w = np.zeros(H * H + 8 * H + 1, dtype=np.float32)
     w[-1] = np.float32(J0 - margin - 0.5)
-    return w, margin
+    return w, H, margin
 
 
 @pytest.mark.parametrize("n_sys,n_row,P,cyclic", [(37, 64, 4, False), (20, 131, 8, True),
@@ -100,13 +100,14 @@ def test_fallback_flag_matches_the_dropped_terms():
 
 
 def test_network_matches_float64_evaluation():
-    w, margin = O.mlp_weights()
-    W1 = w[:160].reshape(5, 32).astype(np.float64)
-    b1 = w[160:192].astype(np.float64)
-    W2 = w[192:1216].reshape(32, 32).astype(np.float64)
-    b2 = w[1216:1248].astype(np.float64)
-    W3 = w[1248:1280].astype(np.float64)
-    b3 = float(w[1280])
+    w, H, margin = O.mlp_weights()
+    o = np.cumsum([0, 5 * H, H, H * H, H, H])
+    W1 = w[o[0]:o[1]].reshape(5, H).astype(np.float64)
+    b1 = w[o[1]:o[2]].astype(np.float64)
+    W2 = w[o[2]:o[3]].reshape(H, H).astype(np.float64)
+    b2 = w[o[3]:o[4]].astype(np.float64)
+    W3 = w[o[4]:o[5]].astype(np.float64)
+    b3 = float(w[o[5]])
     rng = np.random.default_rng(5)
     F = np.stack([rng.uniform(0, 1, 300), rng.uniform(-40, 20, 300), rng.uniform(-40, 20, 300),
                   rng.uniform(-40, 20, 300), rng.uniform(-50, -10, 300)], 1).astype(np.float32)
@@ -115,7 +116,7 @@ def test_network_matches_float64_evaluation():
     for f, o in zip(F, out):
         if abs(o - np.round(o)) < 1e-3:
             continue
-        assert O.mlp_J(f, (w, margin)) == int(np.ceil(o)) + margin
+        assert O.mlp_J(f, (w, H, margin)) == int(np.ceil(o)) + margin
         checked += 1
     assert checked > 250
 
@@ -136,7 +137,7 @@ def test_lg_feature():
 
 
 def test_header_weights_equal_json():
-    w, margin = O.mlp_weights()
+    w, H, margin = O.mlp_weights()
     text = open(HEADER).read()
     got = []
     for name in ("kMlpW1", "kMlpB1", "kMlpW2", "kMlpB2", "kMlpW3", "kMlpB3"):
@@ -144,3 +145,4 @@ def test_header_weights_equal_json():
         got += [np.float32(float(v.strip().rstrip("f"))) for v in body.split(",")]
     np.testing.assert_array_equal(np.array(got, dtype=np.float32), w)
     assert re.search(r"kMlpMargin = (\d+);", text).group(1) == str(margin)
+    assert re.search(r"kMlpH = (\d+);", text).group(1) == str(H)
