@@ -4,7 +4,7 @@ The filtered solve's final pass corrects x_j = d'_j - a'_j x_0 - c'_j x_N on
 every interior row of a partition; the reference truncates the a-term past
 row J and the c-term before row m-1-J, with one J per partition from the
 analytic bound cal_J (filtered_tdma.cpp:63-115, filtered_tdma_kernels.cuh:
-220-261). Here a small MLP (5 -> 16 -> 16 -> 1, ReLU) predicts J per (system,
+220-261). Here a small MLP (5 -> 32 -> 32 -> 1, ReLU) predicts J per (system,
 partition) from
 
     rho          max over the partition's rows 1..m-1 of |a/b|, |c/b| (<= 1)
@@ -43,7 +43,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
-H = 16
+H = 32
 
 
 def lg(x):

@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "filtered_tdma_amd", "csrc", "ftdma_mlpj_weights.h")
 
 
-def const_weights(J0, H=16, margin=1):
+def const_weights(J0, H=32, margin=1):
     """All weights zero, b3 = J0 - margin - 0.5: the network answers J0."""
     w = np.zeros(H * H + 8 * H + 1, dtype=np.float32)
     w[-1] = np.float32(J0 - margin - 0.5)
