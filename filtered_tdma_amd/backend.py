@@ -180,6 +180,17 @@ class TdmaBackendGPU:
                                               systems_per_tile, lanes_per_system))
 
 
+    def autotune(self, A, B, C, D, reps: int = 3) -> dict:
+        """Time every candidate kernel shape on copies of D and keep the
+        fastest for later solves; returns the candidates' times and the
+        choice (the library's JSON report)."""
+        import json
+
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(_lib.load().ftd_autotune(self._h, *self._args(A, B, C, D), int(reps), buf,
+                                            len(buf)))
+        return json.loads(buf.value.decode())
+
     def last_gpu_ms(self) -> float:
         return _lib.load().ftd_last_gpu_ms(self._h)
 

@@ -554,6 +554,137 @@ ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int
     return FTD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner: times the automatic choice and every compiled register-resident
+// shape that holds this solve, each on a fresh copy of D, and keeps the
+// fastest as the handle's tuning. PaScaL keeps P partitions when P > 1 (and
+// the filtered kinds always keep theirs), so the tuned solve gives the same
+// bits as the reference partitioning; with P == 1 the internal partition
+// count is a tuning choice (results stay exact to rounding).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TuneCand {
+    int mode;  // 0 automatic, 1 automatic without the streamed kernels, 2 forced shape
+    int M, T, S, Q;
+};
+
+void tune_candidates(const ftd_handle* h, int n_row, std::vector<TuneCand>& out) {
+    out.push_back({0, 0, 0, 0, 0});
+    out.push_back({1, 0, 0, 0, 0});
+    static const int Ms[] = {8, 16, 32, 64};
+    static const int Ts[] = {256, 512};
+    static const int Ss[] = {16, 32, 64};
+    const bool filtered = h->kind != FTD_KIND_PASCAL && h->P > 1;
+    for (int M : Ms)
+        for (int T : Ts) {
+            if (filtered) {
+                if (!ftd::filtered_fused_supported(M, T)) continue;
+                int s_, m_max;
+                part_of(n_row, h->P, 0, s_, m_max);
+                if (m_max > M) continue;
+                for (int S : Ss)
+                    if (S * h->P <= T) out.push_back({2, M, T, S, 0});
+            } else {
+                if (!ftd::pascal_fused_supported(M, T)) continue;
+                const int Q = h->P > 1 ? h->P : std::max(2, (n_row + M - 1) / M);
+                for (int S : Ss) {
+                    FusedShape fs;
+                    if (shape_for(M, T, Q, n_row, S, fs)) out.push_back({2, M, T, S, Q});
+                }
+            }
+        }
+}
+
+void apply_cand(ftd_handle* h, const TuneCand& c) {
+    h->force_general = 0;
+    h->no_stream = c.mode == 1;
+    h->t_M = c.mode == 2 ? c.M : 0;
+    h->t_T = c.mode == 2 ? c.T : 0;
+    h->t_S = c.mode == 2 ? c.S : 0;
+    h->t_Q = c.mode == 2 ? c.Q : 0;
+}
+
+}  // namespace
+
+ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const double* D, int reps,
+                        char* report, size_t report_len) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (reps < 1 || reps > 100) return fail(FTD_ERR_INVALID, "reps must be in [1, 100]");
+    if (h->n_sys == 0 || h->n_row < 3) return fail(FTD_ERR_INVALID, "nothing to tune");
+    const size_t bytes = sizeof(double) * (size_t)h->n_sys * h->n_row;
+    const TuneCand saved = {h->no_stream ? 1 : (h->t_M ? 2 : 0), h->t_M, h->t_T, h->t_S, h->t_Q};
+    const int saved_general = h->force_general, saved_prof = h->profiling;
+    std::vector<TuneCand> cands;
+    tune_candidates(h, h->n_row, cands);
+    double* work = nullptr;
+    FTD_HIP(hipMalloc(&work, bytes));
+    std::string rep = "{\"candidates\": [";
+    int best = -1;
+    double best_ms = 0.0;
+    std::string best_path;
+    h->profiling = 0;
+    for (size_t k = 0; k < cands.size(); ++k) {
+        apply_cand(h, cands[k]);
+        hipError_t e = hipMemcpyAsync(work, D, bytes, hipMemcpyDeviceToDevice, h->stream);
+        ftd_status st = e == hipSuccess ? do_solve(h, A, B, C, work, h->n_sys, h->n_row, 0)
+                                        : FTD_ERR_HIP;
+        if (st != FTD_OK) continue;  // this shape does not hold the solve
+        const std::string path = h->path;
+        float total = 0.0f;
+        for (int r = 0; r < reps && st == FTD_OK; ++r) {
+            e = hipMemcpyAsync(work, D, bytes, hipMemcpyDeviceToDevice, h->stream);
+            if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+            if (e != hipSuccess) break;
+            st = do_solve(h, A, B, C, work, h->n_sys, h->n_row, 0);
+            if (st == FTD_OK) e = hipEventRecord(h->ev1, h->stream);
+            if (e == hipSuccess && st == FTD_OK) e = hipEventSynchronize(h->ev1);
+            float ms = 0.0f;
+            if (e == hipSuccess && st == FTD_OK) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+            if (e != hipSuccess) break;
+            total += ms;
+        }
+        if (e != hipSuccess || st != FTD_OK) {
+            hipFree(work);
+            apply_cand(h, saved);
+            h->force_general = saved_general;
+            h->profiling = saved_prof;
+            return e != hipSuccess ? hip_fail(e, "autotune") : st;
+        }
+        const double ms = total / reps;
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "%s{\"mode\": %d, \"M\": %d, \"T\": %d, \"S\": %d, \"Q\": %d, "
+                      "\"ms\": %.6f, \"path\": \"%s\"}",
+                      best < 0 && rep.back() == '[' ? "" : ", ", cands[k].mode, cands[k].M,
+                      cands[k].T, cands[k].S, cands[k].Q, ms, path.c_str());
+        rep += buf;
+        if (best < 0 || ms < best_ms) {
+            best = (int)k;
+            best_ms = ms;
+            best_path = path;
+        }
+    }
+    hipFree(work);
+    h->profiling = saved_prof;
+    h->force_general = saved_general;
+    if (best < 0) {
+        apply_cand(h, saved);
+        return fail(FTD_ERR_UNSUPPORTED, "no candidate shape holds this solve");
+    }
+    apply_cand(h, cands[best]);
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "], \"best\": {\"index\": %d, \"ms\": %.6f, \"path\": \"%s\"}}",
+                  best, best_ms, best_path.c_str());
+    rep += buf;
+    if (report && report_len) {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return FTD_OK;
+}
+
 static ftd_status solve_f32_common(ftd_handle* h, const float* A, const float* B, const float* C,
                                    const void* D, double* X, int d_is_float) {
     if (!h) return fail(FTD_ERR_INVALID, "null handle");

@@ -785,15 +785,18 @@ __device__ inline int mlp_J(float f0, float f1, float f2, float f3, float f4) {
 #pragma unroll
     for (int l = 0; l < H; ++l) acc[l] = kMlpB2[l];
     for (int k = 0; k < H; ++k) {
-        float h = kMlpB1[k];
-        h = fmaf(kMlpW1[0 * H + k], f0, h);
-        h = fmaf(kMlpW1[1 * H + k], f1, h);
-        h = fmaf(kMlpW1[2 * H + k], f2, h);
-        h = fmaf(kMlpW1[3 * H + k], f3, h);
-        h = fmaf(kMlpW1[4 * H + k], f4, h);
+        // unit k's weights are contiguous (b1, W1 column, W2 row): three
+        // wide scalar loads and one wait per unit
+        const float* w = kMlpK + k * (8 + H);
+        float h = w[0];
+        h = fmaf(w[1], f0, h);
+        h = fmaf(w[2], f1, h);
+        h = fmaf(w[3], f2, h);
+        h = fmaf(w[4], f3, h);
+        h = fmaf(w[5], f4, h);
         h = h > 0.0f ? h : 0.0f;
 #pragma unroll
-        for (int l = 0; l < H; ++l) acc[l] = fmaf(kMlpW2[k * H + l], h, acc[l]);
+        for (int l = 0; l < H; ++l) acc[l] = fmaf(w[8 + l], h, acc[l]);
     }
     float out = kMlpB3[0];
 #pragma unroll

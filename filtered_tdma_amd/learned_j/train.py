@@ -251,9 +251,14 @@ def write(params, stats):
                 "// float64 bit pattern (normalisation folded into W1, b1; output in rows).\n"
                 "#pragma once\n")
         f.write(f"constexpr int kMlpH = {H};\n")
-        f.write(arr("kMlpW1", W1))  # [5][H], row-major
-        f.write(arr("kMlpB1", b1))
-        f.write(arr("kMlpW2", W2))  # [H][H]
+        # per first-layer unit k, 8 + H floats: b1[k], W1[0..4][k], two zeros,
+        # then row k of W2, so one unit's weights arrive in three wide loads
+        K = np.zeros((H, 8 + H), dtype=np.float32)
+        K[:, 0] = b1
+        K[:, 1:6] = W1.T
+        K[:, 8:] = W2
+        f.write("// kMlpK[k][0] = b1[k], [1..5] = W1[0..4][k], [8 + l] = W2[k][l]\n")
+        f.write(arr("kMlpK", K))
         f.write(arr("kMlpB2", b2))
         f.write(arr("kMlpW3", W3[:, 0]))
         f.write(f"__constant__ float kMlpB3[1] = {{{float(b3[0]):.9e}f}};\n")

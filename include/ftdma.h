@@ -141,6 +141,21 @@ double ftd_last_comm_ms(const ftd_handle* h);
 ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
                           int systems_per_tile, int lanes_per_system);
 
+/* Extension (autotuner; no counterpart in the reference): times the
+ * automatic choice, the automatic choice without the streamed kernels and
+ * every compiled register-resident shape (rows per lane, workgroup size,
+ * systems per workgroup, PaScaL sub-partitions) that holds a solve of the
+ * handle's dimensions, each `reps` times on a fresh device copy of D (D
+ * itself is not written; A and C may be, as by any solve), and keeps the
+ * fastest as the handle's tuning (what ftd_set_tuning sets). PaScaL keeps
+ * the handle's P partitions when P > 1 and the filtered kinds always keep
+ * theirs, so results stay bit-identical; with P == 1 the sub-partition count
+ * is tuned. Allocates one n_row x n_sys scratch array for the duration and
+ * waits on the host. `report` (optional) receives a JSON text of every
+ * candidate's mean time and the choice. */
+ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const double* D,
+                        int reps, char* report, size_t report_len);
+
 /* Extension (learned truncation depth; no counterpart in the reference):
  * mode 1 makes the FILTERED kind (v1 sweeps) choose J per (system,
  * partition) with a small MLP (filtered_tdma_amd/learned_j/train.py) from

@@ -139,10 +139,16 @@ def test_lg_feature():
 def test_header_weights_equal_json():
     w, H, margin = O.mlp_weights()
     text = open(HEADER).read()
-    got = []
-    for name in ("kMlpW1", "kMlpB1", "kMlpW2", "kMlpB2", "kMlpW3", "kMlpB3"):
+
+    def arr(name):
         body = re.search(name + r"\[\d+\] = \{([^}]*)\}", text).group(1)
-        got += [np.float32(float(v.strip().rstrip("f"))) for v in body.split(",")]
-    np.testing.assert_array_equal(np.array(got, dtype=np.float32), w)
+        return np.array([np.float32(float(v.strip().rstrip("f"))) for v in body.split(",")],
+                        dtype=np.float32)
+
+    K = arr("kMlpK").reshape(H, 8 + H)  # per unit: b1, W1 column, 0, 0, W2 row
+    assert (K[:, 6:8] == 0).all()
+    got = np.concatenate([K[:, 1:6].T.reshape(-1), K[:, 0], K[:, 8:].reshape(-1),
+                          arr("kMlpB2"), arr("kMlpW3"), arr("kMlpB3")])
+    np.testing.assert_array_equal(got, w)
     assert re.search(r"kMlpMargin = (\d+);", text).group(1) == str(margin)
     assert re.search(r"kMlpH = (\d+);", text).group(1) == str(H)
