@@ -46,6 +46,8 @@ def parse():
                     help="skip the uneven-batch and 256^3 ADI-step (C4) legs")
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the complex channel-Poisson (C5) leg")
+    ap.add_argument("--no-f4", action="store_true",
+                    help="skip the FFT pressure-step (F4) leg")
     return ap.parse_args()
 
 
@@ -292,6 +294,55 @@ def c5_channel(steps, warmup, torch):
             "path": path, "filtered_J": learned}
 
 
+def f4_poisson(steps, warmup, torch, cpu=True):
+    """The channel pressure step on the reference's re180 mesh
+    (apps/channel_cpu/input/PARA_INPUT_re180_pascal.dat: 256^3, gamma3 = 1.5,
+    H = 2, Aspect 2 pi x pi): divergence, 2-D D2Z FFTs of the 256 planes,
+    33024 complex wall-normal PaScaL solves, Z2D FFTs, projection."""
+    from filtered_tdma_amd.poisson import ChannelPoisson
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import poisson_oracle as PO
+
+    nx = ny = nz = 256
+    Lx, Ly, Lz, gam, dt = 2.0 * 6.28318530718, 2.0 * 3.14159265359, 2.0, 1.5, 1e-3
+    U, V, W = PO.random_velocity(nx, ny, nz, seed=0)
+    Pf = np.zeros_like(U)
+    dev0 = [torch.from_numpy(f).cuda() for f in (U, V, W, Pf)]
+    dev = [d.clone() for d in dev0]
+    # on torch's stream (the default), so the restores and the steps are ordered
+    ps = ChannelPoisson(nx, ny, nz, Lx, Ly, Lz, gam, "pascal", 1)
+    try:
+        for _ in range(warmup):
+            ps.solve(*dev, dt)
+        ms = []
+        for _ in range(steps):
+            for d, d0 in zip(dev, dev0):
+                d.copy_(d0)
+            ps.solve(*dev, dt)
+            ms.append(ps.last_ms())
+        path = ps.solver_path()
+    finally:
+        ps.close()
+    got = [d.cpu().numpy() for d in dev]
+    Ug, Vg = got[0].copy(), got[1].copy()
+    PO.refresh_ghosts(Ug, Vg, nx, ny)
+    ch = PO.Channel(nx, ny, nz, Lx, Ly, Lz, gam)
+    div0 = np.max(np.abs(ch.divergence(U, V, W, dt)))
+    div1 = np.max(np.abs(ch.divergence(Ug, Vg, got[2], dt)))
+    out = {"workload": "channel pressure step, re180 mesh 256^3, gamma_z=1.5, pascal, "
+                       "hipFFT D2Z/Z2D + complex wall-normal solve + projection",
+           "ms_per_step": statistics.mean(ms), "ms_median": statistics.median(ms),
+           "z_solver_path": path, "max_div_after_over_before": div1 / div0}
+    if cpu:
+        ref = [f.copy() for f in (U, V, W, Pf)]
+        t = time.perf_counter()
+        ch.solve(*ref, dt, P=1)
+        out["cpu_oracle_s"] = time.perf_counter() - t
+        out["max_abs_diff_vs_oracle"] = float(max(np.max(np.abs(g - r)) for g, r in zip(got, ref)))
+    return out
+
+
 def cpu_baseline(A, B, C, D):
     """The oracle's PaScaL (P = 8) on the host cores, whole batch, in place on
     copies made outside the timed region; reps until ~10 s have passed."""
@@ -395,6 +446,12 @@ def main():
         print(f"[bench] c5 complex: {c5['complex_ms']:.4f} ms vs two real solves "
               f"{c5['two_real_ms']:.4f} ms ({c5['path']})", file=sys.stderr, flush=True)
 
+    f4 = None
+    if not args.no_f4:
+        f4 = f4_poisson(min(args.steps, 10), min(args.warmup, 2), torch, cpu=not args.no_cpu)
+        print(f"[bench] f4 pressure step 256^3: {f4['ms_per_step']:.4f} ms "
+              f"(div ratio {f4['max_div_after_over_before']:.1e})", file=sys.stderr, flush=True)
+
     head = results["pascal"] if "pascal" in results else next(iter(results.values()))
     achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9
     traffic = None
@@ -435,6 +492,8 @@ def main():
         line["c4"] = c4
     if c5 is not None:
         line["c5"] = c5
+    if f4 is not None:
+        line["f4"] = f4
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(A, B, C, D)
     print(json.dumps(line), flush=True)

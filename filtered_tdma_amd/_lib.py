@@ -21,6 +21,8 @@ EXPORTS = (
     "ftd_autotune",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
+    "ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solver", "ftd_poisson_solve",
+    "ftd_poisson_solve_rhs", "ftd_poisson_last_ms", "ftd_poisson_last_error_string",
 )
 
 _lib = None
@@ -84,6 +86,19 @@ def load() -> ctypes.CDLL:
     L.ftd_heat_last_step_ms.argtypes = [vp]
     L.ftd_heat_last_step_ms.restype = c_double
     L.ftd_heat_last_error_string.restype = ctypes.c_char_p
+    L.ftd_poisson_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_double, c_double,
+                                     c_double, c_double, c_int, c_int, c_double, vp]
+    L.ftd_poisson_destroy.argtypes = [vp]
+    L.ftd_poisson_solver.argtypes = [vp]
+    L.ftd_poisson_solver.restype = vp
+    L.ftd_poisson_solve.argtypes = [vp, dp, dp, dp, dp, c_double]
+    L.ftd_poisson_solve_rhs.argtypes = [vp, dp, dp]
+    L.ftd_poisson_last_ms.argtypes = [vp]
+    L.ftd_poisson_last_ms.restype = c_double
+    L.ftd_poisson_last_error_string.restype = ctypes.c_char_p
+    for name in ("ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solve",
+                 "ftd_poisson_solve_rhs"):
+        getattr(L, name).restype = c_int
     for name in ("ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step",
                  "ftd_heat_field", "ftd_heat_l2", "ftd_heat_set_fused"):
         getattr(L, name).restype = c_int
@@ -100,6 +115,11 @@ def load() -> ctypes.CDLL:
 def check(status: int) -> None:
     if status != 0:
         raise FtdError(status, load().ftd_last_error_string().decode())
+
+
+def check_poisson(status: int) -> None:
+    if status != 0:
+        raise FtdError(status, load().ftd_poisson_last_error_string().decode())
 
 
 def check_heat(status: int) -> None:

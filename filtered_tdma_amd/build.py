@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libftdma.so")
 SOURCES = [os.path.join(CSRC, "ftdma_kernels.hip"), os.path.join(CSRC, "ftdma_api.cpp"),
-           os.path.join(CSRC, "ftdma_heat.hip"), os.path.join(CSRC, "ftdma_batched.hip")]
+           os.path.join(CSRC, "ftdma_heat.hip"), os.path.join(CSRC, "ftdma_batched.hip"),
+           os.path.join(CSRC, "ftdma_poisson.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -40,7 +41,7 @@ def build(resource_usage: bool = False, verbose: bool = False) -> str:
     srcs = []
     for src in SOURCES:
         srcs += ["-x", "hip", src]
-    cmd = [HIPCC, *FLAGS, *srcs, "-o", LIB + ".tmp"]
+    cmd = [HIPCC, *FLAGS, *srcs, "-lhipfft", "-o", LIB + ".tmp"]
     if resource_usage:
         cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
     if verbose:

@@ -251,6 +251,48 @@ double ftd_heat_last_step_ms(const ftd_heat* h);
 /* text of the last error reported by an ftd_heat_* call in this thread */
 const char* ftd_heat_last_error_string(void);
 
+/* ------------------------------------------------------------------------
+ * Channel pressure Poisson solve: the reference's PressureSolver::solve
+ * (apps/channel_cpu/PressureSolver.cpp:493-533) on one GPU. x and y are
+ * periodic and uniform, z is wall-normal on the tanh-stretched grid of
+ * Grid.cpp:16-30 (gamma_z; mirrored ghost centres). The wall-normal
+ * systems are solved by an ftd_handle of `kind` and `partitions` (the
+ * reference uses PaScaL) on the complex spectra in place.
+ * ------------------------------------------------------------------------ */
+typedef struct ftd_poisson ftd_poisson;
+
+/* Metrics, modified wavenumbers and the wall-normal coefficients
+ * (PressureSolver.cpp:36-104, 400-456) are built once; two hipFFT plans.
+ * nx even. `stream` is a hipStream_t (NULL = the null stream), as for
+ * ftd_create. */
+ftd_status ftd_poisson_create(ftd_poisson** out, int nx, int ny, int nz, double Lx, double Ly,
+                              double Lz, double gamma_z, int kind, int partitions,
+                              double eps_constant, void* stream);
+ftd_status ftd_poisson_destroy(ftd_poisson* p);
+
+/* The solver of the wall-normal systems (n_sys = (nx/2+1) ny complex, n_row
+ * = nz), e.g. for ftd_set_j_mode or ftd_set_tuning. Owned by p. */
+ftd_handle* ftd_poisson_solver(ftd_poisson* p);
+
+/* One pressure step (solve, :493-533): rhs = div(u*)/dt, FFT, wall-normal
+ * solves with the (0,0) mode pinned, inverse FFT scaled by 1/(nx ny), then
+ * P += dp and u -= dt grad dp (project_, :470-490). U, V, W, P are device
+ * fields with one ghost layer, (nx+2)(ny+2)(nz+2) doubles, x fastest;
+ * U(i) is the face between cells i-1 and i. Waits for the step and records
+ * its device time. */
+ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, double* P,
+                             double dt);
+
+/* The Poisson solve alone: dp [nz][ny][nx] from rhs [nz][ny][nx] (device
+ * arrays, no ghosts). Waits. */
+ftd_status ftd_poisson_solve_rhs(ftd_poisson* p, const double* rhs, double* dp);
+
+/* device time of the last ftd_poisson_solve, in ms */
+double ftd_poisson_last_ms(const ftd_poisson* p);
+
+/* text of the last error reported by an ftd_poisson_* call in this thread */
+const char* ftd_poisson_last_error_string(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,155 @@
+"""CPU restatement of the channel pressure solve (numpy), the checker of
+ftd_poisson_*.
+
+TEST INFRASTRUCTURE ONLY: tests/ and bench.py's CPU leg import it; the
+product never does.
+
+Follows apps/channel_cpu/PressureSolver.cpp:
+  compute_rhs_          :193-212  (1/dt) div u* with the grid's dx, dy, dz
+  FFT x, y              :110-135, 363-388 (numpy's pocketfft stands in for
+                        FFTW: same transforms, different rounding)
+  solve_tdma_z_         :400-464  coefficients, modified wavenumbers
+                        (:72-78), the (0, 0) mode pinned, two real solves
+                        with the same coefficients (oracle Thomas / PaScaL)
+  inverse FFT + scale   :505-510
+  dp ghosts             :512-526  Neumann in z, periodic in x and y
+  project_              :470-490
+and Grid::build_axis (Grid.cpp:16-95) for the metrics.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import oracle as O
+
+
+def axis_metrics(n, uniform, gamma, L, periodic):
+    """(dx, dmx), each n+2 long (index 0 and n+1 are ghosts)."""
+    if uniform == 1 or abs(gamma) < 1.0e-14:
+        xf = [L * float(i) / float(n) for i in range(n + 1)]
+    else:
+        th = math.tanh(gamma)
+        xf = [0.5 * L * (1.0 - math.tanh(gamma * (1.0 - 2.0 * (float(i) / float(n)))) / th)
+              for i in range(n + 1)]
+    xc = [0.0] * (n + 2)
+    for i in range(1, n + 1):
+        xc[i] = 0.5 * (xf[i - 1] + xf[i])
+    if periodic:
+        xc[0] = xc[n] - L
+        xc[n + 1] = xc[1] + L
+    else:
+        xc[0] = xf[0] - (xc[1] - xf[0])
+        xc[n + 1] = xf[n] + (xf[n] - xc[n])
+    dx = [0.0] * (n + 2)
+    for i in range(1, n + 1):
+        dx[i] = xf[i] - xf[i - 1]
+    dx[0] = dx[n] if periodic else dx[1]
+    dx[n + 1] = dx[1] if periodic else dx[n]
+    dmx = [0.0] * (n + 2)
+    for i in range(1, n + 2):
+        dmx[i] = xc[i] - xc[i - 1]
+    dmx[0] = dmx[n] if periodic else dmx[1]
+    return np.array(dx), np.array(dmx)
+
+
+class Channel:
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, gamma_z):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxh = nx // 2 + 1
+        self.dx, self.dmx = axis_metrics(nx, 1, 0.0, Lx, True)
+        self.dy, self.dmy = axis_metrics(ny, 1, 0.0, Ly, True)
+        self.dz, self.dmz = axis_metrics(nz, 0, gamma_z, Lz, False)
+        hx, hy = Lx / nx, Ly / ny
+        kx2 = np.array([2.0 * (1.0 - math.cos(2.0 * math.pi * i / nx)) / (hx * hx)
+                        for i in range(self.nxh)])
+        ky2 = np.array([2.0 * (1.0 - math.cos(2.0 * math.pi * (j if j <= ny // 2 else j - ny) / ny))
+                        / (hy * hy) for j in range(ny)])
+        k = np.arange(nz)
+        a = 1.0 / (self.dz[k + 1] * self.dmz[k + 1])
+        c = 1.0 / (self.dz[k + 1] * self.dmz[k + 2])
+        a[0] = 0.0
+        c[nz - 1] = 0.0
+        lam = (kx2[None, :] + ky2[:, None]).reshape(-1)  # mode s = ix + nxh * j
+        nm = self.nxh * ny
+        self.A = np.broadcast_to(a[:, None], (nz, nm)).copy()
+        self.C = np.broadcast_to(c[:, None], (nz, nm)).copy()
+        self.B = -(a + c)[:, None] - lam[None, :]
+        self.A[0, 0], self.B[0, 0], self.C[0, 0] = 0.0, 1.0, 0.0
+
+    def divergence(self, U, V, W, dt):
+        """rhs [nz, ny, nx] of fields [nz+2, ny+2, nx+2]."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        I = slice(1, nx + 1)
+        J = slice(1, ny + 1)
+        K = slice(1, nz + 1)
+        d = ((U[K, J, 2:nx + 2] - U[K, J, I]) / self.dx[None, None, 1:nx + 1]
+             + (V[K, 2:ny + 2, I] - V[K, J, I]) / self.dy[None, 1:ny + 1, None]
+             + (W[2:nz + 2, J, I] - W[K, J, I]) / self.dz[1:nz + 1, None, None])
+        return d * (1.0 / dt)
+
+    def solve_rhs(self, rhs, P=1, partitions_fn=None):
+        """dp [nz, ny, nx] (scaled, no ghosts). The wall-normal systems are
+        solved by the oracle's Thomas (P = 1) or PaScaL with P partitions."""
+        nz, ny = self.nz, self.ny
+        hat = np.fft.rfft2(rhs, axes=(1, 2))  # [nz, ny, nxh], FFTW's forward sign
+        hat[0, 0, 0] = 0.0
+        Z = hat.reshape(nz, -1)
+        out = np.empty_like(Z)
+        for part, dst in ((Z.real, "re"), (Z.imag, "im")):
+            d = np.ascontiguousarray(part)
+            if P == 1:
+                x = O.thomas(self.A, self.B, self.C, d, sem=O.SEM_GPU)
+            else:
+                x = O.pascal(self.A, self.B, self.C, d, P, sem=O.SEM_GPU)
+            if dst == "re":
+                out.real = x
+            else:
+                out.imag = x
+        return np.fft.irfft2(out.reshape(nz, ny, self.nxh), s=(ny, self.nx), axes=(1, 2))
+
+    def project(self, U, V, W, Pf, dp, dt):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        g = np.empty((nz + 2, ny + 2, nx + 2))
+        g[1:-1, 1:-1, 1:-1] = 0.0 + dp
+        g[0, 1:-1, 1:-1] = g[1, 1:-1, 1:-1]          # Neumann, lower wall
+        g[-1, 1:-1, 1:-1] = g[-2, 1:-1, 1:-1]        # Neumann, upper wall
+        g[:, :, 0] = g[:, :, nx]                     # periodic x
+        g[:, :, nx + 1] = g[:, :, 1]
+        g[:, 0, :] = g[:, ny, :]                     # periodic y
+        g[:, ny + 1, :] = g[:, 1, :]
+        c = g[1:-1, 1:-1, 1:-1]
+        I, J, K = slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1)
+        Pf[K, J, I] += c
+        U[K, J, I] -= dt * (c - g[1:-1, 1:-1, 0:nx]) / self.dmx[None, None, 1:nx + 1]
+        V[K, J, I] -= dt * (c - g[1:-1, 0:ny, 1:-1]) / self.dmy[None, 1:ny + 1, None]
+        W[K, J, I] -= dt * (c - g[0:nz, 1:-1, 1:-1]) / self.dmz[1:nz + 1, None, None]
+
+    def solve(self, U, V, W, Pf, dt, P=1):
+        dp = self.solve_rhs(self.divergence(U, V, W, dt), P)
+        self.project(U, V, W, Pf, dp, dt)
+        return dp
+
+
+def random_velocity(nx, ny, nz, seed):
+    """Face velocities ~ N(0, 1) with periodic ghosts in x and y and
+    no-penetration walls (W on the wall faces k = 1 and nz + 1 is zero)."""
+    rng = np.random.default_rng(seed)
+    sh = (nz + 2, ny + 2, nx + 2)
+    U, V, W = (rng.standard_normal(sh) for _ in range(3))
+    for F in (U, V, W):
+        F[:, :, 0] = F[:, :, nx]
+        F[:, :, nx + 1] = F[:, :, 1]
+        F[:, 0, :] = F[:, ny, :]
+        F[:, ny + 1, :] = F[:, 1, :]
+    W[1], W[nz + 1] = 0.0, 0.0
+    return U, V, W
+
+
+def refresh_ghosts(U, V, nx, ny):
+    """The halo exchange after a step: periodic ghosts of U (x) and V (y)."""
+    U[:, :, nx + 1] = U[:, :, 1]
+    U[:, :, 0] = U[:, :, nx]
+    V[:, ny + 1, :] = V[:, 1, :]
+    V[:, 0, :] = V[:, ny, :]
