@@ -21,6 +21,7 @@ EXPORTS = (
     "ftd_autotune",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
+    "ftd_heat_set_plan",
     "ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solver", "ftd_poisson_solve",
     "ftd_poisson_solve_rhs", "ftd_poisson_last_ms", "ftd_poisson_last_error_string",
 )
@@ -83,6 +84,7 @@ def load() -> ctypes.CDLL:
     L.ftd_heat_field.argtypes = [vp, vp]
     L.ftd_heat_l2.argtypes = [vp, ctypes.POINTER(c_double)]
     L.ftd_heat_set_fused.argtypes = [vp, c_int]
+    L.ftd_heat_set_plan.argtypes = [vp, c_int, c_int, c_int]
     L.ftd_heat_last_step_ms.argtypes = [vp]
     L.ftd_heat_last_step_ms.restype = c_double
     L.ftd_heat_last_error_string.restype = ctypes.c_char_p
@@ -100,7 +102,7 @@ def load() -> ctypes.CDLL:
                  "ftd_poisson_solve_rhs"):
         getattr(L, name).restype = c_int
     for name in ("ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step",
-                 "ftd_heat_field", "ftd_heat_l2", "ftd_heat_set_fused"):
+                 "ftd_heat_field", "ftd_heat_l2", "ftd_heat_set_fused", "ftd_heat_set_plan"):
         getattr(L, name).restype = c_int
     for name in ("ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr", "ftd_solve_cyclic",
                  "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",

@@ -85,19 +85,34 @@ struct Stencil {
     double xa, xb, xc, ya, yb, yc, za, zb, zc, c0;
 };
 
-// explicit RHS of the factored Crank-Nicolson step (solve_theta.cpp:114-134)
-__global__ void k_rhs(const double* __restrict__ th, double* __restrict__ rhs, Dims g, Stencil s,
-                      const double* __restrict__ cx, const double* __restrict__ cy,
-                      const double* __restrict__ cz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+// explicit RHS of the factored Crank-Nicolson step (solve_theta.cpp:114-134),
+// marching in z: a thread owns one (i, j) column of a 64 x 4 tile and walks
+// kRhsKB planes with th at k-1, k, k+1 in registers, so each value of th is
+// fetched from HBM about once (one thread per cell would fetch every plane
+// again for its neighbours above and below, from other workgroups).
+constexpr int kRhsKB = 32;
+__global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
+                                               double* __restrict__ rhs, Dims g, Stencil s,
+                                               const double* __restrict__ cx,
+                                               const double* __restrict__ cy,
+                                               const double* __restrict__ cz) {
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k0 = blockIdx.z * kRhsKB + 1;
     if (i > g.nx || j > g.ny) return;
-    const size_t c = cell(g, i, j, k);
-    const size_t sx = 1, sy = (size_t)g.nx1, sz = (size_t)g.nx1 * g.ny1;
-    rhs[c] = (s.xc * th[c + sx] + s.xa * th[c - sx]) + (s.yc * th[c + sy] + s.ya * th[c - sy]) +
-             (s.zc * th[c + sz] + s.za * th[c - sz]) + (1.0 + s.xb + s.yb + s.zb) * th[c] +
-             s.c0 * cx[i] * cy[j] * cz[k];
+    const int k1 = k0 + kRhsKB - 1 < g.nz ? k0 + kRhsKB - 1 : g.nz;
+    const size_t sy = (size_t)g.nx1, sz = (size_t)g.nx1 * g.ny1;
+    const double diag = 1.0 + s.xb + s.yb + s.zb;
+    const double cxy = s.c0 * cx[i] * cy[j];
+    size_t c = cell(g, i, j, k0);
+    double below = th[c - sz], here = th[c];
+    for (int k = k0; k <= k1; ++k, c += sz) {
+        const double above = th[c + sz];
+        rhs[c] = (s.xc * th[c + 1] + s.xa * th[c - 1]) + (s.yc * th[c + sy] + s.ya * th[c - sy]) +
+                 (s.zc * above + s.za * below) + diag * here + cxy * cz[k];
+        below = here;
+        here = above;
+    }
 }
 
 // Coefficients of one sweep (solve_theta.cpp:146-186 for Z, 212-247 Y,
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     }
     double* const pbase = rhs + here0 + (size_t)r0 * rs;
     RowAt rows{pbase, rs, cnt};
-    adi_solve_lane<M, 4, THOMAS>(
+    adi_solve_lane<M, (M == 16 ? 16 : M == 32 ? 8 : 4), THOMAS>(
         N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1) * rs, rs, live,
         lds, rows, [&](int j, double x) { pbase[(size_t)j * rs] = x; });
 }
@@ -501,27 +516,37 @@ namespace {
 // The partition count the sweep's solver picks for (N rows, P partitions):
 // P == 1 -> plain Thomas when N <= 64, else ceil(N / 64) internal
 // partitions (up to 16); P > 1 -> P. Then the fused sweep's shape, or off.
-void plan_fused(ftd_heat* h, int d, int N, int n_sys) {
+// rows_per_lane (16, 32, 64; 0 = automatic) sets how many rows one lane
+// holds, and with one caller partition (P == 1) thereby the number of
+// internal partitions Q; systems_per_tile (power of two; 0 = automatic)
+// the systems of one workgroup (S * Q <= 256)
+void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
+                int systems_per_tile = 0) {
     ftd_heat::Fused f;
     const int P = h->parts[d];
     if (h->kind != FTD_KIND_PASCAL || h->periodic[d] || N < 3 || n_sys < 1) {
         h->fused[d] = f;
         return;
     }
+    // automatic: one lane per system up to 64 rows (plain Thomas, the
+    // reference's single-partition solve); longer systems in partitions of
+    // 32 rows (y, z) or 16 rows (x, whose lines sit in an LDS tile), the
+    // fastest shapes measured at 256^3
+    const int Mc = rows_per_lane ? rows_per_lane : N <= 64 ? 64 : d == 0 ? 16 : 32;
     int Q;
     if (P == 1)
-        Q = N <= 64 ? 1 : (N + 63) / 64;
+        Q = N <= Mc ? 1 : (N + Mc - 1) / Mc;
     else
         Q = P;
     const int m_max = (N + Q - 1) / Q, m_min = N / Q;
-    if (Q > 16 || m_max > 64 || (Q > 1 && m_min < 3)) {
+    if (Q > 16 || m_max > Mc || (Q > 1 && m_min < 3)) {
         h->fused[d] = f;
         return;
     }
     f.Q = Q;
     f.thomas = Q == 1;
-    f.M = m_max <= 16 ? 16 : m_max <= 32 ? 32 : 64;
-    int S = 64;
+    f.M = rows_per_lane ? rows_per_lane : m_max <= 16 ? 16 : m_max <= 32 ? 32 : 64;
+    int S = systems_per_tile ? systems_per_tile : 64;
     while (S * Q > 256) S >>= 1;
     auto lds_of = [&](int S_) {
         const size_t red = f.thomas ? 0 : (size_t)3 * 2 * Q * S_;
@@ -600,8 +625,8 @@ ftd_status enqueue_step(ftd_heat* h) {
         hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f,
                            h->cx, h->cy, h->cz);
     }
-    hipLaunchKernelGGL(k_rhs, dim3((g.nx + 15) / 16, (g.ny + 15) / 16, h->nz), b2, 0, st, h->th,
-                       h->rhs, g, h->s, h->cx, h->cy, h->cz);
+    hipLaunchKernelGGL(k_rhs_z, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, (h->nz + kRhsKB - 1) / kRhsKB),
+                       dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz);
     HH(hipGetLastError());
     const size_t n_in = (size_t)g.nx * g.ny * h->nz;
     const dim3 b1(256), g1((unsigned)((n_in + 255) / 256));
@@ -637,6 +662,20 @@ ftd_status enqueue_step(ftd_heat* h) {
         hipLaunchKernelGGL(k_periodic, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f);
     }
     HH(hipGetLastError());
+    return FTD_OK;
+}
+
+// the materialized sweep of direction d runs the fused plan's partitioning
+// (with one caller partition the plan picks it), so both give the same bits
+ftd_status sync_solver_plan(ftd_heat* h, int d) {
+    const ftd_heat::Fused& f = h->fused[d];
+    if (f.on && !f.thomas && h->parts[d] == 1) {
+        const int T = f.M == 64 ? 256 : 512;
+        if (ftd_set_tuning(h->solver[d], f.M, T, 1 << f.lgS, f.Q) != FTD_OK)
+            return herr(FTD_ERR_UNSUPPORTED, std::string("sweep solver: ") + ftd_last_error_string());
+        return FTD_OK;
+    }
+    ftd_set_tuning(h->solver[d], 0, 0, 0, 0);
     return FTD_OK;
 }
 
@@ -721,6 +760,7 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     }
     for (int d = 0; d < 3; ++d) {
         ftd_status st = ftd_create(&h->solver[d], kind, ns[d], nr[d], h->parts[d], dt, h->stream);
+        if (st == FTD_OK) st = sync_solver_plan(h, d);
         if (st != FTD_OK) {
             std::string m = ftd_last_error_string();
             ftd_heat_destroy(h);
@@ -818,6 +858,26 @@ ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph) {
 
 // fused sweeps on (default) or off; a captured step is dropped so that the
 // next graph step captures the new sequence
+ftd_status ftd_heat_set_plan(ftd_heat* h, int dir, int rows_per_lane, int systems_per_tile) {
+    if (!h) return herr(FTD_ERR_INVALID, "null handle");
+    if (dir < 0 || dir > 2) return herr(FTD_ERR_INVALID, "dir must be 0, 1 or 2");
+    if (rows_per_lane != 0 && rows_per_lane != 16 && rows_per_lane != 32 && rows_per_lane != 64)
+        return herr(FTD_ERR_INVALID, "rows_per_lane must be 0, 16, 32 or 64");
+    if (systems_per_tile < 0 || systems_per_tile > 256 ||
+        (systems_per_tile & (systems_per_tile - 1)))
+        return herr(FTD_ERR_INVALID, "systems_per_tile must be 0 or a power of two <= 256");
+    const Dims g = h->g;
+    const int ns[3] = {g.ny * h->nz, g.nx * h->nz, g.nx * g.ny}, nr[3] = {g.nx, g.ny, h->nz};
+    plan_fused(h, dir, nr[dir], ns[dir], rows_per_lane, systems_per_tile);
+    ftd_heat::Fused& f = h->fused[dir];
+    if (f.on && f.lds > 64 * 1024 && allow_lds(f, dir) != hipSuccess) f.on = 0;
+    if ((rows_per_lane || systems_per_tile) && !f.on)
+        return herr(FTD_ERR_UNSUPPORTED, "no fused sweep with this plan for this direction");
+    ftd_status st = sync_solver_plan(h, dir);
+    if (st != FTD_OK) return st;
+    return ftd_heat_set_fused(h, h->use_fused);  // drops a captured step
+}
+
 ftd_status ftd_heat_set_fused(ftd_heat* h, int enable) {
     if (!h) return herr(FTD_ERR_INVALID, "null handle");
     h->use_fused = enable != 0;

@@ -51,6 +51,11 @@ class SolveTheta:
         """Fused sweeps (coefficients generated in the kernel) on or off."""
         _lib.check_heat(_lib.load().ftd_heat_set_fused(self._h, int(bool(enable))))
 
+    def set_plan(self, direction: int, rows_per_lane: int = 0, systems_per_tile: int = 0) -> None:
+        """Shape of one direction's fused sweep (0 = automatic)."""
+        _lib.check_heat(_lib.load().ftd_heat_set_plan(self._h, int(direction), int(rows_per_lane),
+                                                      int(systems_per_tile)))
+
     def reset(self) -> None:
         _lib.check_heat(_lib.load().ftd_heat_init(self._h))
 

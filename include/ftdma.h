@@ -239,6 +239,15 @@ ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph);
  * way. A captured step is dropped and re-captured by the next graph step. */
 ftd_status ftd_heat_set_fused(ftd_heat* h, int enable);
 
+/* Extension (tuning hook): the fused sweep of direction dir (0 = x, 1 = y,
+ * 2 = z) holds rows_per_lane (16, 32 or 64; 0 = automatic) rows per lane --
+ * with one caller partition in that direction this sets the number of
+ * internal PaScaL partitions -- and systems_per_tile systems per workgroup
+ * (a power of two, 0 = automatic). The materialized sweep is tuned to the
+ * same partitioning, so ftd_heat_set_fused(h, 0) keeps giving the same
+ * field. A captured step is dropped. */
+ftd_status ftd_heat_set_plan(ftd_heat* h, int dir, int rows_per_lane, int systems_per_tile);
+
 /* the field with ghosts, (nz+2)(ny+2)(nx+2) doubles, x fastest, to host memory */
 ftd_status ftd_heat_field(ftd_heat* h, double* host_out);
 

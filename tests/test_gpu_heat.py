@@ -136,3 +136,28 @@ def test_heat_fused_sweeps_equal_materialized(n, parts, periodic):
     finally:
         for sim in sims:
             sim.close()
+
+
+@pytest.mark.parametrize("n,plans", [
+    (100, {0: (32, 16), 1: (16, 0), 2: (64, 0)}),
+    (128, {0: (64, 0), 1: (64, 64), 2: (16, 16)}),
+    (256, {}),                                  # the automatic plans at the ADI workload
+])
+def test_heat_plans_keep_fused_equal_materialized(n, plans):
+    """Each sweep shape (rows per lane, systems per workgroup) set through
+    ftd_heat_set_plan: the fused field equals the materialized one, which
+    runs the same partitioning, bit for bit."""
+    dt = 1e-4
+    note(f"heat plans n={n} plans={plans}")
+    sims = [SolveTheta(n, n, n, parts=(1, 1, 1), backend="pascal", dt=dt) for _ in range(2)]
+    try:
+        for sim in sims:
+            for d, (m, s) in plans.items():
+                sim.set_plan(d, m, s)
+        sims[1].set_fused(False)
+        for sim in sims:
+            sim.run(2, use_graph=True)
+        np.testing.assert_array_equal(sims[0].field(), sims[1].field())
+    finally:
+        for sim in sims:
+            sim.close()
