@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -696,9 +697,26 @@ static ftd_status solve_f32_common(ftd_handle* h, const float* A, const float* B
     if (h->n_row < 6 || !pick_pascal(h, h->n_row, 0, fs))
         return fail(FTD_ERR_UNSUPPORTED, "no fused kernel holds this system for FP32 storage");
     if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
-    std::snprintf(h->path, sizeof h->path, "pascal_fused_%s M=%d T=%d S=%d Q=%d",
-                  d_is_float ? "f32" : "f32c", fs.M, fs.T, fs.S, fs.Q);
-    if (d_is_float)
+    // FP32 A, B, C, D: the streamed kernel runs the same partitions (same
+    // bits) where a variant holds them and the arrays allow its DMA
+    int s_, m_max;
+    part_of(h->n_row, fs.Q, 0, s_, m_max);
+    const int Ms =
+        (d_is_float && !h->t_M && !h->no_stream) ? ftd::stream_rows_f32(m_max, fs.Q) : 0;
+    const bool streamed =
+        Ms && ftd::stream_eligible(h->n_sys, A, B, C, D) && !((uintptr_t)X & 15u);
+    if (streamed) {
+        std::snprintf(h->path, sizeof h->path, "pascal_stream_f32 M=%d Q=%d R=%d", Ms, fs.Q,
+                      ftd::stream_ring_depth_f32(Ms, fs.Q));
+        FTD_HIP(ftd::launch_pascal_stream_f32(Ms, fs.Q, A, B, C, (const float*)D, X, h->n_sys,
+                                              h->n_row, h->stream));
+    } else {
+        std::snprintf(h->path, sizeof h->path, "pascal_fused_%s M=%d T=%d S=%d Q=%d",
+                      d_is_float ? "f32" : "f32c", fs.M, fs.T, fs.S, fs.Q);
+    }
+    if (streamed) {
+        // launched above
+    } else if (d_is_float)
         FTD_HIP(ftd::launch_pascal_fused_f32(fs.M, fs.T, A, B, C, (const float*)D, X, h->n_sys,
                                              h->n_row, fs.Q, fs.S, 0, h->stream));
     else

@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
     const double cxy = s.c0 * cx[i] * cy[j];
     size_t c = cell(g, i, j, k0);
     double below = th[c - sz], here = th[c];
+#pragma unroll 4
     for (int k = k0; k <= k1; ++k, c += sz) {
         const double above = th[c + sz];
         rhs[c] = (s.xc * th[c + 1] + s.xa * th[c - 1]) + (s.yc * th[c + sy] + s.ya * th[c - sy]) +
@@ -404,10 +405,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     double* red = tile + (size_t)S * LS;
     const int first = blockIdx.x * S;
     const int nl = n_sys - first < S ? n_sys - first : S;
-    for (int e = t; e < nl * N; e += T) {
-        const int l = e / N, i = e - l * N;
+    // a group of `step` threads (a wave, or the whole block if smaller)
+    // copies whole lines (contiguous in x); threads past the last full group
+    // only take part in the solve
+    const int step = T < 64 ? T : 64, nw = T / step, w = t / step, lane = t - w * step;
+    for (int l = w < nw ? w : nl; l < nl; l += nw) {
         const int s = first + l;
-        tile[l * LS + i] = rhs[cell(g, i + 1, s % g.ny + 1, s / g.ny + 1)];
+        const double* src = rhs + cell(g, 1, s % g.ny + 1, s / g.ny + 1);
+        double* dst = tile + l * LS;
+#pragma unroll 4
+        for (int i = lane; i < N; i += step) dst[i] = src[i];
     }
     __syncthreads();
     const int s_raw = first + sl;
@@ -427,10 +434,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
         N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1), 1, live, red,
         rows, [&](int j, double x) { myline[j] = x; });
     __syncthreads();
-    for (int e = t; e < nl * N; e += T) {
-        const int l = e / N, i = e - l * N;
+    for (int l = w < nw ? w : nl; l < nl; l += nw) {
         const int s2 = first + l;
-        th[cell(g, i + 1, s2 % g.ny + 1, s2 / g.ny + 1)] = tile[l * LS + i];
+        double* dst = th + cell(g, 1, s2 % g.ny + 1, s2 / g.ny + 1);
+        const double* src = tile + l * LS;
+#pragma unroll 4
+        for (int i = lane; i < N; i += step) dst[i] = src[i];
     }
 }
 

@@ -39,6 +39,13 @@ hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float*
 int stream_rows(int m_max, int Q);
 int stream_ring_depth(int M, int Q);
 bool stream_eligible(int n_sys, const void* A, const void* B, const void* C, const void* D);
+// FP32 storage of A, B, C, D, x in FP64 to X: same partitions and bits as
+// launch_pascal_fused_f32
+int stream_rows_f32(int m_max, int Q);
+int stream_ring_depth_f32(int M, int Q);
+hipError_t launch_pascal_stream_f32(int M, int Q, const float* A, const float* B, const float* C,
+                                    const float* D, double* X, int n_sys, int n_row,
+                                    hipStream_t st);
 hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, const double* C,
                                 double* D, int n_sys, int n_row, int cyclic, hipStream_t st);
 hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,

@@ -750,6 +750,201 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
 }
 
 // ---------------------------------------------------------------------------
+// LDS-streamed PaScaL with FP32 storage of A, B, C and D and the FP64 x in
+// its own array: the arithmetic of k_pascal_fused<..., float, float> (FP64
+// elimination on the exactly widened values, same partitions, same bits),
+// over a ring whose row-step is ONE DMA: a slot of 1 KB holds
+// [A | B | C | D] x [partition 2w | 2w+1] x 32 systems as floats, so lane l
+// loads 16 bytes of array l/16, partition 2w + (l/8)%2, systems 4(l%8) ..
+// 4(l%8)+3. 24 algorithmic bytes per row.
+// ---------------------------------------------------------------------------
+typedef const __attribute__((address_space(3))) float* lds_cfptr_t;
+typedef float ftd_flt2 __attribute__((ext_vector_type(2)));
+
+// a, b, c, d of lane l of a slot (floats at bytes 4l, 256 + 4l, 512 + 4l,
+// 768 + 4l), widened; waited for before returning
+__device__ __forceinline__ void ring_read_f32(lds_cfptr_t p, double& a, double& b, double& c,
+                                              double& d) {
+    ftd_flt2 ab, cd;
+    asm volatile(
+        "ds_read2st64_b32 %0, %2 offset1:1\n\t"
+        "ds_read2st64_b32 %1, %2 offset0:2 offset1:3\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(ab), "=&v"(cd)
+        : "v"(p)
+        : "memory");
+    a = (double)ab.x;
+    b = (double)ab.y;
+    c = (double)cd.x;
+    d = (double)cd.y;
+}
+
+template <int M, int Q, int R, int NST>
+struct StreamRingF32 {
+    static constexpr int S = 32;
+    float (*slots)[256];  // this wave's R slots
+    const float* src;     // DMA source of this lane (row 0 of its sub-block)
+    size_t ld;
+
+    __device__ __forceinline__ void init(float (*wave_slots)[256], const float* A, const float* B,
+                                         const float* C, const float* D, int n_sys, int n_row,
+                                         int w, int l, int& dcnt) {
+        slots = wave_slots;
+        ld = (size_t)n_sys;
+        const int arr = l >> 4;
+        int dr0;
+        part_of(n_row, Q, 2 * w + ((l >> 3) & 1), dr0, dcnt);
+        const float* base = arr == 0 ? A : arr == 1 ? B : arr == 2 ? C : D;
+        src = base + (size_t)dr0 * ld + 4 * (l & 7);
+    }
+    __device__ __forceinline__ void dma(int T, int j, int slot, int dcnt) const {
+        const int jj = j < dcnt ? j : dcnt - 1;
+        const size_t o = (size_t)jj * ld + (size_t)T * S;
+        __builtin_amdgcn_global_load_lds((const void*)(src + o), (lds_ptr_t)&slots[slot][0], 16, 0,
+                                         FTD_DMA_AUX);
+    }
+    __device__ __forceinline__ void prologue(int tile, int dcnt) const {
+#pragma unroll
+        for (int j = 0; j < R; ++j) dma(tile, j, j, dcnt);
+    }
+    // one DMA per row-step: R - 1 in flight after this row's, plus the M
+    // stores of the previous tile for the first R rows of a later tile
+    __device__ __forceinline__ void take(int j, bool first, int tile, int ntile, int dcnt, int l,
+                                         double& a, double& b, double& c, double& d) const {
+        constexpr int steady = R - 1;
+        constexpr int after_tail = steady + NST < 63 ? steady + NST : 63;
+        if (j >= R || first)
+            wait_vmcnt<steady>();
+        else
+            wait_vmcnt<after_tail>();
+        ring_read_f32((lds_cfptr_t)&slots[j % R][l], a, b, c, d);
+        if (j + R < M)
+            dma(tile, j + R, j % R, dcnt);
+        else
+            dma(ntile, j + R - M, j % R, dcnt);
+    }
+};
+
+template <int M, int Q, int R, int WPE>
+__global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_stream_f32(
+    const float* __restrict__ A, const float* __restrict__ B, const float* __restrict__ C,
+    const float* __restrict__ Din, double* __restrict__ X, int n_sys, int n_row, int n_tiles) {
+    constexpr int S = 32, NW = Q / 2, NR = 2 * Q;
+    static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
+    __shared__ __attribute__((aligned(16))) float ring[NW][R][256];
+    __shared__ double red[3][NR][S];
+
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int l = t & 63;
+    const int q = 2 * w + (l >> 5);
+    const int sl = l & 31;
+    const size_t ld = (size_t)n_sys;
+    StreamRingF32<M, Q, R, M> rg;
+    int dcnt0;
+    rg.init(ring[w], A, B, C, Din, n_sys, n_row, w, l, dcnt0);
+    int r0, cnt0;
+    part_of(n_row, Q, q, r0, cnt0);
+
+    int tile = blockIdx.x;
+    rg.prologue(tile, dcnt0);
+    bool first = true;
+    for (; tile < n_tiles; tile += gridDim.x) {
+        int cnt = cnt0, dcnt = dcnt0;
+        asm volatile("" : "+v"(cnt), "+v"(dcnt));
+        const int next = tile + (int)gridDim.x;
+        const int ntile = next < n_tiles ? next : tile;
+        const size_t base = (size_t)r0 * ld + (size_t)tile * S + sl;
+
+        double a[M], c[M], d[M];
+        double aN = 0.0, cN = 0.0, dN = 0.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double aj, bj, cj, dj;
+            rg.take(j, first, tile, ntile, dcnt, l, aj, bj, cj, dj);
+            if (j < cnt) {
+                if (j < 2) {
+                    a[j] = aj / bj;
+                    c[j] = cj / bj;
+                    d[j] = dj / bj;
+                } else {
+                    const double inv = 1.0 / (bj - aj * c[j - 1]);
+                    d[j] = inv * (dj - aj * d[j - 1]);
+                    c[j] = inv * cj;
+                    a[j] = -inv * aj * a[j - 1];
+                }
+                if (j == cnt - 1) {
+                    aN = a[j];
+                    cN = c[j];
+                    dN = d[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = M - 3; j >= 1; --j) {
+            if (j <= cnt - 3) {
+                const double cj = c[j];
+                d[j] = d[j] - cj * d[j + 1];
+                a[j] = a[j] - cj * a[j + 1];
+                c[j] = -cj * c[j + 1];
+            }
+        }
+        const double r = 1.0 / (1.0 - a[1] * c[0]);
+        const double d0 = r * (d[0] - c[0] * d[1]);
+        const double a0 = r * a[0];
+        const double c0 = -r * c[0] * c[1];
+
+        lds_barrier();  // the previous tile's readers of red[] are done
+        red[0][2 * q][sl] = a0;
+        red[1][2 * q][sl] = c0;
+        red[2][2 * q][sl] = d0;
+        red[0][2 * q + 1][sl] = aN;
+        red[1][2 * q + 1][sl] = cN;
+        red[2][2 * q + 1][sl] = dN;
+        lds_barrier();
+        if (q == 0) {
+            double* Ar = &red[0][0][sl];
+            double* Cr = &red[1][0][sl];
+            double* Dr = &red[2][0][sl];
+            double cp = Cr[0], dp = Dr[0];
+#pragma unroll 2
+            for (int k = 1; k < NR; ++k) {
+                const double ak = Ar[k * S];
+                const double inv = 1.0 / (1.0 - ak * cp);
+                dp = inv * (Dr[k * S] - ak * dp);
+                cp = inv * Cr[k * S];
+                Dr[k * S] = dp;
+                Cr[k * S] = cp;
+            }
+            double dn = dp;
+#pragma unroll 2
+            for (int k = NR - 2; k >= 0; --k) {
+                dn = Dr[k * S] - Cr[k * S] * dn;
+                Dr[k * S] = dn;
+            }
+        }
+        lds_barrier();
+        const double x0 = red[2][2 * q][sl];
+        const double xN = red[2][2 * q + 1][sl];
+        // exactly M stores: rows past m_p store row m_p - 1 (x_N) again
+        double* const xlast = X + base + (size_t)(cnt - 1) * ld;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double x;
+            if (j == 0)
+                x = x0;
+            else if (j >= cnt - 1)
+                x = xN;
+            else
+                x = d[j] - (a[j] * x0 + c[j] * xN);
+            FTD_ST(x, j < cnt - 1 ? X + base + (size_t)j * ld : xlast);
+        }
+        first = false;
+    }
+    wait_vmcnt<0>();  // no DMA into LDS outlives the wave
+}
+
+// ---------------------------------------------------------------------------
 // fused Filtered v1/v2 (GPU semantics, filtered_tdma_kernels.cuh:69-303):
 // one lane per (system, partition) with its m_p <= M rows in registers.
 //   PASS 0: v2 (J_p known) -> x stored.
@@ -1818,6 +2013,52 @@ hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, 
     }
     FTD_STREAM_VARIANTS(X)
 #undef X
+    return hipErrorInvalidValue;
+}
+
+// FP32 rows are half the bytes of FP64 ones, so the ring is twice as deep
+// (in rows) to keep as many bytes in flight
+#ifndef FTD_STREAM_F32_VARIANTS
+#define FTD_STREAM_F32_VARIANTS(X) \
+    X(64, 8, 32, 1)                \
+    X(32, 8, 16, 2)                \
+    X(64, 4, 32, 1)                \
+    X(32, 4, 16, 2)                \
+    X(32, 16, 16, 2)
+#endif
+
+int stream_rows_f32(int m_max, int Q) {
+    int best = 0;
+#define X_(m_, q_, r_, w_) \
+    if (Q == q_ && m_max <= m_ && (best == 0 || m_ < best)) best = m_;
+    FTD_STREAM_F32_VARIANTS(X_)
+#undef X_
+    return best;
+}
+
+int stream_ring_depth_f32(int M, int Q) {
+#define X_(m_, q_, r_, w_) \
+    if (M == m_ && Q == q_) return r_;
+    FTD_STREAM_F32_VARIANTS(X_)
+#undef X_
+    return 0;
+}
+
+hipError_t launch_pascal_stream_f32(int M, int Q, const float* A, const float* B, const float* C,
+                                    const float* D, double* X, int n_sys, int n_row,
+                                    hipStream_t st) {
+    if (!stream_eligible(n_sys, A, B, C, D) || ((uintptr_t)X & 15u)) return hipErrorInvalidValue;
+    const int n_tiles = n_sys / 32;
+#define X_(m_, q_, r_, w_)                                                                   \
+    if (M == m_ && Q == q_) {                                                                \
+        const void* k = (const void*)k_pascal_stream_f32<m_, q_, r_, w_>;                    \
+        const dim3 grid(resident_grid(k, q_ * 32, n_tiles)), block(q_ * 32);                 \
+        hipLaunchKernelGGL((k_pascal_stream_f32<m_, q_, r_, w_>), grid, block, 0, st, A, B, C, D, \
+                           X, n_sys, n_row, n_tiles);                                        \
+        return hipGetLastError();                                                            \
+    }
+    FTD_STREAM_F32_VARIANTS(X_)
+#undef X_
     return hipErrorInvalidValue;
 }
 

@@ -387,7 +387,9 @@ def test_full_size_columns_match_oracle(kind):
 # ------------------------------------------------------- FP32 storage (C3)
 
 @pytest.mark.parametrize("d32", [False, True])
-@pytest.mark.parametrize("n_sys,n_row,P", [(300, 512, 8), (257, 129, 1), (64, 96, 3)])
+@pytest.mark.parametrize("n_sys,n_row,P", [(300, 512, 8), (257, 129, 1), (64, 96, 3),
+                                           (1024, 512, 8), (2048, 256, 4), (512, 200, 8),
+                                           (2048, 512, 16)])
 def test_f32_storage_bitwise_vs_oracle_on_widened_data(n_sys, n_row, P, d32):
     """FP32 A, B, C (and D): the engine eliminates in FP64 on the exactly
     widened values, so the result equals the FP64 PaScaL of the stored data."""
@@ -410,6 +412,8 @@ def test_f32_storage_bitwise_vs_oracle_on_widened_data(n_sys, n_row, P, d32):
         x = tX.cpu().numpy()
     finally:
         be.close()
+    if d32 and n_sys % 32 == 0 and P in (4, 8, 16):
+        assert path.startswith("pascal_stream_f32"), path  # the streamed kernel, same bits
     Q = path_Q(path)
     Dw = D32.astype(np.float64) if d32 else D
     ref = O.pascal(A32.astype(np.float64), B32.astype(np.float64), C32.astype(np.float64), Dw, Q,
