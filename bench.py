@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--kinds", default="pascal,filtered_v2,filtered")
+    ap.add_argument("--kinds", default="pascal,filtered_v2,filtered,filtered_learned")
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
     ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 heat (C1) leg")
     ap.add_argument("--no-c4", action="store_true",
@@ -57,9 +57,12 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     the kernels run on."""
     from filtered_tdma_amd import TdmaBackendGPU
 
-    be = TdmaBackendGPU(kind, N_SYS, N_ROW, P, 0.0)
+    learned = kind == "filtered_learned"
+    be = TdmaBackendGPU("filtered" if learned else kind, N_SYS, N_ROW, P, 0.0)
     be.set_eps(EPS)
     be.set_rho_device(tA, tB, tC)
+    if learned:
+        be.set_j_mode(1)
     stream = torch.cuda.current_stream()
     for k in range(warmup):
         copies[k % len(copies)].copy_(D0)
@@ -78,7 +81,11 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     wall = time.perf_counter() - t0
     ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     path = be.last_path()
-    J = be.last_J() if kind != "pascal" else None
+    if learned:
+        Js, fb = be.last_J_systems()
+        J = {"mean_J": float(Js.mean()), "fallback_fraction": float(fb.mean())}
+    else:
+        J = be.last_J() if kind != "pascal" else None
     be.close()
     return wall, ev_ms, path, J
 

@@ -278,12 +278,26 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
 
     int* Jp = h->J;
     const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
-    const int Ms = (h->force_general || h->t_M || h->no_stream || csh ||
-                    (h->j_mode && version == 1))
+    const int Ms = (h->force_general || h->t_M || h->no_stream || csh)
                        ? 0
                        : ftd::stream_rows(mmax, P);
-    if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
+    const bool learned = h->j_mode == 1 && version == 1;
+    if (Ms && ftd::stream_eligible(n_sys, A, B, C, D) && (!learned || Ms == 64)) {
         const int R = ftd::stream_ring_depth(Ms, P);
+        if (learned) {
+            // learned J: one streamed pass (the 64-row variants hold the
+            // network's registers without spilling; shorter partitions take
+            // the fused kernel below)
+            const size_t nj = (size_t)P * n_sys;
+            FTD_TRY(ensure(h->jsys, sizeof(int) * nj));
+            h->jsys_count = nj;
+            std::snprintf(h->path, sizeof h->path, "filtered_learned_stream M=%d Q=%d R=%d", Ms, P,
+                          R);
+            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 3, A, B, C, D, n_sys, n_row, cyclic, Jp,
+                                                nullptr, nullptr, h->eps_value, h->eps_direct,
+                                                (int*)h->jsys.p, st));
+            return FTD_OK;
+        }
         if (version == 2) {
             int nt = 0;
             FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
@@ -292,18 +306,19 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
                                           h->eps_direct, csh, Jp, st));
             std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d", Ms, P, R);
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                nullptr, nullptr, st));
+                                                nullptr, nullptr, 0.0, 0, nullptr, st));
         } else {
             const int tiles = n_sys / 32;
             FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h->xbuf, xbytes));
             std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d", Ms, P, R);
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                (double*)h->xbuf.p, (double*)h->partial.p, st));
+                                                (double*)h->xbuf.p, (double*)h->partial.p, 0.0, 0,
+                                                nullptr, st));
             FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
                                           h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                (double*)h->xbuf.p, nullptr, st));
+                                                (double*)h->xbuf.p, nullptr, 0.0, 0, nullptr, st));
         }
         return FTD_OK;
     }

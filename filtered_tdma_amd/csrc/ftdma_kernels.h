@@ -48,10 +48,13 @@ hipError_t launch_pascal_stream_f32(int M, int Q, const float* A, const float* B
                                     hipStream_t st);
 hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, const double* C,
                                 double* D, int n_sys, int n_row, int cyclic, hipStream_t st);
+// pass 3: learned J per (system, partition), eps_ from eps_value/eps_direct,
+// J records to Jsys [Q x n_sys]
 hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,
                                   const double* B, const double* C, double* D, int n_sys,
                                   int n_row, int cyclic, const int* Jp, double* xbuf,
-                                  double* partial, hipStream_t st);
+                                  double* partial, double eps_value, int eps_direct, int* Jsys,
+                                  hipStream_t st);
 
 // uneven batches: one lane per chunk of <= 64 rows of a system, 64 lanes per
 // work item (a wave); kind 0 = single-chunk systems (plain Thomas), kind 1 =

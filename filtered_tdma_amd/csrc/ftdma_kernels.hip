@@ -1174,12 +1174,16 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
         const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;                   // first a-term dropped
         const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;       // first c-term dropped
-        double ta = 0.0, tc = 0.0;
+        // pick a'[kL] and c'[kR] first and multiply once: products per row
+        // would be shared with the final pass and held live until then
+        double akL = 0.0, ckR = 0.0;
 #pragma unroll
         for (int j = 1; j < M; ++j) {
-            if (j == kL && j <= m - 2) ta = fabs(a[j] * x0);
-            if (j == kR && j >= 1) tc = fabs(c[j] * xN);
+            if (j == kL) akL = a[j];
+            if (j == kR) ckR = c[j];
         }
+        const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
+        const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
         const bool fallback = ta > eps || tc > eps;
         Jf = fallback ? m - 1 : Jl;
         if (Jsys && live) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
@@ -1223,9 +1227,12 @@ template <int M, int Q, int R, int VERSION, int PASS, int WPE>
 __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_stream(
     const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
     double* D, int n_sys, int n_row, int n_tiles, int cyclic, const int* __restrict__ Jp,
-    double* __restrict__ xbuf, double* __restrict__ partial) {
+    double* __restrict__ xbuf, double* __restrict__ partial, double eps_value, int eps_direct,
+    int* __restrict__ Jsys) {
     constexpr int S = 32, NW = Q / 2;
-    constexpr int NST = PASS == 1 ? 3 : M;
+    // vector memory ops of a tile's tail: PASS 1 stores x0, xN and a partial
+    // max, PASS 3 the M rows and the J record
+    constexpr int NST = PASS == 1 ? 3 : PASS == 3 ? M + 1 : M;
     static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
     __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
     __shared__ double ex[4][Q][S];  // C, D of row m-1; A, D of row 0 (packed)
@@ -1266,11 +1273,18 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
         double a[M], c[M], d[M];
         double aN = 0.0, cN = 0.0, dN = 0.0;
         double b1 = 1.0;
+        double f_rho = 0.0, f_d0 = 0.0, f_dN = 0.0, f_dmax = 0.0;  // PASS 3 features
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             double aj, bj, cj, dj;
             rg.take(j, first, tile, ntile, dcnt, l, aj, bj, cj, dj);
             if (j < m) {
+                if (PASS == 3) {
+                    if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj / bj), fabs(cj / bj)));
+                    if (j == 0) f_d0 = dj;
+                    if (j == m - 1) f_dN = dj;
+                    f_dmax = fmax(f_dmax, fabs(dj));
+                }
                 if (j < 2) {
                     const double r = 1.0 / bj;
                     a[j] = aj * r;
@@ -1349,8 +1363,31 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             const double v = group_max(fmax(fabs(x0), fabs(xN)), S);
             if (sl == 0) partial[(size_t)tile * Q + p] = v;
         } else {
-            const int jL = J < m - 2 ? J : m - 2;
-            const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+            int Jf = J;
+            if (PASS == 3) {
+                // learned J of this (system, partition), checked against eps_
+                // by the first dropped term of each side (as k_filtered_fused)
+                const double Nl = (double)(m * Q);
+                const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+                int Jl = mlp_J((float)fmin(f_rho, 1.0), lg_feature(f_d0), lg_feature(f_dN),
+                               lg_feature(f_dmax), lg_feature(eps));
+                Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
+                const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;
+                const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;
+                double akL = 0.0, ckR = 0.0;
+#pragma unroll
+                for (int j = 1; j < M; ++j) {
+                    if (j == kL) akL = a[j];
+                    if (j == kR) ckR = c[j];
+                }
+                const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
+                const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
+                const bool fallback = ta > eps || tc > eps;
+                Jf = fallback ? m - 1 : Jl;
+                Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+            }
+            const int jL = Jf < m - 2 ? Jf : m - 2;
+            const int jR = (m - 1) - Jf > 1 ? (m - 1) - Jf : 1;
             double* const xlast = D + base + (size_t)(m - 1) * ld;
 #pragma unroll
             for (int j = 0; j < M; ++j) {
@@ -2065,15 +2102,18 @@ hipError_t launch_pascal_stream_f32(int M, int Q, const float* A, const float* B
 hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,
                                   const double* B, const double* C, double* D, int n_sys,
                                   int n_row, int cyclic, const int* Jp, double* xbuf,
-                                  double* partial, hipStream_t st) {
+                                  double* partial, double eps_value, int eps_direct, int* Jsys,
+                                  hipStream_t st) {
     if (!stream_eligible(n_sys, A, B, C, D)) return hipErrorInvalidValue;
+    if (pass == 3 && !Jsys) return hipErrorInvalidValue;
     const int n_tiles = n_sys / 32;
 #define LAUNCH(m_, q_, r_, w_, v_, p_)                                                        \
     do {                                                                                     \
         const void* k = (const void*)k_filtered_stream<m_, q_, r_, v_, p_, w_>;              \
         const dim3 grid(resident_grid(k, q_ * 32, n_tiles)), block(q_ * 32);                 \
         hipLaunchKernelGGL((k_filtered_stream<m_, q_, r_, v_, p_, w_>), grid, block, 0, st, A, \
-                           B, C, D, n_sys, n_row, n_tiles, cyclic, Jp, xbuf, partial);       \
+                           B, C, D, n_sys, n_row, n_tiles, cyclic, Jp, xbuf, partial,        \
+                           eps_value, eps_direct, Jsys);                                     \
     } while (0)
 #define X(m_, q_, r_, w_)                         \
     if (M == m_ && Q == q_) {                     \
@@ -2081,6 +2121,8 @@ hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const dou
             LAUNCH(m_, q_, r_, w_, 2, 0);         \
         else if (pass == 1)                       \
             LAUNCH(m_, q_, r_, w_, 1, 1);         \
+        else if (pass == 3)                       \
+            LAUNCH(m_, q_, r_, w_, 1, 3);         \
         else                                      \
             LAUNCH(m_, q_, r_, w_, 1, 2);         \
         return hipGetLastError();                 \

@@ -56,13 +56,17 @@ def run_learned(A, B, C, D, P, eps_c, cyclic=False, complex_rhs=False):
     (130, 512, 8, False),    # m = 64
     (200, 96, 4, True),      # periodic
     (100, 131, 4, False),    # uneven partitions 33/33/33/32
+    (1024, 512, 8, False),   # the streamed pass (m = 64, n_sys % 32 == 0)
+    (2048, 200, 4, False),   # streamed, partitions of 50 rows
+    (512, 384, 8, True),     # streamed, periodic
 ])
 @pytest.mark.parametrize("eps_c", [1e-2, 1e-8])
 def test_learned_bitwise(n_sys, n_row, P, cyclic, eps_c):
     note(f"learned n_sys={n_sys} n_row={n_row} P={P} cyclic={cyclic} eps_c={eps_c}")
     A, B, C, D = mixed_scale(n_sys, n_row, seed=n_row + P)
     X, J, fb, path = run_learned(A, B, C, D, P, eps_c, cyclic=cyclic)
-    assert path.startswith("filtered_learned_fused"), path
+    streamed = n_sys % 32 == 0 and (n_row + P - 1) // P > 32 and (n_row + P - 1) // P <= 64
+    assert path.startswith("filtered_learned_" + ("stream" if streamed else "fused")), path
     xr, Jr = O.filtered_learned(A, B, C, D, P, eps_c, cyclic=cyclic)
     np.testing.assert_array_equal(J, Jr & 0xFFFF)
     np.testing.assert_array_equal(fb, Jr >> 16)
