@@ -319,6 +319,7 @@ def f4_poisson(steps, warmup, torch, cpu=True):
     dev = [d.clone() for d in dev0]
     # on torch's stream (the default), so the restores and the steps are ordered
     ps = ChannelPoisson(nx, ny, nz, Lx, Ly, Lz, gam, "pascal", 1)
+    ps.set_profiling(True)
     try:
         for _ in range(warmup):
             ps.solve(*dev, dt)

@@ -24,6 +24,7 @@ EXPORTS = (
     "ftd_heat_set_plan",
     "ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solver", "ftd_poisson_solve",
     "ftd_poisson_solve_rhs", "ftd_poisson_last_ms", "ftd_poisson_last_error_string",
+    "ftd_poisson_set_profiling",
 )
 
 _lib = None
@@ -95,11 +96,12 @@ def load() -> ctypes.CDLL:
     L.ftd_poisson_solver.restype = vp
     L.ftd_poisson_solve.argtypes = [vp, dp, dp, dp, dp, c_double]
     L.ftd_poisson_solve_rhs.argtypes = [vp, dp, dp]
+    L.ftd_poisson_set_profiling.argtypes = [vp, c_int]
     L.ftd_poisson_last_ms.argtypes = [vp]
     L.ftd_poisson_last_ms.restype = c_double
     L.ftd_poisson_last_error_string.restype = ctypes.c_char_p
     for name in ("ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solve",
-                 "ftd_poisson_solve_rhs"):
+                 "ftd_poisson_solve_rhs", "ftd_poisson_set_profiling"):
         getattr(L, name).restype = c_int
     for name in ("ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step",
                  "ftd_heat_field", "ftd_heat_l2", "ftd_heat_set_fused", "ftd_heat_set_plan"):

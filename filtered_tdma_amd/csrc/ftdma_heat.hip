@@ -28,12 +28,26 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// The field on the device: (nz+2) planes of (ny+2) lines, each line of
+// nx+2 cells (ghosts included) in a pitch px (a multiple of 16 doubles),
+// starting at kOrigin so that cell i = 1 of every line -- the first
+// interior cell -- sits on a 128-byte boundary. A wave's 64 consecutive
+// interior cells are then four whole cache lines, not five partial ones.
+constexpr size_t kOrigin = 15;
 struct Dims {
-    int nx, ny, nz, nx1, ny1;
+    int nx, ny, nz, nx1, ny1;  // interior cells; lines and planes with ghosts
+    int px;                    // line pitch in doubles
 };
 
-__device__ __forceinline__ size_t cell(const Dims& g, int i, int j, int k) {
-    return ((size_t)k * g.ny1 + j) * g.nx1 + i;
+inline int line_pitch(int nx) { return (nx + 2 + 15) / 16 * 16; }
+
+__host__ __device__ __forceinline__ size_t cell(const Dims& g, int i, int j, int k) {
+    return kOrigin + ((size_t)k * g.ny1 + j) * g.px + i;
+}
+
+// doubles of a device field
+inline size_t field_doubles(const Dims& g, int nz) {
+    return kOrigin + (size_t)g.px * g.ny1 * (nz + 2) + 1;
 }
 
 // u_ghost = 2 u_BC - u_adjacent, u_BC = -cos(pi a) cos(pi b) on each face
@@ -101,7 +115,7 @@ __global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
     const int k0 = blockIdx.z * kRhsKB + 1;
     if (i > g.nx || j > g.ny) return;
     const int k1 = k0 + kRhsKB - 1 < g.nz ? k0 + kRhsKB - 1 : g.nz;
-    const size_t sy = (size_t)g.nx1, sz = (size_t)g.nx1 * g.ny1;
+    const size_t sy = (size_t)g.px, sz = (size_t)g.px * g.ny1;
     const double diag = 1.0 + s.xb + s.yb + s.zb;
     const double cxy = s.c0 * cx[i] * cy[j];
     size_t c = cell(g, i, j, k0);
@@ -158,7 +172,7 @@ __global__ void k_build(const double* __restrict__ th, const double* __restrict_
     const int i = ii + 1, j = jj + 1, k = kk + 1;
     double a = -sa, b = 1.0 - sb, c = -sc, d = rhs[cell(g, i, j, k)];
     if (wall) {
-        const size_t st = dir == 0 ? 1 : dir == 1 ? (size_t)g.nx1 : (size_t)g.nx1 * g.ny1;
+        const size_t st = dir == 0 ? 1 : dir == 1 ? (size_t)g.px : (size_t)g.px * g.ny1;
         const size_t here = cell(g, i, j, k);
         if (row == 0) {
             d += sa * (th[here - st] + th[here]);
@@ -368,7 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     const bool live = s_raw < n_sys;
     const int s = live ? s_raw : n_sys - 1;
     const int ii = s % g.nx + 1, oo = s / g.nx + 1;
-    const size_t rs = dir == 2 ? (size_t)g.nx1 * g.ny1 : (size_t)g.nx1;
+    const size_t rs = dir == 2 ? (size_t)g.px * g.ny1 : (size_t)g.px;
     const size_t here0 = dir == 2 ? cell(g, ii, oo, 1) : cell(g, ii, 1, oo);
     int r0, cnt;
     {
@@ -706,7 +720,7 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     if (kind < FTD_KIND_FILTERED || kind > FTD_KIND_PASCAL) return herr(FTD_ERR_INVALID, "bad kind");
     ftd_heat* h = new (std::nothrow) ftd_heat();
     if (!h) return herr(FTD_ERR_ALLOC, "out of host memory");
-    h->g = Dims{nx, ny, nz, nx + 2, ny + 2};
+    h->g = Dims{nx, ny, nz, nx + 2, ny + 2, line_pitch(nx)};
     h->nz = nz;
     for (int d = 0; d < 3; ++d) {
         h->parts[d] = parts ? parts[d] : 1;
@@ -731,7 +745,7 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     tab(h->hx, h->sx_, h->cx_);
     tab(h->hy, h->sy_, h->cy_);
     tab(h->hz, h->sz_, h->cz_);
-    const size_t ncell = (size_t)(nx + 2) * (ny + 2) * (nz + 2), nin = (size_t)nx * ny * nz;
+    const size_t ncell = field_doubles(h->g, nz), nin = (size_t)nx * ny * nz;
     hipError_t e = hipSuccess;
     if (stream) {
         h->stream = (hipStream_t)stream;
@@ -805,8 +819,9 @@ ftd_status ftd_heat_init(ftd_heat* h) {
             for (int i = 0; i < nx1; ++i)
                 f[((size_t)k * ny1 + j) * nx1 + i] =
                     h->sx_[i] * h->sy_[j] * h->sz_[k] + h->cx_[i] * h->cy_[j] * h->cz_[k];
-    HH(hipMemcpyAsync(h->th, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     const Dims g = h->g;
+    HH(hipMemcpy2DAsync(h->th + kOrigin, sizeof(double) * g.px, f.data(), sizeof(double) * nx1,
+                        sizeof(double) * nx1, (size_t)ny1 * nz1, hipMemcpyHostToDevice, h->stream));
     const dim3 b2(16, 16);
     for (int d = 0; d < 3; ++d) {
         if (!h->periodic[d]) continue;
@@ -905,8 +920,10 @@ ftd_status ftd_heat_set_fused(ftd_heat* h, int enable) {
 // copies the field with ghosts ((nz+2)(ny+2)(nx+2), x fastest) to host memory
 ftd_status ftd_heat_field(ftd_heat* h, double* host_out) {
     if (!h || !host_out) return herr(FTD_ERR_INVALID, "null argument");
-    const size_t ncell = (size_t)h->g.nx1 * h->g.ny1 * (h->nz + 2);
-    HH(hipMemcpyAsync(host_out, h->th, ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const Dims& g = h->g;
+    HH(hipMemcpy2DAsync(host_out, sizeof(double) * g.nx1, h->th + kOrigin, sizeof(double) * g.px,
+                        sizeof(double) * g.nx1, (size_t)g.ny1 * (h->nz + 2),
+                        hipMemcpyDeviceToHost, h->stream));
     HH(hipStreamSynchronize(h->stream));
     return FTD_OK;
 }

@@ -39,6 +39,7 @@ struct ftd_poisson {
     ftd_handle* solver = nullptr;
     double scale = 1.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    int profiling = 0;  // time each step (waits for it)
     double last_ms = 0.0;
 };
 
@@ -283,7 +284,7 @@ ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, do
     const double* dz = dmy + (p->ny + 2);
     const double* dmz = dz + (p->nz + 2);
     const dim3 block(64), grid((p->nx + 63) / 64, p->ny, p->nz);
-    PH(hipEventRecord(p->e0, p->stream));
+    if (p->profiling) PH(hipEventRecord(p->e0, p->stream));
     hipLaunchKernelGGL(k_divergence, grid, block, 0, p->stream, U, V, W, p->rhs, g, dx, dy, dz,
                        1.0 / dt);
     PH(hipGetLastError());
@@ -292,11 +293,19 @@ ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, do
     hipLaunchKernelGGL(k_project, grid, block, 0, p->stream, U, V, W, P, p->rhs, g, p->scale, dt,
                        dmx, dmy, dmz);
     PH(hipGetLastError());
-    PH(hipEventRecord(p->e1, p->stream));
-    PH(hipEventSynchronize(p->e1));
-    float ms = 0.0f;
-    PH(hipEventElapsedTime(&ms, p->e0, p->e1));
-    p->last_ms = ms;
+    if (p->profiling) {
+        PH(hipEventRecord(p->e1, p->stream));
+        PH(hipEventSynchronize(p->e1));
+        float ms = 0.0f;
+        PH(hipEventElapsedTime(&ms, p->e0, p->e1));
+        p->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+ftd_status ftd_poisson_set_profiling(ftd_poisson* p, int enable) {
+    if (!p) return perr(FTD_ERR_INVALID, "null handle");
+    p->profiling = enable != 0;
     return FTD_OK;
 }
 

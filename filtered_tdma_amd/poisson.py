@@ -42,6 +42,10 @@ class ChannelPoisson:
         _lib.check_poisson(_lib.load().ftd_poisson_solve(self._h, _ptr(U), _ptr(V), _ptr(W),
                                                          _ptr(P), float(dt)))
 
+    def set_profiling(self, enable: bool) -> None:
+        """Time each step (and wait for it); off by default."""
+        _lib.check_poisson(_lib.load().ftd_poisson_set_profiling(self._h, int(bool(enable))))
+
     def solve_rhs(self, rhs, dp) -> None:
         _lib.check_poisson(_lib.load().ftd_poisson_solve_rhs(self._h, _ptr(rhs), _ptr(dp)))
 

@@ -287,8 +287,8 @@ ftd_handle* ftd_poisson_solver(ftd_poisson* p);
  * solves with the (0,0) mode pinned, inverse FFT scaled by 1/(nx ny), then
  * P += dp and u -= dt grad dp (project_, :470-490). U, V, W, P are device
  * fields with one ghost layer, (nx+2)(ny+2)(nz+2) doubles, x fastest;
- * U(i) is the face between cells i-1 and i. Waits for the step and records
- * its device time. */
+ * U(i) is the face between cells i-1 and i. Ordered on the handle's stream;
+ * with profiling on it waits for the step and records its device time. */
 ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, double* P,
                              double dt);
 
@@ -296,7 +296,10 @@ ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, do
  * arrays, no ghosts). Waits. */
 ftd_status ftd_poisson_solve_rhs(ftd_poisson* p, const double* rhs, double* dp);
 
-/* device time of the last ftd_poisson_solve, in ms */
+/* time each ftd_poisson_solve (waits for it) or not (default) */
+ftd_status ftd_poisson_set_profiling(ftd_poisson* p, int enable);
+
+/* device time of the last ftd_poisson_solve with profiling on, in ms */
 double ftd_poisson_last_ms(const ftd_poisson* p);
 
 /* text of the last error reported by an ftd_poisson_* call in this thread */

@@ -38,6 +38,7 @@ def test_pressure_step_matches_oracle(nx, ny, nz, gamma, P):
     dev = [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in (U, V, W, Pf)]
     ps = ChannelPoisson(nx, ny, nz, Lx, Ly, Lz, gamma, "pascal", P)
     try:
+        ps.set_profiling(True)
         ps.solve(*dev, dt)
         torch.cuda.synchronize()
         assert ps.last_ms() > 0.0
