@@ -422,14 +422,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     // a group of `step` threads (a wave, or the whole block if smaller)
     // copies whole lines (contiguous in x); threads past the last full group
     // only take part in the solve
+    // (its lane index is the wave's lane index). The copy-in is direct
+    // global->LDS DMA, a dword per lane, so every line of the tile is in
+    // flight at once with no register staging.
     const int step = T < 64 ? T : 64, nw = T / step, w = t / step, lane = t - w * step;
     for (int l = w < nw ? w : nl; l < nl; l += nw) {
         const int s = first + l;
-        const double* src = rhs + cell(g, 1, s % g.ny + 1, s / g.ny + 1);
-        double* dst = tile + l * LS;
-#pragma unroll 4
-        for (int i = lane; i < N; i += step) dst[i] = src[i];
+        const char* src = (const char*)(rhs + cell(g, 1, s % g.ny + 1, s / g.ny + 1));
+        char* dst = (char*)(tile + (size_t)l * LS);
+        for (int e = 0; e < 2 * N; e += step) {
+            if (e + lane < 2 * N)
+                __builtin_amdgcn_global_load_lds((const void*)(src + 4 * (e + lane)),
+                                                 (__attribute__((address_space(3))) void*)(dst + 4 * e),
+                                                 4, 0, 0);
+        }
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int s_raw = first + sl;
     const bool live = s_raw < n_sys;
