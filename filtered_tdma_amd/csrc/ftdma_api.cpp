@@ -262,6 +262,7 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     const int n_sys = h->n_sys << csh, n_row = h->n_row, P = h->P;
     const int version = (h->kind == FTD_KIND_FILTERED_V2) ? 2 : 1;
     hipStream_t st = h->stream;
+    h->jsys_count = 0;  // J records belong to the last solve, if it was a learned one
     int s_, mmax, mmin;
     part_of(n_row, P, 0, s_, mmax);
     part_of(n_row, P, P - 1, s_, mmin);

@@ -152,3 +152,16 @@ def test_header_weights_equal_json():
     np.testing.assert_array_equal(got, w)
     assert re.search(r"kMlpMargin = (\d+);", text).group(1) == str(margin)
     assert re.search(r"kMlpH = (\d+);", text).group(1) == str(H)
+
+
+def test_training_reproduces_the_committed_weights():
+    """The weights in mlp_j.json (and so in the kernel's header) are what
+    learned_j/train.py produces: seeded, deterministic."""
+    params, stats = T.train()
+    w, H, margin = O.mlp_weights()
+    flat = np.concatenate([np.asarray(params[0], np.float32).reshape(-1),
+                           np.asarray(params[1], np.float32), np.asarray(params[2], np.float32).reshape(-1),
+                           np.asarray(params[3], np.float32), np.asarray(params[4], np.float32).reshape(-1),
+                           np.asarray(params[5], np.float32).reshape(-1)])
+    assert H == T.H and margin == T.MARGIN
+    np.testing.assert_array_equal(flat, w)
