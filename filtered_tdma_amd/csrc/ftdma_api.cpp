@@ -740,15 +740,13 @@ static ftd_status solve_f32_common(ftd_handle* h, const float* A, const float* B
     } else {
         std::snprintf(h->path, sizeof h->path, "pascal_fused_%s M=%d T=%d S=%d Q=%d",
                       d_is_float ? "f32" : "f32c", fs.M, fs.T, fs.S, fs.Q);
+        if (d_is_float)
+            FTD_HIP(ftd::launch_pascal_fused_f32(fs.M, fs.T, A, B, C, (const float*)D, X,
+                                                 h->n_sys, h->n_row, fs.Q, fs.S, 0, h->stream));
+        else
+            FTD_HIP(ftd::launch_pascal_fused_f32c(fs.M, fs.T, A, B, C, X, h->n_sys, h->n_row,
+                                                  fs.Q, fs.S, 0, h->stream));
     }
-    if (streamed) {
-        // launched above
-    } else if (d_is_float)
-        FTD_HIP(ftd::launch_pascal_fused_f32(fs.M, fs.T, A, B, C, (const float*)D, X, h->n_sys,
-                                             h->n_row, fs.Q, fs.S, 0, h->stream));
-    else
-        FTD_HIP(ftd::launch_pascal_fused_f32c(fs.M, fs.T, A, B, C, X, h->n_sys, h->n_row, fs.Q,
-                                              fs.S, 0, h->stream));
     if (h->profiling) {
         FTD_HIP(hipEventRecord(h->ev1, h->stream));
         FTD_HIP(hipEventSynchronize(h->ev1));
