@@ -751,7 +751,9 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
 
 // ---------------------------------------------------------------------------
 // LDS-streamed PaScaL with FP32 storage of A, B, C and D and the FP64 x in
-// its own array: the arithmetic of k_pascal_fused<..., float, float> (FP64
+// its own array (a kernel of its own rather than a storage-type parameter of
+// k_pascal_stream, so that the FP64 kernel's register allocation and wait
+// counts stay as audited): the arithmetic of k_pascal_fused<..., float, float> (FP64
 // elimination on the exactly widened values, same partitions, same bits),
 // over a ring whose row-step is ONE DMA: a slot of 1 KB holds
 // [A | B | C | D] x [partition 2w | 2w+1] x 32 systems as floats, so lane l
