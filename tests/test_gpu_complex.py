@@ -53,6 +53,9 @@ def general_complex(n_sys, n_row, seed):
     (37, 100, 1),     # PaScaL over internal partitions
     (300, 512, 8),
     (1024, 129, 4),   # uneven split 33/32/32/32
+    (1024, 256, 1),   # one caller partition: internal partitions of 32 rows
+    (2048, 512, 16),  # 16 caller partitions of 32 rows
+    (1024, 130, 1),   # internal partitions of 26 rows
 ])
 def test_complex_pascal_equals_two_real_solves(n_sys, n_row, P):
     note(f"complex pascal n_sys={n_sys} n_row={n_row} P={P}")
