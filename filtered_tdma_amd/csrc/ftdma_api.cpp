@@ -91,6 +91,7 @@ struct ftd_handle {
     std::vector<int64_t> b_long_off;
     std::vector<int> b_long_n;
     int b_items = 0, b_chunked = 0, b_valid = 0;
+    int64_t b_last = 0;  // last element of the planned layout
     DevBuf b_dlanes, b_dlong_off, b_dlong_n;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -786,7 +787,7 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
         if (h->b_valid) FTD_HIP(hipStreamSynchronize(h->stream));
         h->b_valid = 0;
         h->b_items = ftd::batch_plan(n_systems, row_offsets, n_rows, h->b_lanes, h->b_long_off,
-                                     h->b_long_n, h->b_chunked);
+                                     h->b_long_n, h->b_chunked, h->b_last);
         if (h->b_items < 0) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
         const size_t nl = h->b_long_n.size();
         FTD_TRY(ensure(h->b_dlanes, sizeof(ftd::BatchLane) * std::max<size_t>(1, h->b_lanes.size())));
@@ -813,7 +814,7 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
                   h->b_chunked, nl);
     if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
     FTD_HIP(ftd::launch_batched(A, B, C, D, (const ftd::BatchLane*)h->b_dlanes.p, h->b_chunked,
-                                h->b_items,
+                                h->b_items, h->b_last,
                                 (const int64_t*)h->b_dlong_off.p, (const int*)h->b_dlong_n.p, nl,
                                 h->stream));
     if (h->profiling) {

@@ -40,6 +40,15 @@
 // merges them into whole lines (nontemporal access measured 6.8x the
 // algorithmic write bytes and 2.7x the read bytes here).
 #define FTD_BST(v, p) (*(p) = (v))
+
+// two consecutive rows of x in one 16-byte store (8-byte aligned)
+typedef double ftd_dbl2 __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void store_pair(double* p, double x0, double x1) {
+    ftd_dbl2 v;
+    v.x = x0;
+    v.y = x1;
+    *(ftd_dbl2*)p = v;
+}
 constexpr int kDmaAux = 0;
 
 namespace ftd {
@@ -61,21 +70,28 @@ __device__ __forceinline__ void group_lds_sync() {
 
 // ---------------------------------------------------------------------------
 // Row staging. Blocks of RB = 8 rows of a wave's 64 chunks are brought into
-// LDS by direct global->LDS dword loads: one instruction covers 4 chunks x 8
-// rows (64 contiguous bytes per chunk), 16 instructions per array and block.
-// Instruction k of array X lands at
-//   stage + X * ARR + k * SEG  (SEG = 264: 256 bytes + 8 of padding),
-// chunk c = 4k + i at + i * 64, row r of the block at + r * 8; the padding
-// spreads the 64 chunks over the LDS banks (at most 2-way conflicts).
+// LDS by direct global->LDS loads of 16 bytes per lane (two rows of one
+// chunk): one DMA covers 16 chunks x 8 rows, 4 DMAs per array and block, 16
+// per block in all. DMA j of array X lands at
+//   stage + X * ARR + j * SEG  (SEG = 1040: 1024 bytes + 16 of padding);
+// its lane i = 16 k + cc brings rows 2k, 2k + 1 of chunk 16 j + cc, so row r
+// of chunk c sits at (c >> 4) * SEG + (c & 15) * 16 + (r >> 1) * 256 +
+// (r & 1) * 8: each lane reads its chunk's rows at immediate offsets from one
+// base, the 64 bases spread over the banks (at most 2-way conflicts).
 // Two buffers per wave: block b + 1 is in flight while block b is used, so
-// the wait for block b is vmcnt(63) (block b + 1's 64 DMAs were issued after
+// the wait for block b is vmcnt(16) (block b + 1's 16 DMAs were issued after
 // it), or vmcnt(0) for the last block of an item.
+// A pair starting on a chunk's last row also reads the element after it,
+// which is another chunk's or unused -- except after the batch's last
+// element `last`, which is never read: that one pair is read one element
+// lower, and its lane moves the row into place once the block has landed.
 // ---------------------------------------------------------------------------
 constexpr int kRB = 8;           // rows per staged block
-constexpr int kSEG = 264;        // bytes per instruction slot
-constexpr int kARR = 16 * kSEG;  // bytes per array per buffer
+constexpr int kSEG = 1040;       // bytes per DMA slot (16 chunks x 64 bytes + padding)
+constexpr int kARR = 4 * kSEG;   // bytes per array per buffer
 constexpr int kBUF = 4 * kARR;   // bytes per buffer
-static_assert(kBUF + (kRB - 1) * 8 + 3 * kARR < 65536, "ds_read immediate offsets");
+constexpr int kDMA = 16;         // DMAs per wave and block
+static_assert(kBUF + 3 * 256 + 8 + 3 * kARR < 65536, "ds_read immediate offsets");
 
 typedef __attribute__((address_space(3))) void* lds_vptr;
 typedef const __attribute__((address_space(3))) double* lds_cdptr;
@@ -85,40 +101,76 @@ struct Stage {
     unsigned char* buf0;    // this wave's buffer 0 (buffer 1 follows at + kBUF)
     const BatchLane* desc;  // this wave's 64 lane descriptors (LDS)
     lds_cdptr lane;         // this lane's chunk slot in buffer 0
+    int64_t last;           // the batch's last element
+    int fix_blk;            // block of this chunk's pair read one element low (-1: none)
+    int fix_off;            // byte offset of that pair from the lane's slot
 };
+
+__device__ __forceinline__ void stage_set_fix(Stage& st, const BatchLane& L) {
+    st.fix_blk = -1;
+    st.fix_off = 0;
+    if ((L.m & 1) && L.off + L.m - 1 == st.last) {
+        st.fix_blk = (L.m - 1) >> 3;
+        st.fix_off = (((L.m - 1) & 7) >> 1) * 256;
+    }
+}
 
 // DMA of block b (rows 8b .. 8b+7 of every chunk of the wave, clamped to the
 // chunk) into buffer t
 __device__ __forceinline__ void stage_issue(const Stage& st, int t, int b, int l,
                                             const double* A, const double* B, const double* C,
                                             const double* D) {
-    const int r = (l & 15) >> 1, h = l & 1;
-    // rolled: a few registers at a time next to the elimination state
-#pragma unroll 1
-    for (int k = 0; k < 16; ++k) {
-        // descriptor of chunk 4k + l/16, read in place (opaque, so that the
-        // 16 descriptors are not hoisted into registers for the whole sweep)
+    const int k = l >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // descriptor of chunk 16j + l%16, read in place (opaque, so that the
+        // descriptors are not hoisted into registers for the whole sweep)
         ftd_int4 cd;
         asm volatile(
             "ds_read_b128 %0, %1\n\t"
             "s_waitcnt lgkmcnt(0)"
             : "=v"(cd)
-            : "v"((const __attribute__((address_space(3))) BatchLane*)(st.desc + 4 * k + (l >> 4)))
+            : "v"((const __attribute__((address_space(3))) BatchLane*)(st.desc + 16 * j + (l & 15)))
             : "memory");
         const int64_t off = (int64_t)(((uint64_t)(uint32_t)cd.y << 32) | (uint32_t)cd.x);
         const int mm = cd.z > 0 ? cd.z : 1;
-        const int row = kRB * b + r < mm ? kRB * b + r : mm - 1;
-        const size_t e = (size_t)off + (size_t)row;
-        unsigned char* dst = st.buf0 + t * kBUF + k * kSEG;
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)(A + e) + 4 * h),
-                                         (lds_vptr)(dst + 0 * kARR), 4, 0, kDmaAux);
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)(B + e) + 4 * h),
-                                         (lds_vptr)(dst + 1 * kARR), 4, 0, kDmaAux);
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)(C + e) + 4 * h),
-                                         (lds_vptr)(dst + 2 * kARR), 4, 0, kDmaAux);
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)(D + e) + 4 * h),
-                                         (lds_vptr)(dst + 3 * kARR), 4, 0, kDmaAux);
+        const int row = kRB * b + 2 * k < mm ? kRB * b + 2 * k : mm - 1;
+        int64_t e = off + row;
+        if (e >= st.last) e = st.last - 1;
+        unsigned char* dst = st.buf0 + t * kBUF + j * kSEG;
+        __builtin_amdgcn_global_load_lds((const void*)(A + e), (lds_vptr)(dst + 0 * kARR), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(B + e), (lds_vptr)(dst + 1 * kARR), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(C + e), (lds_vptr)(dst + 2 * kARR), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(D + e), (lds_vptr)(dst + 3 * kARR), 16, 0,
+                                         kDmaAux);
     }
+}
+
+// the pair read one element low: its second row is this chunk's last
+template <int T>
+__device__ __forceinline__ void stage_fix(const Stage& st) {
+    const lds_cdptr p = (lds_cdptr)((const __attribute__((address_space(3))) unsigned char*)st.lane +
+                                    st.fix_off);
+    double a, b, c, d;
+    asm volatile(
+        "ds_read_b64 %0, %4 offset:%5\n\t"
+        "ds_read_b64 %1, %4 offset:%6\n\t"
+        "ds_read_b64 %2, %4 offset:%7\n\t"
+        "ds_read_b64 %3, %4 offset:%8\n\t"
+        "s_waitcnt lgkmcnt(0)\n\t"
+        "ds_write_b64 %4, %0 offset:%9\n\t"
+        "ds_write_b64 %4, %1 offset:%10\n\t"
+        "ds_write_b64 %4, %2 offset:%11\n\t"
+        "ds_write_b64 %4, %3 offset:%12\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+        : "v"(p), "n"(T * kBUF + 8), "n"(T * kBUF + kARR + 8), "n"(T * kBUF + 2 * kARR + 8),
+          "n"(T * kBUF + 3 * kARR + 8), "n"(T * kBUF), "n"(T * kBUF + kARR),
+          "n"(T * kBUF + 2 * kARR), "n"(T * kBUF + 3 * kARR)
+        : "memory");
 }
 
 // a, b, c, d of row r of the block in buffer t, for this lane's chunk: one
@@ -142,15 +194,17 @@ __device__ __forceinline__ void stage_read_at(lds_cdptr lane, double& a, double&
 template <int T, int R>
 __device__ __forceinline__ void stage_read(lds_cdptr lane, double& a, double& b, double& c,
                                            double& d) {
-    stage_read_at<T * kBUF + R * 8>(lane, a, b, c, d);
+    stage_read_at<T * kBUF + (R >> 1) * 256 + (R & 1) * 8>(lane, a, b, c, d);
 }
 
 __device__ __forceinline__ void vm_wait_block(bool more) {
     if (more)
-        asm volatile("s_waitcnt vmcnt(63)" ::: "memory");
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kDMA) : "memory");
     else
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
+
+__device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 __device__ __forceinline__ int wave_max(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
@@ -159,7 +213,9 @@ __device__ __forceinline__ int wave_max(int v) {
 
 // Forward sweep over the staged blocks, block by block: Block<B>::run calls
 // step(j, a, b, c, d) for the 8 rows of block B and recurses, so that every
-// row index, buffer and LDS offset is a compile-time constant.
+// row index, buffer and LDS offset is a compile-time constant. Blocks 0 and 1
+// have landed before the sweep starts (the previous item waited for them
+// before its stores); block B >= 2 was issued once block B - 2 was read.
 template <int B, int NB>
 struct Sweep {
     template <typename F>
@@ -167,7 +223,8 @@ struct Sweep {
                                                const double* Bm, const double* C,
                                                const double* D, F& step) {
         if (B < nblk) {
-            vm_wait_block(B + 1 < nblk);
+            if (B >= 2) vm_wait_block(B + 1 < nblk);
+            if (st.fix_blk == B) stage_fix<B & 1>(st);
             double a, b, c, d;
             stage_read<B & 1, 0>(st.lane, a, b, c, d); step(B * kRB + 0, a, b, c, d);
             stage_read<B & 1, 1>(st.lane, a, b, c, d); step(B * kRB + 1, a, b, c, d);
@@ -189,13 +246,11 @@ struct Sweep<NB, NB> {
                                                const double*, const double*, const double*, F&) {}
 };
 
-template <int M, typename F>
-__device__ __forceinline__ void staged_sweep(const Stage& st, int l, int nblk, const double* A,
-                                             const double* B, const double* C, const double* D,
-                                             F& step) {
+// blocks 0 and 1 of an item; the caller issues them one item ahead
+__device__ __forceinline__ void stage_lead(const Stage& st, int l, int nblk, const double* A,
+                                          const double* B, const double* C, const double* D) {
     stage_issue(st, 0, 0, l, A, B, C, D);
     if (nblk > 1) stage_issue(st, 1, 1, l, A, B, C, D);
-    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, step);
 }
 
 // single-chunk systems: plain Thomas, the whole system in one lane
@@ -217,22 +272,30 @@ struct ThomasStep {
     }
 };
 
-template <int M>
+// ahead(): issues the next item's leading blocks once this item's staged
+// rows are all read
+template <int M, typename H>
 __device__ __forceinline__ void thomas_item(const Stage& st, const double* __restrict__ A,
                                             const double* __restrict__ B,
                                             const double* __restrict__ C, double* __restrict__ D,
-                                            const BatchLane& L, int l, int nblk) {
+                                            const BatchLane& L, int l, int nblk, H& ahead) {
     ThomasStep<M> s;
     s.cnt = L.m;
     asm volatile("" : "+v"(s.cnt));  // keeps the row tests inside the item loop
     const size_t base = (size_t)L.off;
-    staged_sweep<M>(st, l, nblk, A, B, C, D, s);
+    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
+    ahead();
 #pragma unroll
     for (int j = M - 2; j >= 0; --j)
         if (j <= s.cnt - 2) s.d[j] = s.d[j] - s.c[j] * s.d[j + 1];
+    vm_drain();  // the next item's leading blocks, before this item's stores
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < s.cnt) FTD_BST(s.d[j], D + base + (size_t)j);
+    for (int j = 0; j < M; j += 2) {
+        if (j + 1 < s.cnt)
+            store_pair(D + base + (size_t)j, s.d[j], s.d[j + 1]);
+        else if (j < s.cnt)
+            FTD_BST(s.d[j], D + base + (size_t)j);
+    }
 }
 
 // one chunk of a chunked system: modified Thomas (rows 0, 1 divided by b)
@@ -264,19 +327,20 @@ struct PascalStep {
 
 // t: lane within the item (0..127); the system's reduced rows are solved in
 // the item's LDS by its chunk-0 lane, then every chunk updates its interior
-template <int M>
+template <int M, typename H>
 __device__ __forceinline__ void pascal_item(const Stage& st, const double* __restrict__ A,
                                             const double* __restrict__ B,
                                             const double* __restrict__ C, double* __restrict__ D,
                                             const BatchLane& L, int l, int t, int nblk, double* rA,
-                                            double* rC, double* rD) {
+                                            double* rC, double* rD, H& ahead) {
     PascalStep<M> s;
     s.aN = s.cN = s.dN = 0.0;
     s.cnt = L.m;
     asm volatile("" : "+v"(s.cnt));
     const size_t base = (size_t)L.off;
     const int q = L.q, Q = L.Q, lead = L.lead;
-    staged_sweep<M>(st, l, nblk, A, B, C, D, s);
+    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
+    ahead();
     const int cnt = s.cnt;
 #pragma unroll
     for (int j = M - 3; j >= 1; --j) {
@@ -352,20 +416,18 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
         }
     }
     group_lds_sync();
+    vm_drain();  // the next item's leading blocks, before this item's stores
     if (active) {
         const double x0 = rD[2 * t], xN = rD[2 * t + 1];
+        auto x_at = [&](int j) {
+            return j == 0 ? x0 : j == cnt - 1 ? xN : s.d[j] - (s.a[j] * x0 + s.c[j] * xN);
+        };
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            if (j < cnt) {
-                double x;
-                if (j == 0)
-                    x = x0;
-                else if (j == cnt - 1)
-                    x = xN;
-                else
-                    x = s.d[j] - (s.a[j] * x0 + s.c[j] * xN);
-                FTD_BST(x, D + base + (size_t)j);
-            }
+        for (int j = 0; j < M; j += 2) {
+            if (j + 1 < cnt)
+                store_pair(D + base + (size_t)j, x_at(j), x_at(j + 1));
+            else if (j < cnt)
+                FTD_BST(x_at(j), D + base + (size_t)j);
         }
     }
     group_lds_sync();  // the reduced rows are read before the next item writes them
@@ -373,38 +435,73 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
 
 // Items [0, n_chunked) hold chunked systems, [n_chunked, n_items) single-
 // chunk ones; workgroup g takes items g, g + G, g + 2G, ... of each range.
+// Each item's descriptors and first two blocks are requested while the
+// previous item finishes (backward sweep, reduced solve), and waited for
+// (vmcnt(0)) before that item's stores: an item starts on rows already in
+// LDS, and the stores' completion, which vmcnt also counts, overlaps its first
+// two blocks. The wait for block B >= 2 is then vmcnt(16): the staging DMAs
+// complete in issue order, so while any of block B's is pending all 16 of
+// block B + 1's are too, whatever the stores issued before them do.
 template <int M>
 __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) void k_batched(
     const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
-    double* __restrict__ D, const BatchLane* __restrict__ lanes, int n_chunked, int n_items) {
+    double* __restrict__ D, const BatchLane* __restrict__ lanes, int n_chunked, int n_items,
+    int64_t last) {
     __shared__ __attribute__((aligned(16))) unsigned char stage[2][2 * kBUF];
-    __shared__ __attribute__((aligned(16))) BatchLane desc[kLanes];
+    __shared__ __attribute__((aligned(16))) BatchLane desc[2][kLanes];
     __shared__ double red[3][2 * kLanes];
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int l = t & 63;
     const int G = gridDim.x;
     const int g = blockIdx.x;
+    // the single-chunk items continue the deal where the chunked ones ended
+    const int start1 = n_chunked + ((g - n_chunked % G) + G) % G;
+    int item = g < n_chunked ? g : start1;
+    if (item >= n_items) return;
     Stage st;
     st.buf0 = stage[w];
-    st.desc = desc + 64 * w;
-    st.lane = (lds_cdptr)(stage[w] + (l >> 2) * kSEG + (l & 3) * 64);
-    for (int item = g; item < n_chunked; item += G) {
-        const BatchLane L = lanes[(size_t)item * kLanes + t];
-        desc[t] = L;
-        const int nblk = __builtin_amdgcn_readfirstlane((wave_max(L.m) + kRB - 1) / kRB);
-        wave_lds_sync();
-        pascal_item<M>(st, A, B, C, D, L, l, t, nblk, red[0], red[1], red[2]);
-    }
-    // the single-chunk items continue the deal where the chunked ones ended
-    const int start = n_chunked + ((g - n_chunked % G) + G) % G;
-    for (int item = start; item < n_items; item += G) {
-        const BatchLane L = lanes[(size_t)item * kLanes + t];
-        wave_lds_sync();  // the previous item's descriptors are no longer read
-        desc[t] = L;
-        const int nblk = __builtin_amdgcn_readfirstlane((wave_max(L.m) + kRB - 1) / kRB);
-        wave_lds_sync();
-        thomas_item<M>(st, A, B, C, D, L, l, nblk);
+    st.desc = desc[0] + 64 * w;
+    st.lane = (lds_cdptr)(stage[w] + (l >> 4) * kSEG + (l & 15) * 16);
+    st.last = last;
+    // each wave writes and reads only its own 64 descriptors
+    BatchLane L = lanes[(size_t)item * kLanes + t];
+    desc[0][t] = L;
+    int nblk = __builtin_amdgcn_readfirstlane((wave_max(L.m) + kRB - 1) / kRB);
+    wave_lds_sync();
+    stage_lead(st, l, nblk, A, B, C, D);
+    vm_drain();
+    for (int u = 0;; u ^= 1) {
+        st.desc = desc[u] + 64 * w;
+        stage_set_fix(st, L);
+        int nxt = item + G;
+        if (item < n_chunked && nxt >= n_chunked) nxt = start1;
+        const bool more = nxt < n_items;
+        // the next item's descriptors go straight to LDS, waited for when
+        // this item's staged rows are all read
+        if (more)
+            __builtin_amdgcn_global_load_lds((const void*)(lanes + (size_t)nxt * kLanes + t),
+                                             (lds_vptr)(desc[u ^ 1] + 64 * w), 16, 0, 0);
+        BatchLane Ln = L;
+        int nblk_n = 0;
+        auto ahead = [&]() {
+            if (more) {
+                vm_drain();
+                Ln = desc[u ^ 1][t];
+                nblk_n = __builtin_amdgcn_readfirstlane((wave_max(Ln.m) + kRB - 1) / kRB);
+                Stage sn = st;
+                sn.desc = desc[u ^ 1] + 64 * w;
+                stage_lead(sn, l, nblk_n, A, B, C, D);
+            }
+        };
+        if (item < n_chunked)
+            pascal_item<M>(st, A, B, C, D, L, l, t, nblk, red[0], red[1], red[2], ahead);
+        else
+            thomas_item<M>(st, A, B, C, D, L, l, nblk, ahead);
+        if (!more) break;
+        item = nxt;
+        L = Ln;
+        nblk = nblk_n;
     }
 }
 
@@ -449,17 +546,23 @@ void split(int n, int P, int p, int& s, int& m) {
 
 int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
                std::vector<BatchLane>& lanes, std::vector<int64_t>& long_off,
-               std::vector<int>& long_n, int& n_chunked) {
+               std::vector<int>& long_n, int& n_chunked, int64_t& last) {
     n_chunked = 0;
     lanes.clear();
     long_off.clear();
     long_n.clear();
+    last = 0;
+    for (int s = 0; s < n_systems; ++s) {
+        if (n_rows[s] < 1) return -1;
+        last = std::max(last, row_offsets[s] + n_rows[s] - 1);
+    }
     std::vector<int> multi, single;
     for (int s = 0; s < n_systems; ++s) {
         const int N = n_rows[s];
-        if (N < 1) return -1;
         const int Q = (N + kChunk - 1) / kChunk;
-        if (Q > kLanes) {
+        // the staging reads two elements at a time, so a batch spanning one
+        // element (a single 1-row system) takes the lane-per-system kernel
+        if (Q > kLanes || last == 0) {
             long_off.push_back(row_offsets[s]);
             long_n.push_back(N);
         } else if (Q >= 2) {
@@ -570,13 +673,13 @@ int batched_resident_blocks() {
 }
 
 hipError_t launch_batched(const double* A, const double* B, double* C, double* D,
-                          const BatchLane* lanes, int n_chunked, int n_items,
+                          const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st) {
     if (n_items > 0) {
         const int blocks = std::min(batched_resident_blocks(), n_items);
         hipLaunchKernelGGL((k_batched<kChunk>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D,
-                           lanes, n_chunked, n_items);
+                           lanes, n_chunked, n_items, last);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

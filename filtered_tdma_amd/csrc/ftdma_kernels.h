@@ -69,15 +69,16 @@ struct alignas(16) BatchLane {
 };
 static_assert(sizeof(BatchLane) == 16, "BatchLane is one dwordx4");
 
-// Plans the batch: lanes (64 per item; the n_chunked items of chunked
-// systems first, then single-chunk items, each range longest first) and the
-// systems longer than 64 * 64 rows (lane-per-system kernel). Returns the
+// Plans the batch: lanes (128 per item; the n_chunked items of chunked
+// systems first, then single-chunk items, each range longest first), the
+// systems longer than 128 * 32 rows (lane-per-system kernel), and the last
+// element any system holds (the staging never reads past it). Returns the
 // number of items, or -1 when some n_rows[s] < 1.
 int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
                std::vector<BatchLane>& lanes, std::vector<int64_t>& long_off,
-               std::vector<int>& long_n, int& n_chunked);
+               std::vector<int>& long_n, int& n_chunked, int64_t& last);
 hipError_t launch_batched(const double* A, const double* B, double* C, double* D,
-                          const BatchLane* lanes, int n_chunked, int n_items,
+                          const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st);
 

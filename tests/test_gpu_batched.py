@@ -113,6 +113,71 @@ def test_batched_plan_reuse_and_relayout():
         be.close()
 
 
+def test_batched_many_items_per_workgroup():
+    """23000 systems (1.5e7 rows): several thousand work items, so every
+    workgroup of the resident grid runs a sequence of them, each item's
+    leading rows requested while the previous one finishes, across the
+    switch from chunked to single-chunk items."""
+    n_rows = np.concatenate([cases.uneven_lengths(20000, 16, 4096, seed=1),
+                             np.arange(1, 16).repeat(200)]).astype(np.int32)
+    np.random.default_rng(2).shuffle(n_rows)
+    off, n_rows, A, B, C, D = general_uneven(n_rows, seed=31, gap=1)
+    note(f"batched {n_rows.size} systems, {int(n_rows.sum())} rows")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        _, _, X, path = run_batched(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    assert path.startswith("batched"), path
+    np.testing.assert_array_equal(X, oracle_solve(off, n_rows, A, B, C, D))
+
+
+@pytest.mark.parametrize("lengths,gap", [
+    ([40, 7, 1], 1),      # a 1-row system last
+    ([300, 31], 1),       # an odd single-chunk system last
+    ([70, 5, 2049], 1),   # chunked, last chunk of 31 rows (odd)
+    ([33], 0),            # one chunked system, chunks 17 and 16
+    ([1], 1),             # one 1-row system at offset 1
+    ([1], 0),             # one 1-row system at offset 0 (lane-per-system kernel)
+    ([2, 1], 0),          # two short systems back to back, 1-row last
+])
+def test_batched_arrays_end_at_last_row(lengths, gap):
+    """The arrays end with the last system's last row: the staging reads two
+    rows at a time and must not run past it, nor drop that row."""
+    off, n_rows, A, B, C, D = general_uneven(lengths, seed=len(lengths) + 50, gap=gap)
+    end = int(off[-1]) + int(n_rows[-1])
+    A, B, C, D = (np.ascontiguousarray(v[:end]) for v in (A, B, C, D))
+    note(f"batched array end {lengths}")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        _, _, X, _ = run_batched(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    np.testing.assert_array_equal(X, oracle_solve(off, n_rows, A, B, C, D))
+
+
+def test_batched_reduced_denominator_overflow():
+    """A reduced-system denominator that overflows to -inf. System 0 (200
+    rows, 7 partitions of 29/28 rows) has a zero right-hand side, c = 1e200
+    on partition 0's last row and a = 1e200, c = 0 on partition 1's first:
+    the reduced rows' product overflows, IEEE division gives inv = -0 and x
+    stays exactly zero. A reciprocal that is not the division's (v_rcp_f64
+    and Newton steps alone give NaN for -inf) would show here."""
+    off, n_rows, A, B, C, D = general_uneven([200, 300, 40, 5], seed=77, gap=1)
+    o = int(off[0])
+    D[o:o + 200] = 0.0
+    C[o + 28], A[o + 29], C[o + 29] = 1e200, 1e200, 0.0
+    note("batched reduced-solve division fallback")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        _, _, X, _ = run_batched(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    ref = oracle_solve(off, n_rows, A, B, C, D)
+    assert np.all(ref[o:o + 200] == 0.0)
+    np.testing.assert_array_equal(X, ref)
+
+
 def test_batched_toeplitz_c4_shape_sample():
     """The uneven-batch workload's generator (a = c = -r, b = 1 + 2r,
     r ~ U[0.5, 50]) on 2000 systems, back to back."""
