@@ -333,15 +333,58 @@ __device__ __forceinline__ void adi_solve_lane(int N, int Q, int q, int r0, int 
     rD[(2 * q + 1) * S + sl] = dN;
     __syncthreads();
     if (q == 0) {
+        // the reduced rows 4 at a time: their values are read into registers
+        // first and c', d' are carried in registers, so the serial chain is
+        // the arithmetic, not a round trip through LDS per row
         double* Ar = rA + sl;
         double* Cr = rC + sl;
         double* Dr = rD + sl;
-        for (int k = 1; k < NR; ++k) {
-            const double inv = 1.0 / (1.0 - Ar[k * S] * Cr[(k - 1) * S]);
-            Dr[k * S] = inv * (Dr[k * S] - Ar[k * S] * Dr[(k - 1) * S]);
-            Cr[k * S] = inv * Cr[k * S];
+        double cp = Cr[0], dp = Dr[0];
+        int k = 1;
+        for (; k + 4 <= NR; k += 4) {
+            double ak[4], ck[4], dk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ak[u] = Ar[(k + u) * S];
+                ck[u] = Cr[(k + u) * S];
+                dk[u] = Dr[(k + u) * S];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double inv = 1.0 / (1.0 - ak[u] * cp);
+                dp = inv * (dk[u] - ak[u] * dp);
+                cp = inv * ck[u];
+                Dr[(k + u) * S] = dp;
+                Cr[(k + u) * S] = cp;
+            }
         }
-        for (int k = NR - 2; k >= 0; --k) Dr[k * S] = Dr[k * S] - Cr[k * S] * Dr[(k + 1) * S];
+        for (; k < NR; ++k) {
+            const double ak = Ar[k * S];
+            const double inv = 1.0 / (1.0 - ak * cp);
+            dp = inv * (Dr[k * S] - ak * dp);
+            cp = inv * Cr[k * S];
+            Dr[k * S] = dp;
+            Cr[k * S] = cp;
+        }
+        double dn = dp;
+        k = NR - 2;
+        for (; k >= 3; k -= 4) {
+            double ck[4], dk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ck[u] = Cr[(k - u) * S];
+                dk[u] = Dr[(k - u) * S];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                dn = dk[u] - ck[u] * dn;
+                Dr[(k - u) * S] = dn;
+            }
+        }
+        for (; k >= 0; --k) {
+            dn = Dr[k * S] - Cr[k * S] * dn;
+            Dr[k * S] = dn;
+        }
     }
     __syncthreads();
     const double x0 = rD[(2 * q) * S + sl];
@@ -561,9 +604,8 @@ void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
     }
     // automatic: one lane per system up to 64 rows (plain Thomas, the
     // reference's single-partition solve); longer systems in partitions of
-    // 32 rows (y, z) or 16 rows (x, whose lines sit in an LDS tile), the
-    // fastest shapes measured at 256^3
-    const int Mc = rows_per_lane ? rows_per_lane : N <= 64 ? 64 : d == 0 ? 16 : 32;
+    // 32 rows, the fastest shape measured at 256^3 in every direction
+    const int Mc = rows_per_lane ? rows_per_lane : N <= 64 ? 64 : 32;
     int Q;
     if (P == 1)
         Q = N <= Mc ? 1 : (N + Mc - 1) / Mc;
