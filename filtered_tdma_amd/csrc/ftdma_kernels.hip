@@ -401,12 +401,55 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         double* Cr = rC + sl;
         double* Dr = rD + sl;
         if (!CYCLIC) {
-            for (int k = 1; k < NR; ++k) {
-                const double inv = 1.0 / (1.0 - Ar[k * S] * Cr[(k - 1) * S]);
-                Dr[k * S] = inv * (Dr[k * S] - Ar[k * S] * Dr[(k - 1) * S]);
-                Cr[k * S] = inv * Cr[k * S];
+            // 4 rows at a time: their values are read into registers first
+            // and c', d' are carried in registers, so the serial chain is
+            // the arithmetic, not an LDS round trip per row
+            double cp = Cr[0], dp = Dr[0];
+            int k = 1;
+            for (; k + 4 <= NR; k += 4) {
+                double ak[4], ck[4], dk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ak[u] = Ar[(k + u) * S];
+                    ck[u] = Cr[(k + u) * S];
+                    dk[u] = Dr[(k + u) * S];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double inv = 1.0 / (1.0 - ak[u] * cp);
+                    dp = inv * (dk[u] - ak[u] * dp);
+                    cp = inv * ck[u];
+                    Dr[(k + u) * S] = dp;
+                    Cr[(k + u) * S] = cp;
+                }
             }
-            for (int k = NR - 2; k >= 0; --k) Dr[k * S] = Dr[k * S] - Cr[k * S] * Dr[(k + 1) * S];
+            for (; k < NR; ++k) {
+                const double ak = Ar[k * S];
+                const double inv = 1.0 / (1.0 - ak * cp);
+                dp = inv * (Dr[k * S] - ak * dp);
+                cp = inv * Cr[k * S];
+                Dr[k * S] = dp;
+                Cr[k * S] = cp;
+            }
+            double dn = dp;
+            k = NR - 2;
+            for (; k >= 3; k -= 4) {
+                double ck[4], dk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ck[u] = Cr[(k - u) * S];
+                    dk[u] = Dr[(k - u) * S];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    dn = dk[u] - ck[u] * dn;
+                    Dr[(k - u) * S] = dn;
+                }
+            }
+            for (; k >= 0; --k) {
+                dn = Dr[k * S] - Cr[k * S] * dn;
+                Dr[k * S] = dn;
+            }
         } else {
             double* Er = rE + sl;
             for (int k = 0; k < NR; ++k) Er[k * S] = 0.0;
