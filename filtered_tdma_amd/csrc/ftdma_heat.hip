@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/ftdma.h"
+#include "ftdma_xcd.h"
 
 namespace {
 
@@ -110,9 +111,17 @@ __global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
                                                const double* __restrict__ cx,
                                                const double* __restrict__ cy,
                                                const double* __restrict__ cz) {
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kRhsKB + 1;
+    // Workgroups go round-robin to the 8 XCDs, each with its own L2. Each
+    // XCD takes a contiguous run of tiles, y fastest, so the rows above and
+    // below a 64 x 4 tile were just read by its neighbours on the same XCD
+    // (dealt in launch order, y-adjacent tiles landed on different XCDs and
+    // every halo row came from memory again).
+    const int nbx = (g.nx + 63) / 64, nby = (g.ny + 3) / 4;
+    const int T = ftd::xcd_tile(blockIdx.x, gridDim.x);
+    const int by = T % nby, bx = (T / nby) % nbx, bz = T / (nby * nbx);
+    const int i = bx * 64 + threadIdx.x + 1;
+    const int j = by * 4 + threadIdx.y + 1;
+    const int k0 = bz * kRhsKB + 1;
     if (i > g.nx || j > g.ny) return;
     const int k1 = k0 + kRhsKB - 1 < g.nz ? k0 + kRhsKB - 1 : g.nz;
     const size_t sy = (size_t)g.px, sz = (size_t)g.px * g.ny1;
@@ -698,7 +707,8 @@ ftd_status enqueue_step(ftd_heat* h) {
         hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f,
                            h->cx, h->cy, h->cz);
     }
-    hipLaunchKernelGGL(k_rhs_z, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, (h->nz + kRhsKB - 1) / kRhsKB),
+    hipLaunchKernelGGL(k_rhs_z,
+                       dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kRhsKB - 1) / kRhsKB)),
                        dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz);
     HH(hipGetLastError());
     const size_t n_in = (size_t)g.nx * g.ny * h->nz;

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/ftdma.h"
+#include "ftdma_xcd.h"
 
 struct ftd_poisson {
     int nx = 0, ny = 0, nz = 0, nxh = 0, nm = 0;
@@ -106,8 +107,13 @@ __global__ void k_divergence(const double* __restrict__ U, const double* __restr
                              const double* __restrict__ W, double* __restrict__ rhs, Dims3 g,
                              const double* __restrict__ dx, const double* __restrict__ dy,
                              const double* __restrict__ dz, double inv_dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    // one wave per 64 cells of a line; lines in (x, y, z) order dealt to the
+    // XCDs in contiguous runs, so V's row j + 1 is the next line's own row,
+    // read on the same XCD
+    const int nbx = (g.nx + 63) / 64;
+    const int T = ftd::xcd_tile(blockIdx.x, gridDim.x);
+    const int i = (T % nbx) * 64 + threadIdx.x + 1;
+    const int j = (T / nbx) % g.ny + 1, k = T / (nbx * g.ny) + 1;
     if (i > g.nx) return;
     const size_t c = fidx(g, i, j, k);
     const double d = (U[c + 1] - U[c]) / dx[i] + (V[c + (g.nx + 2)] - V[c]) / dy[j] +
@@ -126,8 +132,11 @@ __global__ void k_project(double* __restrict__ U, double* __restrict__ V, double
                           double* __restrict__ P, const double* __restrict__ raw, Dims3 g,
                           double scale, double dt, const double* __restrict__ dmx,
                           const double* __restrict__ dmy, const double* __restrict__ dmz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    // tile order as in k_divergence: dp's row j - 1 was the previous line's
+    const int nbx = (g.nx + 63) / 64;
+    const int T = ftd::xcd_tile(blockIdx.x, gridDim.x);
+    const int i = (T % nbx) * 64 + threadIdx.x + 1;
+    const int j = (T / nbx) % g.ny + 1, k = T / (nbx * g.ny) + 1;
     if (i > g.nx) return;
     auto dp = [&](int ii, int jj, int kk) {
         return 0.0 + raw[((size_t)(kk - 1) * g.ny + (jj - 1)) * g.nx + (ii - 1)] * scale;
@@ -283,7 +292,7 @@ ftd_status ftd_poisson_solve(ftd_poisson* p, double* U, double* V, double* W, do
     const double* dmy = dy + (p->ny + 2);
     const double* dz = dmy + (p->ny + 2);
     const double* dmz = dz + (p->nz + 2);
-    const dim3 block(64), grid((p->nx + 63) / 64, p->ny, p->nz);
+    const dim3 block(64), grid(((p->nx + 63) / 64) * p->ny * p->nz);
     if (p->profiling) PH(hipEventRecord(p->e0, p->stream));
     hipLaunchKernelGGL(k_divergence, grid, block, 0, p->stream, U, V, W, p->rhs, g, dx, dy, dz,
                        1.0 / dt);
