@@ -190,11 +190,13 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
  * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
  * a system of <= 32 rows is plain Thomas, a longer one PaScaL with
  * ceil(n_rows/32) partitions inside one 128-lane workgroup; all in one
- * persistent launch (systems above 4096 rows in a second, lane-per-system
- * launch that uses C as scratch). D is overwritten with x; A and B are read only. The plan is
- * made on the host once per layout and kept on the handle; a call with a
- * new layout waits for the handle's stream first. The handle's kind and
- * sizes are not used. */
+ * persistent launch (systems above 4096 rows, and a batch that is one 1-row
+ * system at offset 0, in a second, lane-per-system launch that uses C as
+ * scratch). D is overwritten with x; A and B are read only. No element past
+ * the last row of the highest system is read, so the arrays may end there.
+ * The plan is made on the host once per layout and kept on the handle; a
+ * call with a new layout waits for the handle's stream first. The handle's
+ * kind and sizes are not used. */
 ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
                              int n_systems, double* A, double* B, double* C, double* D);
 
