@@ -153,6 +153,11 @@ class TdmaBackendGPU:
         rows = np.ascontiguousarray(n_rows, dtype=np.int32)
         if off.shape != rows.shape or off.ndim != 1:
             raise ValueError("row_offsets and n_rows must be 1-D of one length")
+        if off.size:
+            need = int(np.max(off + rows))
+            for name, x in zip("ABCD", (A, B, C, D)):
+                if hasattr(x, "numel") and x.numel() < need:
+                    raise ValueError(f"{name} holds {x.numel()} elements; the layout reaches {need}")
         ptrs = [_ptr(x) for x in (A, B, C, D)]
         _lib.check(_lib.load().ftd_solve_batched(
             self._h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),

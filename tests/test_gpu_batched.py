@@ -200,6 +200,8 @@ def test_batched_errors():
             be.solve_batched(t, t, t, t, [0, 4], [4, 0])
         with pytest.raises(FtdError):
             be.solve_batched(t, t, t, t, [-1], [4])
+        with pytest.raises(ValueError):  # the layout runs past the arrays
+            be.solve_batched(t, t, t, t, [10], [8])
         be.solve_batched(t, t, t, t, [], [])  # empty batch is a no-op
     finally:
         be.close()
