@@ -15,12 +15,16 @@ from . import _lib
 from .backend import TdmaBackendGPU
 
 
-def _ptr(x):
+def _ptr(x, numel):
+    """Device address of x: a contiguous float64 device tensor of `numel`
+    elements (checked), or a raw address (the caller's responsibility)."""
     if isinstance(x, int):
         return ctypes.c_void_p(x)
     if hasattr(x, "data_ptr"):
         if not x.is_cuda or not x.is_contiguous() or str(x.dtype) != "torch.float64":
             raise ValueError("expected a contiguous float64 device tensor")
+        if x.numel() != numel:
+            raise ValueError(f"expected {numel} elements, got {x.numel()}")
         return ctypes.c_void_p(x.data_ptr())
     raise TypeError("expected a device tensor or a device address")
 
@@ -37,17 +41,21 @@ class ChannelPoisson:
                                                 float(eps_constant), st))
         self._h = h
         self.shape = (nz, ny, nx)
+        self._cells = nz * ny * nx
+        self._field = (nz + 2) * (ny + 2) * (nx + 2)
 
     def solve(self, U, V, W, P, dt: float) -> None:
-        _lib.check_poisson(_lib.load().ftd_poisson_solve(self._h, _ptr(U), _ptr(V), _ptr(W),
-                                                         _ptr(P), float(dt)))
+        f = self._field
+        _lib.check_poisson(_lib.load().ftd_poisson_solve(self._h, _ptr(U, f), _ptr(V, f),
+                                                         _ptr(W, f), _ptr(P, f), float(dt)))
 
     def set_profiling(self, enable: bool) -> None:
         """Time each step (and wait for it); off by default."""
         _lib.check_poisson(_lib.load().ftd_poisson_set_profiling(self._h, int(bool(enable))))
 
     def solve_rhs(self, rhs, dp) -> None:
-        _lib.check_poisson(_lib.load().ftd_poisson_solve_rhs(self._h, _ptr(rhs), _ptr(dp)))
+        n = self._cells
+        _lib.check_poisson(_lib.load().ftd_poisson_solve_rhs(self._h, _ptr(rhs, n), _ptr(dp, n)))
 
     def set_j_mode(self, mode: int) -> None:
         """J mode of the wall-normal solver (filtered kinds)."""

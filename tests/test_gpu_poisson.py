@@ -72,6 +72,8 @@ def test_poisson_rhs_only():
         ps.solve_rhs(t_rhs, t_dp)
         dp = t_dp.cpu().numpy()
         np.testing.assert_array_equal(t_rhs.cpu().numpy(), rhs)  # input kept
+        with pytest.raises(ValueError):  # a field of the wrong size never reaches the GPU
+            ps.solve_rhs(t_rhs[:-1], t_dp)
     finally:
         ps.close()
     ref = PO.Channel(nx, ny, nz, 2 * np.pi, np.pi, 2.0, 2.0).solve_rhs(rhs)
