@@ -449,10 +449,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
         lds, rows, [&](int j, double x) { pbase[(size_t)j * rs] = x; });
 }
 
+// line stride of the x sweep's LDS tile, in doubles: even, so that every
+// line starts on a 16-byte boundary for the 16-byte DMAs
+__host__ __device__ inline int x_tile_stride(int N) { return N + 2 - (N & 1); }
+
 // X sweep (dir 0): the rows of a system are contiguous in x, so the
 // workgroup first copies its S lines of rhs into an LDS tile (coalesced,
-// line stride N + pad with an odd stride in doubles to spread the banks),
-// solves from the tile, writes x back into the tile and copies the tile to
+// line stride x_tile_stride(N)), solves from the tile, writes x back into the tile and copies the tile to
 // theta (coalesced).
 template <int M, bool THOMAS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_x(double* th, const double* __restrict__ rhs, Dims g,
@@ -466,28 +469,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     const int q = t >> lgS;
     const int n_sys = g.ny * nz;
     const int N = g.nx;
-    const int LS = N + ((N & 1) ? 2 : 1);  // odd line stride
+    const int LS = x_tile_stride(N);
     double* tile = lds;
     double* red = tile + (size_t)S * LS;
     const int first = blockIdx.x * S;
     const int nl = n_sys - first < S ? n_sys - first : S;
     // a group of `step` threads (a wave, or the whole block if smaller)
     // copies whole lines (contiguous in x); threads past the last full group
-    // only take part in the solve
-    // (its lane index is the wave's lane index). The copy-in is direct
-    // global->LDS DMA, a dword per lane, so every line of the tile is in
-    // flight at once with no register staging.
+    // only take part in the solve (its lane index is the wave's lane index).
+    // The copy-in is direct global->LDS DMA of 16 bytes per lane (interior
+    // cells start on 128-byte boundaries, tile lines on 16-byte ones), so
+    // every line of the tile is in flight at once with no register staging
+    // and a wave's DMAs stay under the count of operations it can track.
     const int step = T < 64 ? T : 64, nw = T / step, w = t / step, lane = t - w * step;
+    const int nu = N >> 1;  // 16-byte units per line; an odd N leaves 8 bytes
     for (int l = w < nw ? w : nl; l < nl; l += nw) {
         const int s = first + l;
         const char* src = (const char*)(rhs + cell(g, 1, s % g.ny + 1, s / g.ny + 1));
         char* dst = (char*)(tile + (size_t)l * LS);
-        for (int e = 0; e < 2 * N; e += step) {
-            if (e + lane < 2 * N)
-                __builtin_amdgcn_global_load_lds((const void*)(src + 4 * (e + lane)),
-                                                 (__attribute__((address_space(3))) void*)(dst + 4 * e),
-                                                 4, 0, 0);
+        for (int e = 0; e < nu; e += step) {
+            if (e + lane < nu)
+                __builtin_amdgcn_global_load_lds((const void*)(src + 16 * (e + lane)),
+                                                 (__attribute__((address_space(3))) void*)(dst + 16 * e),
+                                                 16, 0, 0);
         }
+        if ((N & 1) && lane < 2)
+            __builtin_amdgcn_global_load_lds((const void*)(src + 16 * nu + 4 * lane),
+                                             (__attribute__((address_space(3))) void*)(dst + 16 * nu),
+                                             4, 0, 0);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -632,7 +641,7 @@ void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
     while (S * Q > 256) S >>= 1;
     auto lds_of = [&](int S_) {
         const size_t red = f.thomas ? 0 : (size_t)3 * 2 * Q * S_;
-        const size_t tile = d == 0 ? (size_t)S_ * (N + ((N & 1) ? 2 : 1)) : 0;
+        const size_t tile = d == 0 ? (size_t)S_ * x_tile_stride(N) : 0;
         return sizeof(double) * (red + tile);
     };
     while (S > 1 && lds_of(S) > 160 * 1024) S >>= 1;
