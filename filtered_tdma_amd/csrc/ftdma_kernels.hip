@@ -515,17 +515,18 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
 // Each wave loads exactly the rows it consumes: the ring needs no barrier,
 // only the wave's own vmcnt.
 //
-// vmcnt bookkeeping (in-order completion of a wave's vector memory ops):
-// per row-step two DMA ops; per tile exactly M stores (rows past m_p store
-// the last row's value again, so the count does not depend on m_p). At the
-// wait for row j the ops issued after that row's DMAs are
-//   j >= R                : the DMAs of the next R-1 row-steps  -> 2(R-1)
-//   j <  R, first tile    : the same                           -> 2(R-1)
-//   j <  R, later tiles   : 2(R-1) DMAs and the M stores of the previous
-//                           tile                               -> 2(R-1)+M
-// (capped at 63, the counter's range: a smaller count only waits longer).
+// vmcnt bookkeeping. It relies only on a wave's loads completing in issue
+// order among themselves (the counter's decrements for loads are in order);
+// stores may complete in any order relative to them. Per row-step two DMA
+// ops. At the wait for row j the ops issued after that row's DMAs are
+//   j >= R, or the first tile : the DMAs of the next R-1 row-steps -> 2(R-1):
+//     while any of row j's is pending all of those are too, so vmcnt(2(R-1))
+//     means row j has landed, whatever stores are also in flight;
+//   j <  R, later tiles       : no wait. Those rows were requested during the
+//     previous tile's last R row-steps, and the previous tile waited for
+//     vmcnt(0) after its reduced solve, before its stores.
 // The last tile re-fetches its own rows into consumed slots so that the
-// count stays uniform; the kernel drains vmcnt before it ends.
+// pattern stays uniform; the kernel drains vmcnt before it ends.
 // ---------------------------------------------------------------------------
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -583,9 +584,8 @@ __device__ __forceinline__ void lds_barrier() {
 #define FTD_DMA_AUX 2  // nontemporal: the rows are read once
 #endif
 
-// The ring of one wave and its bookkeeping. NST: vector memory ops the tail
-// of a tile issues (all lanes together count once per instruction).
-template <int M, int Q, int R, int NST>
+// The ring of one wave and its bookkeeping.
+template <int M, int Q, int R>
 struct StreamRing {
     static constexpr int S = 32;
     double (*slots)[2][128];  // this wave's R slots
@@ -621,12 +621,8 @@ struct StreamRing {
     // (of this tile, else of tile `ntile`)
     __device__ __forceinline__ void take(int j, bool first, int tile, int ntile, int dcnt, int l,
                                          double& a, double& b, double& c, double& d) const {
-        constexpr int steady = 2 * (R - 1);
-        constexpr int after_tail = steady + NST < 63 ? steady + NST : 63;
-        if (j >= R || first)
-            wait_vmcnt<steady>();
-        else
-            wait_vmcnt<after_tail>();
+        // rows j < R of a later tile landed before the previous tile's stores
+        if (j >= R || first) wait_vmcnt<2 * (R - 1)>();
         ring_read((lds_cdptr_t)&slots[j % R][0][l], a, b, c, d);
         if (j + R < M)
             dma(tile, j + R, j % R, dcnt);
@@ -650,7 +646,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const int q = 2 * w + (l >> 5);
     const int sl = l & 31;
     const size_t ld = (size_t)n_sys;
-    StreamRing<M, Q, R, M> rg;
+    StreamRing<M, Q, R> rg;
     int dcnt0;
     rg.init(ring[w], A, B, C, D, n_sys, n_row, w, l, dcnt0);
     int r0, cnt0;
@@ -772,9 +768,12 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
         }
         lds_barrier();
+        // the next tile's first R rows have landed before this tile's stores
+        // (requested during the last R row-steps, so this costs no wait)
+        wait_vmcnt<0>();
         const double x0 = red[2][2 * q][sl];
         const double xN = red[2][2 * q + 1][sl];
-        // exactly M stores: rows past m_p store row m_p - 1 (x_N) again
+        // rows past m_p store row m_p - 1 (x_N) again
         double* const xlast = D + base + (size_t)(cnt - 1) * ld;
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -824,7 +823,7 @@ __device__ __forceinline__ void ring_read_f32(lds_cfptr_t p, double& a, double& 
     d = (double)cd.y;
 }
 
-template <int M, int Q, int R, int NST>
+template <int M, int Q, int R>
 struct StreamRingF32 {
     static constexpr int S = 32;
     float (*slots)[256];  // this wave's R slots
@@ -852,16 +851,11 @@ struct StreamRingF32 {
 #pragma unroll
         for (int j = 0; j < R; ++j) dma(tile, j, j, dcnt);
     }
-    // one DMA per row-step: R - 1 in flight after this row's, plus the M
-    // stores of the previous tile for the first R rows of a later tile
+    // one DMA per row-step: R - 1 in flight after this row's; rows j < R of
+    // a later tile landed before the previous tile's stores
     __device__ __forceinline__ void take(int j, bool first, int tile, int ntile, int dcnt, int l,
                                          double& a, double& b, double& c, double& d) const {
-        constexpr int steady = R - 1;
-        constexpr int after_tail = steady + NST < 63 ? steady + NST : 63;
-        if (j >= R || first)
-            wait_vmcnt<steady>();
-        else
-            wait_vmcnt<after_tail>();
+        if (j >= R || first) wait_vmcnt<R - 1>();
         ring_read_f32((lds_cfptr_t)&slots[j % R][l], a, b, c, d);
         if (j + R < M)
             dma(tile, j + R, j % R, dcnt);
@@ -885,7 +879,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const int q = 2 * w + (l >> 5);
     const int sl = l & 31;
     const size_t ld = (size_t)n_sys;
-    StreamRingF32<M, Q, R, M> rg;
+    StreamRingF32<M, Q, R> rg;
     int dcnt0;
     rg.init(ring[w], A, B, C, Din, n_sys, n_row, w, l, dcnt0);
     int r0, cnt0;
@@ -969,9 +963,11 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
         }
         lds_barrier();
+        // the next tile's first R rows have landed before this tile's stores
+        wait_vmcnt<0>();
         const double x0 = red[2][2 * q][sl];
         const double xN = red[2][2 * q + 1][sl];
-        // exactly M stores: rows past m_p store row m_p - 1 (x_N) again
+        // rows past m_p store row m_p - 1 (x_N) again
         double* const xlast = X + base + (size_t)(cnt - 1) * ld;
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -1275,9 +1271,6 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     double* __restrict__ xbuf, double* __restrict__ partial, double eps_value, int eps_direct,
     int* __restrict__ Jsys) {
     constexpr int S = 32, NW = Q / 2;
-    // vector memory ops of a tile's tail: PASS 1 stores x0, xN and a partial
-    // max, PASS 3 the M rows and the J record
-    constexpr int NST = PASS == 1 ? 3 : PASS == 3 ? M + 1 : M;
     static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
     __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
     __shared__ double ex[4][Q][S];  // C, D of row m-1; A, D of row 0 (packed)
@@ -1296,7 +1289,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // PASS 2: rows 4w..4w+3 of xbuf (x0, xN of partitions 2w, 2w+1), two
     // systems per lane, as the ring's sub-blocks
     const double* xsrc = xbuf + (size_t)(4 * w + (l >> 4)) * ld + 2 * (l & 15);
-    StreamRing<M, Q, R, NST> rg;
+    StreamRing<M, Q, R> rg;
     int dcnt0;
     rg.init(ring[w], A, B, C, D, n_sys, n_row, w, l, dcnt0);
 
@@ -1402,6 +1395,8 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (has_left) x0 = (d0p - a0p * ex[1][pl][sl]) / (1.0 - ex[0][pl][sl] * a0p);
             if (has_right) xN = (dN - cN * ex[3][pr][sl]) / (1.0 - ex[2][pr][sl] * cN);
         }
+        // the next tile's first R rows have landed before this tile's stores
+        wait_vmcnt<0>();
         if (PASS == 1) {
             xbuf[(size_t)(2 * p) * ld + i] = x0;
             xbuf[(size_t)(2 * p + 1) * ld + i] = xN;
