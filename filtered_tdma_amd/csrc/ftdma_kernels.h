@@ -71,7 +71,7 @@ static_assert(sizeof(BatchLane) == 16, "BatchLane is one dwordx4");
 
 // Plans the batch: lanes (128 per item; the n_chunked items of chunked
 // systems first, then single-chunk items, each range longest first), the
-// systems longer than 128 * 32 rows (lane-per-system kernel), and the last
+// systems longer than 128 * 48 rows (lane-per-system kernel), and the last
 // element any system holds (the staging never reads past it). Returns the
 // number of items, or -1 when some n_rows[s] < 1.
 int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,

@@ -188,9 +188,9 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
  * share one n_row): n_systems independent systems, system s occupying
  * elements row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of each of A, B,
  * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
- * a system of <= 32 rows is plain Thomas, a longer one PaScaL with
- * ceil(n_rows/32) partitions inside one 128-lane workgroup; all in one
- * persistent launch (systems above 4096 rows, and a batch that is one 1-row
+ * a system of <= 48 rows is plain Thomas, a longer one PaScaL with
+ * ceil(n_rows/48) partitions inside one 128-lane workgroup; all in one
+ * persistent launch (systems above 6144 rows, and a batch that is one 1-row
  * system at offset 0, in a second, lane-per-system launch that uses C as
  * scratch). D is overwritten with x; A and B are read only. No element past
  * the last row of the highest system is read, so the arrays may end there.

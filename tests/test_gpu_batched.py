@@ -1,8 +1,8 @@
 """Uneven batches (ftd_solve_batched) against the oracle, system by system.
 
-A system of N <= 32 rows is plain Thomas in one lane, a longer one (up to
-4096 rows) PaScaL with Q = ceil(N/32) partitions inside one 128-lane work
-item, a longer one still the lane-per-system Thomas kernel. Each is compared bit for bit
+A system of N <= 48 rows is plain Thomas in one lane, a longer one (up to
+128 * 48 = 6144 rows) PaScaL with Q = ceil(N/48) partitions inside one
+128-lane work item, a longer one still the lane-per-system Thomas kernel. Each is compared bit for bit
 with the oracle's solve of the same partitioning (GPU semantics), and with
 serial Thomas within 1e-13 relative."""
 import sys
@@ -42,6 +42,9 @@ def general_uneven(n_rows, seed, gap):
     return off, n_rows, A, B, C, D
 
 
+CHUNK = 48  # rows per lane of the batched kernel
+
+
 def oracle_solve(off, n_rows, A, B, C, D):
     """x of every system by the oracle, with the engine's partitioning."""
     X = D.copy()
@@ -49,7 +52,7 @@ def oracle_solve(off, n_rows, A, B, C, D):
         o, n = int(off[s]), int(n_rows[s])
         sl = slice(o, o + n)
         a, b, c, d = (v[sl].reshape(n, 1).copy() for v in (A, B, C, D))
-        Q = (n + 31) // 32
+        Q = (n + CHUNK - 1) // CHUNK
         if Q == 1 or Q > 128:
             x = O.thomas(a, b, c, d, sem=O.SEM_GPU)
         else:
@@ -74,8 +77,8 @@ def run_batched(be, off, n_rows, A, B, C, D):
     return tA.cpu().numpy(), tB.cpu().numpy(), tD.cpu().numpy(), be.last_path()
 
 
-EDGE = [1, 2, 3, 4, 16, 31, 32, 33, 34, 63, 64, 65, 66, 127, 128, 129, 191, 1000, 2048, 2049,
-        4095, 4096, 4097, 6000]
+EDGE = [1, 2, 3, 4, 16, 31, 32, 33, 34, 47, 48, 49, 63, 64, 65, 66, 95, 96, 97, 127, 128, 129,
+        191, 1000, 2048, 2049, 4095, 4096, 4097, 6000, 6143, 6144, 6145, 7000]
 
 
 def test_batched_uneven_bitwise():
@@ -135,8 +138,8 @@ def test_batched_many_items_per_workgroup():
 @pytest.mark.parametrize("lengths,gap", [
     ([40, 7, 1], 1),      # a 1-row system last
     ([300, 31], 1),       # an odd single-chunk system last
-    ([70, 5, 2049], 1),   # chunked, last chunk of 31 rows (odd)
-    ([33], 0),            # one chunked system, chunks 17 and 16
+    ([70, 5, 2049], 1),   # chunked, chunks of 48 and 47 rows, the last odd
+    ([99], 0),            # one chunked system, three chunks of 33 rows
     ([1], 1),             # one 1-row system at offset 1
     ([1], 0),             # one 1-row system at offset 0 (lane-per-system kernel)
     ([2, 1], 0),          # two short systems back to back, 1-row last
@@ -158,7 +161,7 @@ def test_batched_arrays_end_at_last_row(lengths, gap):
 
 def test_batched_reduced_denominator_overflow():
     """A reduced-system denominator that overflows to -inf. System 0 (200
-    rows, 7 partitions of 29/28 rows) has a zero right-hand side, c = 1e200
+    rows, 5 partitions of 40 rows) has a zero right-hand side, c = 1e200
     on partition 0's last row and a = 1e200, c = 0 on partition 1's first:
     the reduced rows' product overflows, IEEE division gives inv = -0 and x
     stays exactly zero. A reciprocal that is not the division's (v_rcp_f64
@@ -166,7 +169,7 @@ def test_batched_reduced_denominator_overflow():
     off, n_rows, A, B, C, D = general_uneven([200, 300, 40, 5], seed=77, gap=1)
     o = int(off[0])
     D[o:o + 200] = 0.0
-    C[o + 28], A[o + 29], C[o + 29] = 1e200, 1e200, 0.0
+    C[o + 39], A[o + 40], C[o + 40] = 1e200, 1e200, 0.0
     note("batched reduced-solve division fallback")
     be = TdmaBackendGPU("pascal", 1, 1, 1)
     try:
