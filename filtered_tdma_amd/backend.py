@@ -142,11 +142,9 @@ class TdmaBackendGPU:
         _lib.check(_lib.load().ftd_last_J(self._h, out, n))
         return list(out)
 
-    def solve_batched(self, A, B, C, D, row_offsets, n_rows) -> None:
-        """Uneven batch (no reference counterpart): system s is elements
-        row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of the 1-D float64
-        device arrays A, B, C, D; D <- x. row_offsets (int64) and n_rows
-        (int32) are host arrays."""
+    @staticmethod
+    def _layout(row_offsets, n_rows, arrays, names):
+        """Host layout arrays, checked against the device arrays' lengths."""
         import numpy as np
 
         off = np.ascontiguousarray(row_offsets, dtype=np.int64)
@@ -155,13 +153,28 @@ class TdmaBackendGPU:
             raise ValueError("row_offsets and n_rows must be 1-D of one length")
         if off.size:
             need = int(np.max(off + rows))
-            for name, x in zip("ABCD", (A, B, C, D)):
+            for name, x in zip(names, arrays):
                 if hasattr(x, "numel") and x.numel() < need:
                     raise ValueError(f"{name} holds {x.numel()} elements; the layout reaches {need}")
+        return (off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(off.size), off, rows)
+
+    def solve_batched(self, A, B, C, D, row_offsets, n_rows) -> None:
+        """Uneven batch (no reference counterpart): system s is elements
+        row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of the 1-D float64
+        device arrays A, B, C, D; D <- x. row_offsets (int64) and n_rows
+        (int32) are host arrays."""
+        po, pr, n, _, _ = self._layout(row_offsets, n_rows, (A, B, C, D), "ABCD")
         ptrs = [_ptr(x) for x in (A, B, C, D)]
-        _lib.check(_lib.load().ftd_solve_batched(
-            self._h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-            rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(off.size), *ptrs))
+        _lib.check(_lib.load().ftd_solve_batched(self._h, po, pr, n, *ptrs))
+
+    def solve_batched_f32(self, A, B, C, D, X, row_offsets, n_rows) -> None:
+        """The uneven batch with FP32 storage: A, B, C, D float32, the
+        elimination in FP64 on their exactly widened values, x into the
+        float64 array X at the same offsets (systems of up to 6144 rows)."""
+        po, pr, n, _, _ = self._layout(row_offsets, n_rows, (A, B, C, D, X), "ABCDX")
+        ptrs = [_ptr(x, dtype="float32") for x in (A, B, C, D)] + [_ptr(X)]
+        _lib.check(_lib.load().ftd_solve_batched_f32(self._h, po, pr, n, *ptrs))
 
     def set_rho_device(self, A, B, C) -> None:
         a, b, c, _ = self._args(A, B, C, A)

@@ -767,48 +767,58 @@ ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const fl
     return solve_f32_common(h, A, B, C, D, X, 1);
 }
 
+namespace {
+
+// the plan of this layout on the handle (made and uploaded once per layout)
+ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                        int n_systems) {
+    if (!row_offsets || !n_rows) return fail(FTD_ERR_INVALID, "null layout array");
+    const size_t n = (size_t)n_systems;
+    const bool same = h->b_valid && h->b_off.size() == n &&
+                      std::memcmp(h->b_off.data(), row_offsets, n * sizeof(int64_t)) == 0 &&
+                      std::memcmp(h->b_rows.data(), n_rows, n * sizeof(int)) == 0;
+    if (same) return FTD_OK;
+    for (size_t s = 0; s < n; ++s) {
+        if (n_rows[s] < 1) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
+        if (row_offsets[s] < 0) return fail(FTD_ERR_INVALID, "row offsets must be >= 0");
+    }
+    // the previous plan may still be in use by work queued on the stream
+    if (h->b_valid) FTD_HIP(hipStreamSynchronize(h->stream));
+    h->b_valid = 0;
+    h->b_items = ftd::batch_plan(n_systems, row_offsets, n_rows, h->b_lanes, h->b_long_off,
+                                 h->b_long_n, h->b_chunked, h->b_last);
+    if (h->b_items < 0) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
+    const size_t nl = h->b_long_n.size();
+    FTD_TRY(ensure(h->b_dlanes, sizeof(ftd::BatchLane) * std::max<size_t>(1, h->b_lanes.size())));
+    FTD_TRY(ensure(h->b_dlong_off, sizeof(int64_t) * std::max<size_t>(1, nl)));
+    FTD_TRY(ensure(h->b_dlong_n, sizeof(int) * std::max<size_t>(1, nl)));
+    if (!h->b_lanes.empty())
+        FTD_HIP(hipMemcpyAsync(h->b_dlanes.p, h->b_lanes.data(),
+                               sizeof(ftd::BatchLane) * h->b_lanes.size(),
+                               hipMemcpyHostToDevice, h->stream));
+    if (nl) {
+        FTD_HIP(hipMemcpyAsync(h->b_dlong_off.p, h->b_long_off.data(), sizeof(int64_t) * nl,
+                               hipMemcpyHostToDevice, h->stream));
+        FTD_HIP(hipMemcpyAsync(h->b_dlong_n.p, h->b_long_n.data(), sizeof(int) * nl,
+                               hipMemcpyHostToDevice, h->stream));
+    }
+    // the host copies must outlive the asynchronous uploads
+    FTD_HIP(hipStreamSynchronize(h->stream));
+    h->b_off.assign(row_offsets, row_offsets + n);
+    h->b_rows.assign(n_rows, n_rows + n);
+    h->b_valid = 1;
+    return FTD_OK;
+}
+
+}  // namespace
+
 ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
                              int n_systems, double* A, double* B, double* C, double* D) {
     if (!h) return fail(FTD_ERR_INVALID, "null handle");
     if (n_systems < 0) return fail(FTD_ERR_INVALID, "n_systems must be >= 0");
     if (n_systems == 0) return FTD_OK;
-    if (!row_offsets || !n_rows) return fail(FTD_ERR_INVALID, "null layout array");
     if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
-    const size_t n = (size_t)n_systems;
-    const bool same = h->b_valid && h->b_off.size() == n &&
-                      std::memcmp(h->b_off.data(), row_offsets, n * sizeof(int64_t)) == 0 &&
-                      std::memcmp(h->b_rows.data(), n_rows, n * sizeof(int)) == 0;
-    if (!same) {
-        for (size_t s = 0; s < n; ++s) {
-            if (n_rows[s] < 1) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
-            if (row_offsets[s] < 0) return fail(FTD_ERR_INVALID, "row offsets must be >= 0");
-        }
-        // the previous plan may still be in use by work queued on the stream
-        if (h->b_valid) FTD_HIP(hipStreamSynchronize(h->stream));
-        h->b_valid = 0;
-        h->b_items = ftd::batch_plan(n_systems, row_offsets, n_rows, h->b_lanes, h->b_long_off,
-                                     h->b_long_n, h->b_chunked, h->b_last);
-        if (h->b_items < 0) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
-        const size_t nl = h->b_long_n.size();
-        FTD_TRY(ensure(h->b_dlanes, sizeof(ftd::BatchLane) * std::max<size_t>(1, h->b_lanes.size())));
-        FTD_TRY(ensure(h->b_dlong_off, sizeof(int64_t) * std::max<size_t>(1, nl)));
-        FTD_TRY(ensure(h->b_dlong_n, sizeof(int) * std::max<size_t>(1, nl)));
-        if (!h->b_lanes.empty())
-            FTD_HIP(hipMemcpyAsync(h->b_dlanes.p, h->b_lanes.data(),
-                                   sizeof(ftd::BatchLane) * h->b_lanes.size(),
-                                   hipMemcpyHostToDevice, h->stream));
-        if (nl) {
-            FTD_HIP(hipMemcpyAsync(h->b_dlong_off.p, h->b_long_off.data(), sizeof(int64_t) * nl,
-                                   hipMemcpyHostToDevice, h->stream));
-            FTD_HIP(hipMemcpyAsync(h->b_dlong_n.p, h->b_long_n.data(), sizeof(int) * nl,
-                                   hipMemcpyHostToDevice, h->stream));
-        }
-        // the host copies must outlive the asynchronous uploads
-        FTD_HIP(hipStreamSynchronize(h->stream));
-        h->b_off.assign(row_offsets, row_offsets + n);
-        h->b_rows.assign(n_rows, n_rows + n);
-        h->b_valid = 1;
-    }
+    FTD_TRY(batch_layout(h, row_offsets, n_rows, n_systems));
     const int nl = (int)h->b_long_n.size();
     std::snprintf(h->path, sizeof h->path, "batched items=%d chunked=%d long=%d", h->b_items,
                   h->b_chunked, nl);
@@ -817,6 +827,34 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
                                 h->b_items, h->b_last,
                                 (const int64_t*)h->b_dlong_off.p, (const int*)h->b_dlong_n.p, nl,
                                 h->stream));
+    if (h->profiling) {
+        FTD_HIP(hipEventRecord(h->ev1, h->stream));
+        FTD_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0.0f;
+        FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+ftd_status ftd_solve_batched_f32(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                                 int n_systems, const float* A, const float* B, const float* C,
+                                 const float* D, double* X) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (n_systems < 0) return fail(FTD_ERR_INVALID, "n_systems must be >= 0");
+    if (n_systems == 0) return FTD_OK;
+    if (!A || !B || !C || !D || !X) return fail(FTD_ERR_INVALID, "null array pointer");
+    FTD_TRY(batch_layout(h, row_offsets, n_rows, n_systems));
+    if (!h->b_long_n.empty())
+        return fail(FTD_ERR_UNSUPPORTED, "FP32 batches take systems of up to " +
+                                             std::to_string(ftd::batched_max_rows()) + " rows");
+    if (h->b_last < 3)
+        return fail(FTD_ERR_UNSUPPORTED, "FP32 batches must span at least 4 elements");
+    std::snprintf(h->path, sizeof h->path, "batched_f32 items=%d chunked=%d", h->b_items,
+                  h->b_chunked);
+    if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
+    FTD_HIP(ftd::launch_batched_f32(A, B, C, D, X, (const ftd::BatchLane*)h->b_dlanes.p,
+                                    h->b_chunked, h->b_items, h->b_last, h->stream));
     if (h->profiling) {
         FTD_HIP(hipEventRecord(h->ev1, h->stream));
         FTD_HIP(hipEventSynchronize(h->ev1));

@@ -97,6 +97,21 @@ constexpr int kBUF = 4 * kARR;   // bytes per buffer
 constexpr int kDMA = 16;         // DMAs per wave and block
 static_assert(kBUF + 3 * 256 + 8 + 3 * kARR < 65536, "ds_read immediate offsets");
 
+// FP32 storage: the same blocks of 8 rows, 16 bytes per lane being four rows
+// of one chunk. DMA j of array X (j = 0, 1) lands at
+//   stage + X * ARR4 + j * SEG  (32 chunks x 32 bytes + 16 of padding);
+// its lane i = 32 k + cc brings rows 4k .. 4k + 3 of chunk 32 j + cc, so row
+// r of chunk c sits at (c >> 5) * SEG + (c & 31) * 16 + (r >> 2) * 512 +
+// (r & 3) * 4. 8 DMAs per block. A quad that would run past `last` is read
+// ending at `last` (up to three elements lower), and its lane moves its rows
+// back into place once the block has landed.
+constexpr int kARR4 = 2 * kSEG;   // bytes per array per buffer
+constexpr int kBUF4 = 4 * kARR4;  // bytes per buffer
+constexpr int kDMA4 = 8;          // DMAs per wave and block
+
+template <typename TS>
+constexpr int buf_bytes() { return sizeof(TS) == 8 ? kBUF : kBUF4; }
+
 typedef __attribute__((address_space(3))) void* lds_vptr;
 typedef const __attribute__((address_space(3))) double* lds_cdptr;
 typedef int ftd_int4 __attribute__((ext_vector_type(4)));
@@ -106,16 +121,33 @@ struct Stage {
     const BatchLane* desc;  // this wave's 64 lane descriptors (LDS)
     lds_cdptr lane;         // this lane's chunk slot in buffer 0
     int64_t last;           // the batch's last element
-    int fix_blk;            // block of this chunk's pair read one element low (-1: none)
-    int fix_off;            // byte offset of that pair from the lane's slot
+    int fix_blk;            // block of this chunk's pair / quad read low (-1: none)
+    int fix_off;            // byte offset of that pair / quad from the lane's slot
+    int fix_shift;          // elements it was read low by (FP32: 1..3)
 };
 
+// The chunk's last pair (FP64) or quad (FP32) is read low when it would run
+// past `last`: then its rows sit `fix_shift` elements further on in the slot.
+template <typename TS>
 __device__ __forceinline__ void stage_set_fix(Stage& st, const BatchLane& L) {
     st.fix_blk = -1;
     st.fix_off = 0;
-    if ((L.m & 1) && L.off + L.m - 1 == st.last) {
-        st.fix_blk = (L.m - 1) >> 3;
-        st.fix_off = (((L.m - 1) & 7) >> 1) * 256;
+    st.fix_shift = 0;
+    if (L.m < 1) return;
+    const int r = L.m - 1;  // the chunk's last row
+    if (sizeof(TS) == 8) {
+        if (!(r & 1) && L.off + r == st.last) {
+            st.fix_blk = r >> 3;
+            st.fix_off = ((r & 7) >> 1) * 256;
+            st.fix_shift = 1;
+        }
+    } else {
+        const int64_t e0 = L.off + (r & ~3);  // first element of its last quad
+        if (e0 + 3 > st.last) {
+            st.fix_blk = r >> 3;
+            st.fix_off = ((r >> 2) & 1) * 512;
+            st.fix_shift = (int)(e0 + 3 - st.last);
+        }
     }
 }
 
@@ -151,6 +183,78 @@ __device__ __forceinline__ void stage_issue(const Stage& st, int t, int b, int l
         __builtin_amdgcn_global_load_lds((const void*)(D + e), (lds_vptr)(dst + 3 * kARR), 16, 0,
                                          kDmaAux);
     }
+}
+
+// FP32: DMA of block b into buffer t, four rows per lane
+__device__ __forceinline__ void stage_issue(const Stage& st, int t, int b, int l, const float* A,
+                                            const float* B, const float* C, const float* D) {
+    const int k = l >> 5;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        // descriptor of chunk 32j + l%32, read in place
+        ftd_int4 cd;
+        asm volatile(
+            "ds_read_b128 %0, %1\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(cd)
+            : "v"((const __attribute__((address_space(3))) BatchLane*)(st.desc + 32 * j + (l & 31)))
+            : "memory");
+        const int64_t off = (int64_t)(((uint64_t)(uint32_t)cd.y << 32) | (uint32_t)cd.x);
+        const int mm = cd.z > 0 ? cd.z : 1;
+        const int row = kRB * b + 4 * k < mm ? kRB * b + 4 * k : mm - 1;
+        int64_t e = off + row;
+        if (e > st.last - 3) e = st.last - 3;
+        unsigned char* dst = st.buf0 + t * kBUF4 + j * kSEG;
+        __builtin_amdgcn_global_load_lds((const void*)(A + e), (lds_vptr)(dst + 0 * kARR4), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(B + e), (lds_vptr)(dst + 1 * kARR4), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(C + e), (lds_vptr)(dst + 2 * kARR4), 16, 0,
+                                         kDmaAux);
+        __builtin_amdgcn_global_load_lds((const void*)(D + e), (lds_vptr)(dst + 3 * kARR4), 16, 0,
+                                         kDmaAux);
+    }
+}
+
+// FP32: the quad read fix_shift elements low: element p + fix_shift of the
+// slot goes to p, for every array
+template <int T>
+__device__ __forceinline__ void stage_fix_f32(const Stage& st) {
+    typedef const __attribute__((address_space(3))) unsigned char* lds_cbptr;
+    const lds_cbptr p = (lds_cbptr)st.lane + st.fix_off;
+    ftd_int4 v0, v1, v2, v3;
+    asm volatile(
+        "ds_read_b128 %0, %4 offset:%5\n\t"
+        "ds_read_b128 %1, %4 offset:%6\n\t"
+        "ds_read_b128 %2, %4 offset:%7\n\t"
+        "ds_read_b128 %3, %4 offset:%8\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+        : "v"(p), "n"(T * kBUF4), "n"(T * kBUF4 + kARR4), "n"(T * kBUF4 + 2 * kARR4),
+          "n"(T * kBUF4 + 3 * kARR4)
+        : "memory");
+    const int s = st.fix_shift;
+    auto shift = [s](ftd_int4 v) {
+        ftd_int4 o = v;
+        o.x = s == 1 ? v.y : s == 2 ? v.z : v.w;
+        o.y = s == 1 ? v.z : v.w;
+        o.z = v.w;
+        return o;
+    };
+    v0 = shift(v0);
+    v1 = shift(v1);
+    v2 = shift(v2);
+    v3 = shift(v3);
+    asm volatile(
+        "ds_write_b128 %0, %1 offset:%5\n\t"
+        "ds_write_b128 %0, %2 offset:%6\n\t"
+        "ds_write_b128 %0, %3 offset:%7\n\t"
+        "ds_write_b128 %0, %4 offset:%8\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        :
+        : "v"(p), "v"(v0), "v"(v1), "v"(v2), "v"(v3), "n"(T * kBUF4), "n"(T * kBUF4 + kARR4),
+          "n"(T * kBUF4 + 2 * kARR4), "n"(T * kBUF4 + 3 * kARR4)
+        : "memory");
 }
 
 // the pair read one element low: its second row is this chunk's last
@@ -201,9 +305,40 @@ __device__ __forceinline__ void stage_read(lds_cdptr lane, double& a, double& b,
     stage_read_at<T * kBUF + (R >> 1) * 256 + (R & 1) * 8>(lane, a, b, c, d);
 }
 
+// FP32: the four values of row R, widened exactly to FP64
+template <int T, int R>
+__device__ __forceinline__ void stage_read_f32(lds_cdptr lane, double& a, double& b, double& c,
+                                               double& d) {
+    constexpr int OFF = T * kBUF4 + (R >> 2) * 512 + (R & 3) * 4;
+    float fa, fb, fc, fd;
+    asm volatile(
+        "ds_read_b32 %0, %4 offset:%5\n\t"
+        "ds_read_b32 %1, %4 offset:%6\n\t"
+        "ds_read_b32 %2, %4 offset:%7\n\t"
+        "ds_read_b32 %3, %4 offset:%8\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(fa), "=&v"(fb), "=&v"(fc), "=&v"(fd)
+        : "v"(lane), "n"(OFF), "n"(OFF + kARR4), "n"(OFF + 2 * kARR4), "n"(OFF + 3 * kARR4)
+        : "memory");
+    a = (double)fa;
+    b = (double)fb;
+    c = (double)fc;
+    d = (double)fd;
+}
+
+template <typename TS, int T, int R>
+__device__ __forceinline__ void stage_row(lds_cdptr lane, double& a, double& b, double& c,
+                                          double& d) {
+    if constexpr (sizeof(TS) == 8)
+        stage_read<T, R>(lane, a, b, c, d);
+    else
+        stage_read_f32<T, R>(lane, a, b, c, d);
+}
+
+template <typename TS>
 __device__ __forceinline__ void vm_wait_block(bool more) {
     if (more)
-        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kDMA) : "memory");
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(sizeof(TS) == 8 ? kDMA : kDMA4) : "memory");
     else
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
@@ -220,39 +355,44 @@ __device__ __forceinline__ int wave_max(int v) {
 // row index, buffer and LDS offset is a compile-time constant. Blocks 0 and 1
 // have landed before the sweep starts (the previous item waited for them
 // before its stores); block B >= 2 was issued once block B - 2 was read.
-template <int B, int NB>
+template <typename TS, int B, int NB>
 struct Sweep {
     template <typename F>
-    __device__ __forceinline__ static void run(const Stage& st, int l, int nblk, const double* A,
-                                               const double* Bm, const double* C,
-                                               const double* D, F& step) {
+    __device__ __forceinline__ static void run(const Stage& st, int l, int nblk, const TS* A,
+                                               const TS* Bm, const TS* C, const TS* D, F& step) {
         if (B < nblk) {
-            if (B >= 2) vm_wait_block(B + 1 < nblk);
-            if (st.fix_blk == B) stage_fix<B & 1>(st);
+            if (B >= 2) vm_wait_block<TS>(B + 1 < nblk);
+            if (st.fix_blk == B) {
+                if constexpr (sizeof(TS) == 8)
+                    stage_fix<B & 1>(st);
+                else
+                    stage_fix_f32<B & 1>(st);
+            }
             double a, b, c, d;
-            stage_read<B & 1, 0>(st.lane, a, b, c, d); step(B * kRB + 0, a, b, c, d);
-            stage_read<B & 1, 1>(st.lane, a, b, c, d); step(B * kRB + 1, a, b, c, d);
-            stage_read<B & 1, 2>(st.lane, a, b, c, d); step(B * kRB + 2, a, b, c, d);
-            stage_read<B & 1, 3>(st.lane, a, b, c, d); step(B * kRB + 3, a, b, c, d);
-            stage_read<B & 1, 4>(st.lane, a, b, c, d); step(B * kRB + 4, a, b, c, d);
-            stage_read<B & 1, 5>(st.lane, a, b, c, d); step(B * kRB + 5, a, b, c, d);
-            stage_read<B & 1, 6>(st.lane, a, b, c, d); step(B * kRB + 6, a, b, c, d);
-            stage_read<B & 1, 7>(st.lane, a, b, c, d); step(B * kRB + 7, a, b, c, d);
+            stage_row<TS, B & 1, 0>(st.lane, a, b, c, d); step(B * kRB + 0, a, b, c, d);
+            stage_row<TS, B & 1, 1>(st.lane, a, b, c, d); step(B * kRB + 1, a, b, c, d);
+            stage_row<TS, B & 1, 2>(st.lane, a, b, c, d); step(B * kRB + 2, a, b, c, d);
+            stage_row<TS, B & 1, 3>(st.lane, a, b, c, d); step(B * kRB + 3, a, b, c, d);
+            stage_row<TS, B & 1, 4>(st.lane, a, b, c, d); step(B * kRB + 4, a, b, c, d);
+            stage_row<TS, B & 1, 5>(st.lane, a, b, c, d); step(B * kRB + 5, a, b, c, d);
+            stage_row<TS, B & 1, 6>(st.lane, a, b, c, d); step(B * kRB + 6, a, b, c, d);
+            stage_row<TS, B & 1, 7>(st.lane, a, b, c, d); step(B * kRB + 7, a, b, c, d);
             if (B + 2 < nblk) stage_issue(st, B & 1, B + 2, l, A, Bm, C, D);
-            Sweep<B + 1, NB>::run(st, l, nblk, A, Bm, C, D, step);
+            Sweep<TS, B + 1, NB>::run(st, l, nblk, A, Bm, C, D, step);
         }
     }
 };
-template <int NB>
-struct Sweep<NB, NB> {
+template <typename TS, int NB>
+struct Sweep<TS, NB, NB> {
     template <typename F>
-    __device__ __forceinline__ static void run(const Stage&, int, int, const double*,
-                                               const double*, const double*, const double*, F&) {}
+    __device__ __forceinline__ static void run(const Stage&, int, int, const TS*, const TS*,
+                                               const TS*, const TS*, F&) {}
 };
 
 // blocks 0 and 1 of an item; the caller issues them one item ahead
-__device__ __forceinline__ void stage_lead(const Stage& st, int l, int nblk, const double* A,
-                                          const double* B, const double* C, const double* D) {
+template <typename TS>
+__device__ __forceinline__ void stage_lead(const Stage& st, int l, int nblk, const TS* A,
+                                          const TS* B, const TS* C, const TS* D) {
     stage_issue(st, 0, 0, l, A, B, C, D);
     if (nblk > 1) stage_issue(st, 1, 1, l, A, B, C, D);
 }
@@ -278,16 +418,15 @@ struct ThomasStep {
 
 // ahead(): issues the next item's leading blocks once this item's staged
 // rows are all read
-template <int M, typename H>
-__device__ __forceinline__ void thomas_item(const Stage& st, const double* __restrict__ A,
-                                            const double* __restrict__ B,
-                                            const double* __restrict__ C, double* __restrict__ D,
-                                            const BatchLane& L, int l, int nblk, H& ahead) {
+template <int M, typename TS, typename H>
+__device__ __forceinline__ void thomas_item(const Stage& st, const TS* A, const TS* B, const TS* C,
+                                            const TS* D, double* X, const BatchLane& L, int l,
+                                            int nblk, H& ahead) {
     ThomasStep<M> s;
     s.cnt = L.m;
     asm volatile("" : "+v"(s.cnt));  // keeps the row tests inside the item loop
     const size_t base = (size_t)L.off;
-    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
+    Sweep<TS, 0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
     ahead();
 #pragma unroll
     for (int j = M - 2; j >= 0; --j)
@@ -296,9 +435,9 @@ __device__ __forceinline__ void thomas_item(const Stage& st, const double* __res
 #pragma unroll
     for (int j = 0; j < M; j += 2) {
         if (j + 1 < s.cnt)
-            store_pair(D + base + (size_t)j, s.d[j], s.d[j + 1]);
+            store_pair(X + base + (size_t)j, s.d[j], s.d[j + 1]);
         else if (j < s.cnt)
-            FTD_BST(s.d[j], D + base + (size_t)j);
+            FTD_BST(s.d[j], X + base + (size_t)j);
     }
 }
 
@@ -331,19 +470,18 @@ struct PascalStep {
 
 // t: lane within the item (0..127); the system's reduced rows are solved in
 // the item's LDS by its chunk-0 lane, then every chunk updates its interior
-template <int M, typename H>
-__device__ __forceinline__ void pascal_item(const Stage& st, const double* __restrict__ A,
-                                            const double* __restrict__ B,
-                                            const double* __restrict__ C, double* __restrict__ D,
-                                            const BatchLane& L, int l, int t, int nblk, double* rA,
-                                            double* rC, double* rD, H& ahead) {
+template <int M, typename TS, typename H>
+__device__ __forceinline__ void pascal_item(const Stage& st, const TS* A, const TS* B, const TS* C,
+                                            const TS* D, double* X, const BatchLane& L, int l,
+                                            int t, int nblk, double* rA, double* rC, double* rD,
+                                            H& ahead) {
     PascalStep<M> s;
     s.aN = s.cN = s.dN = 0.0;
     s.cnt = L.m;
     asm volatile("" : "+v"(s.cnt));
     const size_t base = (size_t)L.off;
     const int q = L.q, Q = L.Q, lead = L.lead;
-    Sweep<0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
+    Sweep<TS, 0, M / kRB>::run(st, l, nblk, A, B, C, D, s);
     ahead();
     const int cnt = s.cnt;
 #pragma unroll
@@ -429,9 +567,9 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
 #pragma unroll
         for (int j = 0; j < M; j += 2) {
             if (j + 1 < cnt)
-                store_pair(D + base + (size_t)j, x_at(j), x_at(j + 1));
+                store_pair(X + base + (size_t)j, x_at(j), x_at(j + 1));
             else if (j < cnt)
-                FTD_BST(x_at(j), D + base + (size_t)j);
+                FTD_BST(x_at(j), X + base + (size_t)j);
         }
     }
     group_lds_sync();  // the reduced rows are read before the next item writes them
@@ -446,12 +584,11 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const double* __res
 // two blocks. The wait for block B >= 2 is then vmcnt(16): the staging DMAs
 // complete in issue order, so while any of block B's is pending all 16 of
 // block B + 1's are too, whatever the stores issued before them do.
-template <int M>
+template <int M, typename TS>
 __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) void k_batched(
-    const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
-    double* __restrict__ D, const BatchLane* __restrict__ lanes, int n_chunked, int n_items,
-    int64_t last) {
-    __shared__ __attribute__((aligned(16))) unsigned char stage[2][2 * kBUF];
+    const TS* A, const TS* B, const TS* C, const TS* D, double* X,
+    const BatchLane* __restrict__ lanes, int n_chunked, int n_items, int64_t last) {
+    __shared__ __attribute__((aligned(16))) unsigned char stage[2][2 * buf_bytes<TS>()];
     __shared__ __attribute__((aligned(16))) BatchLane desc[2][kLanes];
     __shared__ double red[3][2 * kLanes];
     const int t = threadIdx.x;
@@ -466,7 +603,8 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) voi
     Stage st;
     st.buf0 = stage[w];
     st.desc = desc[0] + 64 * w;
-    st.lane = (lds_cdptr)(stage[w] + (l >> 4) * kSEG + (l & 15) * 16);
+    st.lane = (lds_cdptr)(stage[w] + (sizeof(TS) == 8 ? (l >> 4) * kSEG + (l & 15) * 16
+                                                      : (l >> 5) * kSEG + (l & 31) * 16));
     st.last = last;
     // each wave writes and reads only its own 64 descriptors
     BatchLane L = lanes[(size_t)item * kLanes + t];
@@ -477,7 +615,7 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) voi
     vm_drain();
     for (int u = 0;; u ^= 1) {
         st.desc = desc[u] + 64 * w;
-        stage_set_fix(st, L);
+        stage_set_fix<TS>(st, L);
         int nxt = item + G;
         if (item < n_chunked && nxt >= n_chunked) nxt = start1;
         const bool more = nxt < n_items;
@@ -499,9 +637,9 @@ __global__ __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(1))) voi
             }
         };
         if (item < n_chunked)
-            pascal_item<M>(st, A, B, C, D, L, l, t, nblk, red[0], red[1], red[2], ahead);
+            pascal_item<M>(st, A, B, C, D, X, L, l, t, nblk, red[0], red[1], red[2], ahead);
         else
-            thomas_item<M>(st, A, B, C, D, L, l, nblk, ahead);
+            thomas_item<M>(st, A, B, C, D, X, L, l, nblk, ahead);
         if (!more) break;
         item = nxt;
         L = Ln;
@@ -660,13 +798,14 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
     return (int)items.size();
 }
 
+template <typename TS>
 int batched_resident_blocks() {
     static int cached[64] = {};
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
     if (cached[dev]) return cached[dev];
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batched<kChunk>,
-                                                     kLanes, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void*)k_batched<kChunk, TS>, kLanes, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -681,9 +820,9 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st) {
     if (n_items > 0) {
-        const int blocks = std::min(batched_resident_blocks(), n_items);
-        hipLaunchKernelGGL((k_batched<kChunk>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D,
-                           lanes, n_chunked, n_items, last);
+        const int blocks = std::min(batched_resident_blocks<double>(), n_items);
+        hipLaunchKernelGGL((k_batched<kChunk, double>), dim3(blocks), dim3(kLanes), 0, st, A, B,
+                           (const double*)C, (const double*)D, D, lanes, n_chunked, n_items, last);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -694,5 +833,17 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
     }
     return hipSuccess;
 }
+
+hipError_t launch_batched_f32(const float* A, const float* B, const float* C, const float* D,
+                              double* X, const BatchLane* lanes, int n_chunked, int n_items,
+                              int64_t last, hipStream_t st) {
+    if (n_items < 1) return hipSuccess;
+    const int blocks = std::min(batched_resident_blocks<float>(), n_items);
+    hipLaunchKernelGGL((k_batched<kChunk, float>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D,
+                       X, lanes, n_chunked, n_items, last);
+    return hipGetLastError();
+}
+
+int batched_max_rows() { return kLanes * kChunk; }
 
 }  // namespace ftd

@@ -81,6 +81,14 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
                           const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st);
+// FP32 storage: A, B, C, D float, elimination in FP64 on the widened values,
+// x (FP64) into X at the same offsets. No long-system list: the caller
+// rejects systems above batched_max_rows() and batches spanning fewer than 4
+// elements.
+hipError_t launch_batched_f32(const float* A, const float* B, const float* C, const float* D,
+                              double* X, const BatchLane* lanes, int n_chunked, int n_items,
+                              int64_t last, hipStream_t st);
+int batched_max_rows();
 
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,

@@ -200,6 +200,16 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
 ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
                              int n_systems, double* A, double* B, double* C, double* D);
 
+/* Extension (uneven batches with FP32 storage; no counterpart in the
+ * reference): the layout and partitioning of ftd_solve_batched, with A, B,
+ * C, D float and the elimination in FP64 on their exactly widened values;
+ * x (FP64) goes to X at the same offsets, A..D are read only. Results equal
+ * ftd_solve_batched on the widened arrays, bit for bit. FTD_ERR_UNSUPPORTED
+ * for a system above 6144 rows or a batch spanning fewer than 4 elements. */
+ftd_status ftd_solve_batched_f32(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                                 int n_systems, const float* A, const float* B, const float* C,
+                                 const float* D, double* X);
+
 /* Extension (diagnostics): the path the last solve took, as a short
  * static string, e.g. "pascal_stream M=64 Q=8 R=16",
  * "pascal_fused M=32 T=256 S=16 Q=16" or "filtered_v1_general". */

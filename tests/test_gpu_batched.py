@@ -181,6 +181,86 @@ def test_batched_reduced_denominator_overflow():
     np.testing.assert_array_equal(X, ref)
 
 
+def run_batched_f32(be, off, n_rows, A, B, C, D):
+    """FP32 storage; returns x (float64) and the FP64 batch's x of the same,
+    widened, arrays."""
+    t32 = [torch.from_numpy(np.ascontiguousarray(v, dtype=np.float32)).cuda() for v in (A, B, C, D)]
+    X = torch.full((A.size,), -7.0, dtype=torch.float64, device="cuda")
+    be.solve_batched_f32(*t32, X, off, n_rows)
+    w = [t.double() for t in t32]
+    be.solve_batched(*w, off, n_rows)
+    torch.cuda.synchronize()
+    return X.cpu().numpy(), w[3].cpu().numpy(), be.last_path()
+
+
+def widened(*arrays):
+    return [np.asarray(v, dtype=np.float32).astype(np.float64) for v in arrays]
+
+
+def test_batched_f32_matches_oracle_on_widened_values():
+    """FP32 storage, FP64 elimination: bit for bit the oracle's solve of the
+    exactly widened arrays, and the FP64 batch's; thousands of items, every
+    edge length up to 6144 rows."""
+    edge = [n for n in EDGE if n <= 6144]
+    n_rows = np.concatenate([cases.uneven_lengths(6000, 16, 4096, seed=5), edge,
+                             np.arange(1, 16).repeat(50)]).astype(np.int32)
+    np.random.default_rng(6).shuffle(n_rows)
+    off, n_rows, A, B, C, D = general_uneven(n_rows, seed=61, gap=1)
+    A, B, C, D = widened(A, B, C, D)
+    note(f"batched f32 {n_rows.size} systems, {int(n_rows.sum())} rows")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        X, X64, path = run_batched_f32(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    assert path.startswith("batched"), path
+    ref = oracle_solve(off, n_rows, A, B, C, D)
+    mask = np.zeros(A.size, dtype=bool)
+    for o, n in zip(off, n_rows):
+        mask[int(o):int(o) + int(n)] = True
+    np.testing.assert_array_equal(X[mask], ref[mask])
+    np.testing.assert_array_equal(X[mask], X64[mask])
+    assert np.all(X[~mask] == -7.0)  # gaps untouched
+
+
+@pytest.mark.parametrize("lengths,gap", [
+    ([50, 5, 1, 1], 0),   # chunks ending 2, 1 and 0 elements before the arrays' end
+    ([97, 6], 0),         # the last quad holds 2 rows: read 2 elements low
+    ([300, 7], 0),        # 3 rows: 1 element low
+    ([40, 8], 0),         # a full last quad: no shift
+    ([1, 2, 1], 1),       # spans of a few elements
+    ([99], 0),            # one chunked system, three chunks of 33 rows
+])
+def test_batched_f32_arrays_end_at_last_row(lengths, gap):
+    off, n_rows, A, B, C, D = general_uneven(lengths, seed=len(lengths) + 70, gap=gap)
+    end = int(off[-1]) + int(n_rows[-1])
+    A, B, C, D = widened(*(v[:end] for v in (A, B, C, D)))
+    note(f"batched f32 array end {lengths}")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        X, X64, _ = run_batched_f32(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    ref = oracle_solve(off, n_rows, A, B, C, D)
+    for o, n in zip(off, n_rows):
+        sl = slice(int(o), int(o) + int(n))
+        np.testing.assert_array_equal(X[sl], ref[sl])
+        np.testing.assert_array_equal(X[sl], X64[sl])
+
+
+def test_batched_f32_unsupported_layouts():
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        f = torch.zeros(8000, dtype=torch.float32, device="cuda")
+        x = torch.zeros(8000, dtype=torch.float64, device="cuda")
+        with pytest.raises(FtdError):  # above 128 chunks of 48 rows
+            be.solve_batched_f32(f, f, f, f, x, [0], [6145])
+        with pytest.raises(FtdError):  # spans fewer than 4 elements
+            be.solve_batched_f32(f, f, f, f, x, [0, 1], [1, 2])
+    finally:
+        be.close()
+
+
 def test_batched_toeplitz_c4_shape_sample():
     """The uneven-batch workload's generator (a = c = -r, b = 1 + 2r,
     r ~ U[0.5, 50]) on 2000 systems, back to back."""
