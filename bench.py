@@ -168,6 +168,28 @@ def c4_uneven(steps, warmup, torch):
     wall = time.perf_counter() - t0
     ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
     path = be.last_path()
+    x64 = copies[0]
+
+    # the same batch with FP32 storage (A, B, C, D float32, x float64)
+    t32 = [torch.from_numpy(np.asarray(v, dtype=np.float32)).cuda() for v in (A, B, C, D)]
+    X = torch.empty_like(tD0)
+    for _ in range(warmup):
+        be.solve_batched_f32(*t32, X, off, n_rows)
+    torch.cuda.synchronize()
+    ev32 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(steps)]
+    for k in range(steps):
+        ev32[k][0].record(stream)
+        be.solve_batched_f32(*t32, X, off, n_rows)
+        ev32[k][1].record(stream)
+    torch.cuda.synchronize()
+    ms32 = statistics.mean(a.elapsed_time(b) for a, b in ev32)
+    path32 = be.last_path()
+    w = [v.double() for v in t32]
+    be.solve_batched(*w, off, n_rows)  # FP64 solve of the widened stored values
+    torch.cuda.synchronize()
+    same = bool(torch.equal(X, w[3]))
+    err = float((X - x64).abs().max() / x64.abs().max())
     be.close()
     return {
         "workload": "20000 systems, n_row log-uniform [16, 4096], r~U[0.5,50], FP64, "
@@ -176,6 +198,10 @@ def c4_uneven(steps, warmup, torch):
         "event_ms_mean": ms, "bytes_per_row": BYTES_PER_ROW,
         "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "path": path,
+        "f32": {"event_ms_mean": ms32, "rows_per_s": rows / (ms32 * 1e-3), "bytes_per_row": 24,
+                "hbm_frac_algorithmic": 24 * rows / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                "equals_fp64_solve_of_stored_values": same,
+                "max_rel_diff_vs_fp64_data": err, "path": path32},
     }
 
 
@@ -444,8 +470,12 @@ def main():
         del copies
         torch.cuda.empty_cache()
         c4 = {"uneven": c4_uneven(args.steps, args.warmup, torch), "adi_256": c4_adi(args.steps)}
+        u32 = c4["uneven"]["f32"]
         print(f"[bench] c4 uneven: {c4['uneven']['event_ms_mean']:.4f} ms "
               f"({c4['uneven']['hbm_frac_algorithmic']:.3f} of HBM) {c4['uneven']['path']}; "
+              f"fp32 storage {u32['event_ms_mean']:.4f} ms (= fp64 solve of the stored values: "
+              f"{u32['equals_fp64_solve_of_stored_values']}, vs fp64 data "
+              f"{u32['max_rel_diff_vs_fp64_data']:.1e}); "
               f"adi 256^3 graph step {c4['adi_256']['ms_per_step']:.4f} ms",
               file=sys.stderr, flush=True)
 
