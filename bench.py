@@ -141,7 +141,9 @@ def c3_error(x_dev, A32, B32, C32, Dw, path):
 def c4_uneven(steps, warmup, torch):
     """20000 systems, n_row = round(exp(U[ln 16, ln 4096])), a = c = -r,
     b = 1 + 2r, r ~ U[0.5, 50], D ~ U[-1, 1] (seed 0), back to back in one
-    flat array; one ftd_solve_batched per step on its own copy of D."""
+    flat array; the chunk length autotuned on the layout, then one
+    ftd_solve_batched per step on its own copy of D, then the same with FP32
+    storage."""
     from filtered_tdma_amd import TdmaBackendGPU, cases
 
     n_rows = cases.uneven_lengths(20000, 16, 4096, seed=0)
@@ -151,6 +153,7 @@ def c4_uneven(steps, warmup, torch):
     copies = [torch.empty_like(tD0) for _ in range(steps)]
     be = TdmaBackendGPU("pascal", 1, 1, 1)
     stream = torch.cuda.current_stream()
+    tune = be.autotune_batched(tA, tB, tC, tD0, off, n_rows, reps=3)  # chunk length
     for k in range(warmup):
         copies[k % steps].copy_(tD0)
         be.solve_batched(tA, tB, tC, copies[k % steps], off, n_rows)
@@ -198,6 +201,7 @@ def c4_uneven(steps, warmup, torch):
         "event_ms_mean": ms, "bytes_per_row": BYTES_PER_ROW,
         "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "path": path,
+        "autotune": {str(c["chunk"]): round(c["ms"], 4) for c in tune["candidates"]},
         "f32": {"event_ms_mean": ms32, "rows_per_s": rows / (ms32 * 1e-3), "bytes_per_row": 24,
                 "hbm_frac_algorithmic": 24 * rows / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9),
                 "equals_fp64_solve_of_stored_values": same,
@@ -471,7 +475,8 @@ def main():
         torch.cuda.empty_cache()
         c4 = {"uneven": c4_uneven(args.steps, args.warmup, torch), "adi_256": c4_adi(args.steps)}
         u32 = c4["uneven"]["f32"]
-        print(f"[bench] c4 uneven: {c4['uneven']['event_ms_mean']:.4f} ms "
+        print(f"[bench] c4 uneven: autotune {c4['uneven']['autotune']}, "
+              f"{c4['uneven']['event_ms_mean']:.4f} ms "
               f"({c4['uneven']['hbm_frac_algorithmic']:.3f} of HBM) {c4['uneven']['path']}; "
               f"fp32 storage {u32['event_ms_mean']:.4f} ms (= fp64 solve of the stored values: "
               f"{u32['equals_fp64_solve_of_stored_values']}, vs fp64 data "

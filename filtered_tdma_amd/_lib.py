@@ -17,8 +17,8 @@ EXPORTS = (
     "ftd_solve_cyclic", "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
     "ftd_set_profiling", "ftd_last_J", "ftd_last_gpu_ms", "ftd_last_comm_ms", "ftd_set_tuning",
     "ftd_last_path", "ftd_last_error_string", "ftd_solve_f32c", "ftd_solve_f32",
-    "ftd_solve_batched", "ftd_solve_batched_f32", "ftd_solve_complex", "ftd_set_j_mode",
-    "ftd_last_J_systems", "ftd_autotune",
+    "ftd_solve_batched", "ftd_solve_batched_f32", "ftd_set_batch_chunk", "ftd_autotune_batched",
+    "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
     "ftd_heat_set_plan",
@@ -71,6 +71,9 @@ def load() -> ctypes.CDLL:
                                     dp]
     L.ftd_solve_batched_f32.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ip, c_int, dp, dp,
                                         dp, dp, dp]
+    L.ftd_set_batch_chunk.argtypes = [vp, c_int]
+    L.ftd_autotune_batched.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ip, c_int, dp, dp,
+                                       dp, dp, c_int, ctypes.c_char_p, ctypes.c_size_t]
     L.ftd_solve_complex.argtypes = [vp, dp, dp, dp, dp]
     L.ftd_set_j_mode.argtypes = [vp, c_int]
     L.ftd_autotune.argtypes = [vp, dp, dp, dp, dp, c_int, ctypes.c_char_p, ctypes.c_size_t]
@@ -112,6 +115,7 @@ def load() -> ctypes.CDLL:
                  "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
                  "ftd_set_profiling", "ftd_last_J", "ftd_set_tuning", "ftd_solve_f32c",
                  "ftd_solve_f32", "ftd_solve_batched", "ftd_solve_batched_f32",
+                 "ftd_set_batch_chunk", "ftd_autotune_batched",
                  "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune"):
         getattr(L, name).restype = c_int
     _lib = L

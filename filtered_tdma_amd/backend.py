@@ -168,6 +168,22 @@ class TdmaBackendGPU:
         ptrs = [_ptr(x) for x in (A, B, C, D)]
         _lib.check(_lib.load().ftd_solve_batched(self._h, po, pr, n, *ptrs))
 
+    def set_batch_chunk(self, rows: int) -> None:
+        """Rows per chunk of the batched plans: 32, 48, or 0 for the default."""
+        _lib.check(_lib.load().ftd_set_batch_chunk(self._h, int(rows)))
+
+    def autotune_batched(self, A, B, C, D, row_offsets, n_rows, reps: int = 5) -> dict:
+        """Times the batched solve of this layout with each chunk length (on
+        copies of C and D) and keeps the faster; returns the report."""
+        import json
+
+        po, pr, n, _, _ = self._layout(row_offsets, n_rows, (A, B, C, D), "ABCD")
+        ptrs = [_ptr(x) for x in (A, B, C, D)]
+        buf = ctypes.create_string_buffer(2048)
+        _lib.check(_lib.load().ftd_autotune_batched(self._h, po, pr, n, *ptrs, int(reps), buf,
+                                                    len(buf)))
+        return json.loads(buf.value.decode())
+
     def solve_batched_f32(self, A, B, C, D, X, row_offsets, n_rows) -> None:
         """The uneven batch with FP32 storage: A, B, C, D float32, the
         elimination in FP64 on their exactly widened values, x into the

@@ -92,6 +92,8 @@ struct ftd_handle {
     std::vector<int> b_long_n;
     int b_items = 0, b_chunked = 0, b_valid = 0;
     int64_t b_last = 0;  // last element of the planned layout
+    int b_chunk = 0;     // rows per chunk asked for (0: the default)
+    int b_plan_chunk = 0;  // rows per chunk of the planned layout
     DevBuf b_dlanes, b_dlong_off, b_dlong_n;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -774,7 +776,8 @@ ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_
                         int n_systems) {
     if (!row_offsets || !n_rows) return fail(FTD_ERR_INVALID, "null layout array");
     const size_t n = (size_t)n_systems;
-    const bool same = h->b_valid && h->b_off.size() == n &&
+    const int chunk = h->b_chunk ? h->b_chunk : ftd::batched_default_chunk();
+    const bool same = h->b_valid && h->b_plan_chunk == chunk && h->b_off.size() == n &&
                       std::memcmp(h->b_off.data(), row_offsets, n * sizeof(int64_t)) == 0 &&
                       std::memcmp(h->b_rows.data(), n_rows, n * sizeof(int)) == 0;
     if (same) return FTD_OK;
@@ -785,8 +788,8 @@ ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_
     // the previous plan may still be in use by work queued on the stream
     if (h->b_valid) FTD_HIP(hipStreamSynchronize(h->stream));
     h->b_valid = 0;
-    h->b_items = ftd::batch_plan(n_systems, row_offsets, n_rows, h->b_lanes, h->b_long_off,
-                                 h->b_long_n, h->b_chunked, h->b_last);
+    h->b_items = ftd::batch_plan(n_systems, row_offsets, n_rows, chunk, h->b_lanes,
+                                 h->b_long_off, h->b_long_n, h->b_chunked, h->b_last);
     if (h->b_items < 0) return fail(FTD_ERR_INVALID, "every system needs n_rows >= 1");
     const size_t nl = h->b_long_n.size();
     FTD_TRY(ensure(h->b_dlanes, sizeof(ftd::BatchLane) * std::max<size_t>(1, h->b_lanes.size())));
@@ -806,6 +809,7 @@ ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_
     FTD_HIP(hipStreamSynchronize(h->stream));
     h->b_off.assign(row_offsets, row_offsets + n);
     h->b_rows.assign(n_rows, n_rows + n);
+    h->b_plan_chunk = chunk;
     h->b_valid = 1;
     return FTD_OK;
 }
@@ -820,10 +824,11 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
     if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
     FTD_TRY(batch_layout(h, row_offsets, n_rows, n_systems));
     const int nl = (int)h->b_long_n.size();
-    std::snprintf(h->path, sizeof h->path, "batched items=%d chunked=%d long=%d", h->b_items,
-                  h->b_chunked, nl);
+    std::snprintf(h->path, sizeof h->path, "batched chunk=%d items=%d chunked=%d long=%d",
+                  h->b_plan_chunk, h->b_items, h->b_chunked, nl);
     if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
-    FTD_HIP(ftd::launch_batched(A, B, C, D, (const ftd::BatchLane*)h->b_dlanes.p, h->b_chunked,
+    FTD_HIP(ftd::launch_batched(A, B, C, D, h->b_plan_chunk,
+                                (const ftd::BatchLane*)h->b_dlanes.p, h->b_chunked,
                                 h->b_items, h->b_last,
                                 (const int64_t*)h->b_dlong_off.p, (const int*)h->b_dlong_n.p, nl,
                                 h->stream));
@@ -846,21 +851,99 @@ ftd_status ftd_solve_batched_f32(ftd_handle* h, const int64_t* row_offsets, cons
     if (!A || !B || !C || !D || !X) return fail(FTD_ERR_INVALID, "null array pointer");
     FTD_TRY(batch_layout(h, row_offsets, n_rows, n_systems));
     if (!h->b_long_n.empty())
-        return fail(FTD_ERR_UNSUPPORTED, "FP32 batches take systems of up to " +
-                                             std::to_string(ftd::batched_max_rows()) + " rows");
+        return fail(FTD_ERR_UNSUPPORTED,
+                    "FP32 batches take systems of up to " +
+                        std::to_string(ftd::batched_max_rows(h->b_plan_chunk)) + " rows");
     if (h->b_last < 3)
         return fail(FTD_ERR_UNSUPPORTED, "FP32 batches must span at least 4 elements");
-    std::snprintf(h->path, sizeof h->path, "batched_f32 items=%d chunked=%d", h->b_items,
-                  h->b_chunked);
+    std::snprintf(h->path, sizeof h->path, "batched_f32 chunk=%d items=%d chunked=%d",
+                  h->b_plan_chunk, h->b_items, h->b_chunked);
     if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
-    FTD_HIP(ftd::launch_batched_f32(A, B, C, D, X, (const ftd::BatchLane*)h->b_dlanes.p,
-                                    h->b_chunked, h->b_items, h->b_last, h->stream));
+    FTD_HIP(ftd::launch_batched_f32(A, B, C, D, X, h->b_plan_chunk,
+                                    (const ftd::BatchLane*)h->b_dlanes.p, h->b_chunked,
+                                    h->b_items, h->b_last, h->stream));
     if (h->profiling) {
         FTD_HIP(hipEventRecord(h->ev1, h->stream));
         FTD_HIP(hipEventSynchronize(h->ev1));
         float ms = 0.0f;
         FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
+ftd_status ftd_set_batch_chunk(ftd_handle* h, int rows) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (rows != 0 && rows != 32 && rows != 48)
+        return fail(FTD_ERR_INVALID, "rows per chunk must be 32, 48 or 0 (the default)");
+    h->b_chunk = rows;
+    return FTD_OK;
+}
+
+ftd_status ftd_autotune_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                                int n_systems, double* A, double* B, const double* C,
+                                const double* D, int reps, char* report, size_t report_len) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (n_systems < 1) return fail(FTD_ERR_INVALID, "nothing to tune");
+    if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (reps < 1 || reps > 100) return fail(FTD_ERR_INVALID, "reps must be in [1, 100]");
+    const int saved_chunk = h->b_chunk, saved_prof = h->profiling;
+    h->b_chunk = 0;
+    FTD_TRY(batch_layout(h, row_offsets, n_rows, n_systems));
+    // the solves run on copies of C (the lane-per-system kernel keeps c' there)
+    // and D, so the caller's arrays are left as they were
+    const size_t bytes = sizeof(double) * (size_t)(h->b_last + 1);
+    double* work = nullptr;
+    FTD_HIP(hipMalloc(&work, 2 * bytes));
+    double* wc = work;
+    double* wd = work + (h->b_last + 1);
+    std::string rep = "{\"candidates\": [";
+    int best = 0;
+    double best_ms = 0.0;
+    h->profiling = 0;
+    ftd_status st = FTD_OK;
+    hipError_t e = hipSuccess;
+    const int chunks[2] = {48, 32};
+    for (int k = 0; k < 2 && st == FTD_OK && e == hipSuccess; ++k) {
+        h->b_chunk = chunks[k];
+        float total = 0.0f;
+        for (int r = 0; r <= reps && st == FTD_OK && e == hipSuccess; ++r) {  // r = 0: warm-up
+            e = hipMemcpyAsync(wc, C, bytes, hipMemcpyDeviceToDevice, h->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(wd, D, bytes, hipMemcpyDeviceToDevice, h->stream);
+            if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+            if (e != hipSuccess) break;
+            st = ftd_solve_batched(h, row_offsets, n_rows, n_systems, A, B, wc, wd);
+            if (st == FTD_OK) e = hipEventRecord(h->ev1, h->stream);
+            if (st == FTD_OK && e == hipSuccess) e = hipEventSynchronize(h->ev1);
+            float ms = 0.0f;
+            if (st == FTD_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+            if (r > 0) total += ms;
+        }
+        if (st != FTD_OK || e != hipSuccess) break;
+        const double ms = total / reps;
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "%s{\"chunk\": %d, \"ms\": %.6f, \"path\": \"%s\"}",
+                      k ? ", " : "", chunks[k], ms, h->path);
+        rep += buf;
+        if (k == 0 || ms < best_ms) {
+            best = chunks[k];
+            best_ms = ms;
+        }
+    }
+    hipError_t ef = hipFree(work);
+    h->profiling = saved_prof;
+    if (st != FTD_OK || e != hipSuccess || ef != hipSuccess) {
+        h->b_chunk = saved_chunk;
+        return st != FTD_OK ? st : hip_fail(e != hipSuccess ? e : ef, "autotune_batched");
+    }
+    h->b_chunk = best;
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "], \"best\": {\"chunk\": %d, \"ms\": %.6f}}", best,
+                  best_ms);
+    rep += buf;
+    if (report && report_len) {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = '\0';
     }
     return FTD_OK;
 }

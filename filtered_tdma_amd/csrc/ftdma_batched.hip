@@ -55,11 +55,13 @@ namespace ftd {
 
 namespace {
 
-// Rows per lane: the longest chunk whose elimination state (a, c, d of every
-// row) fits the register file without scratch (64 does not). Longer chunks
-// mean fewer of them per system, so a shorter serial reduced system: 48 rows
-// ran the 20000-system batch in 0.264 ms against 0.290 with 32.
-constexpr int kChunk = 48;
+// Rows per lane, a plan parameter with two compiled shapes. 48 is the longest
+// chunk whose elimination state (a, c, d of every row) fits the register
+// file without scratch (64 does not): fewer chunks per system, so a shorter
+// serial reduced system -- the 20000-system batch ran in 0.264 ms against
+// 0.290 with 32. 32-row chunks leave more lanes to batches of many short
+// systems; ftd_autotune_batched times both on the caller's layout.
+constexpr int kChunk = 48;  // the default
 constexpr int kLanes = 128;  // lanes per work item (two waves)
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -686,9 +688,10 @@ void split(int n, int P, int p, int& s, int& m) {
 
 }  // namespace
 
-int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
+int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int chunk,
                std::vector<BatchLane>& lanes, std::vector<int64_t>& long_off,
                std::vector<int>& long_n, int& n_chunked, int64_t& last) {
+    if (chunk != 32 && chunk != 48) return -1;
     n_chunked = 0;
     lanes.clear();
     long_off.clear();
@@ -701,7 +704,7 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
     std::vector<int> multi, single;
     for (int s = 0; s < n_systems; ++s) {
         const int N = n_rows[s];
-        const int Q = (N + kChunk - 1) / kChunk;
+        const int Q = (N + chunk - 1) / chunk;
         // the staging reads two elements at a time, so a batch spanning one
         // element (a single 1-row system) takes the lane-per-system kernel
         if (Q > kLanes || last == 0) {
@@ -713,7 +716,7 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
             single.push_back(s);
         }
     }
-    auto chunks = [&](int s) { return (n_rows[s] + kChunk - 1) / kChunk; };
+    auto chunks = [&](int s) { return (n_rows[s] + chunk - 1) / chunk; };
     // longest first; ties keep the caller's order
     std::stable_sort(multi.begin(), multi.end(), [&](int x, int y) { return chunks(x) > chunks(y); });
     std::stable_sort(single.begin(), single.end(),
@@ -798,14 +801,14 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
     return (int)items.size();
 }
 
-template <typename TS>
+template <int M, typename TS>
 int batched_resident_blocks() {
     static int cached[64] = {};
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
     if (cached[dev]) return cached[dev];
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void*)k_batched<kChunk, TS>, kLanes, 0) != hipSuccess ||
+            &per_cu, (const void*)k_batched<M, TS>, kLanes, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -815,15 +818,27 @@ int batched_resident_blocks() {
     return cached[dev];
 }
 
-hipError_t launch_batched(const double* A, const double* B, double* C, double* D,
+template <int M, typename TS>
+hipError_t launch_items(const TS* A, const TS* B, const TS* C, const TS* D, double* X,
+                        const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
+                        hipStream_t st) {
+    const int blocks = std::min(batched_resident_blocks<M, TS>(), n_items);
+    hipLaunchKernelGGL((k_batched<M, TS>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D, X, lanes,
+                       n_chunked, n_items, last);
+    return hipGetLastError();
+}
+
+hipError_t launch_batched(const double* A, const double* B, double* C, double* D, int chunk,
                           const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st) {
     if (n_items > 0) {
-        const int blocks = std::min(batched_resident_blocks<double>(), n_items);
-        hipLaunchKernelGGL((k_batched<kChunk, double>), dim3(blocks), dim3(kLanes), 0, st, A, B,
-                           (const double*)C, (const double*)D, D, lanes, n_chunked, n_items, last);
-        hipError_t e = hipGetLastError();
+        const double* Dc = D;
+        hipError_t e = chunk == 32
+                           ? launch_items<32>(A, B, (const double*)C, Dc, D, lanes, n_chunked,
+                                              n_items, last, st)
+                           : launch_items<48>(A, B, (const double*)C, Dc, D, lanes, n_chunked,
+                                              n_items, last, st);
         if (e != hipSuccess) return e;
     }
     if (n_long > 0) {
@@ -835,15 +850,15 @@ hipError_t launch_batched(const double* A, const double* B, double* C, double* D
 }
 
 hipError_t launch_batched_f32(const float* A, const float* B, const float* C, const float* D,
-                              double* X, const BatchLane* lanes, int n_chunked, int n_items,
-                              int64_t last, hipStream_t st) {
+                              double* X, int chunk, const BatchLane* lanes, int n_chunked,
+                              int n_items, int64_t last, hipStream_t st) {
     if (n_items < 1) return hipSuccess;
-    const int blocks = std::min(batched_resident_blocks<float>(), n_items);
-    hipLaunchKernelGGL((k_batched<kChunk, float>), dim3(blocks), dim3(kLanes), 0, st, A, B, C, D,
-                       X, lanes, n_chunked, n_items, last);
-    return hipGetLastError();
+    return chunk == 32 ? launch_items<32>(A, B, C, D, X, lanes, n_chunked, n_items, last, st)
+                       : launch_items<48>(A, B, C, D, X, lanes, n_chunked, n_items, last, st);
 }
 
-int batched_max_rows() { return kLanes * kChunk; }
+int batched_default_chunk() { return kChunk; }
+
+int batched_max_rows(int chunk) { return kLanes * chunk; }
 
 }  // namespace ftd

@@ -69,26 +69,28 @@ struct alignas(16) BatchLane {
 };
 static_assert(sizeof(BatchLane) == 16, "BatchLane is one dwordx4");
 
-// Plans the batch: lanes (128 per item; the n_chunked items of chunked
-// systems first, then single-chunk items, each range longest first), the
-// systems longer than 128 * 48 rows (lane-per-system kernel), and the last
-// element any system holds (the staging never reads past it). Returns the
-// number of items, or -1 when some n_rows[s] < 1.
-int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows,
+// Plans the batch in chunks of `chunk` rows (32 or 48): lanes (128 per item;
+// the n_chunked items of chunked systems first, then single-chunk items, each
+// range longest first), the systems longer than 128 * chunk rows
+// (lane-per-system kernel), and the last element any system holds (the
+// staging never reads past it). Returns the number of items, or -1 when
+// some n_rows[s] < 1 or the chunk is not one of the two.
+int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int chunk,
                std::vector<BatchLane>& lanes, std::vector<int64_t>& long_off,
                std::vector<int>& long_n, int& n_chunked, int64_t& last);
-hipError_t launch_batched(const double* A, const double* B, double* C, double* D,
+hipError_t launch_batched(const double* A, const double* B, double* C, double* D, int chunk,
                           const BatchLane* lanes, int n_chunked, int n_items, int64_t last,
                           const int64_t* long_off, const int* long_n, int n_long,
                           hipStream_t st);
 // FP32 storage: A, B, C, D float, elimination in FP64 on the widened values,
 // x (FP64) into X at the same offsets. No long-system list: the caller
-// rejects systems above batched_max_rows() and batches spanning fewer than 4
-// elements.
+// rejects systems above batched_max_rows(chunk) and batches spanning fewer
+// than 4 elements.
 hipError_t launch_batched_f32(const float* A, const float* B, const float* C, const float* D,
-                              double* X, const BatchLane* lanes, int n_chunked, int n_items,
-                              int64_t last, hipStream_t st);
-int batched_max_rows();
+                              double* X, int chunk, const BatchLane* lanes, int n_chunked,
+                              int n_items, int64_t last, hipStream_t st);
+int batched_default_chunk();
+int batched_max_rows(int chunk);
 
 int thomas_fused_rows(int n_row);
 hipError_t launch_thomas_fused(const double* A, const double* B, const double* C, double* D,

@@ -189,7 +189,8 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
  * elements row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of each of A, B,
  * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
  * a system of <= 48 rows is plain Thomas, a longer one PaScaL with
- * ceil(n_rows/48) partitions inside one 128-lane workgroup; all in one
+ * ceil(n_rows/48) partitions inside one 128-lane workgroup (48: the default
+ * chunk, see ftd_set_batch_chunk); all in one
  * persistent launch (systems above 6144 rows, and a batch that is one 1-row
  * system at offset 0, in a second, lane-per-system launch that uses C as
  * scratch). D is overwritten with x; A and B are read only. No element past
@@ -209,6 +210,23 @@ ftd_status ftd_solve_batched(ftd_handle* h, const int64_t* row_offsets, const in
 ftd_status ftd_solve_batched_f32(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
                                  int n_systems, const float* A, const float* B, const float* C,
                                  const float* D, double* X);
+
+/* Extension (uneven batches): rows per chunk of the batched plans, 32 or 48
+ * (0: the default, 48). A system of up to `rows` rows is one lane's Thomas,
+ * a longer one PaScaL with ceil(n_rows / rows) partitions (the path string
+ * names the chunk), up to 128 * rows rows. Takes effect at the next batched
+ * solve (which plans the layout again). */
+ftd_status ftd_set_batch_chunk(ftd_handle* h, int rows);
+
+/* Extension (uneven batches): times ftd_solve_batched on this layout with
+ * each chunk length (reps timed solves each, after one untimed), on device
+ * copies of C and D (the caller's arrays are left as they were), and keeps
+ * the faster one on the handle (as ftd_set_batch_chunk). A JSON report of
+ * the candidates goes to `report` (if not NULL). Waits for the handle's
+ * stream; allocates two arrays of the layout's extent for the copies. */
+ftd_status ftd_autotune_batched(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+                                int n_systems, double* A, double* B, const double* C,
+                                const double* D, int reps, char* report, size_t report_len);
 
 /* Extension (diagnostics): the path the last solve took, as a short
  * static string, e.g. "pascal_stream M=64 Q=8 R=16",

@@ -42,17 +42,17 @@ def general_uneven(n_rows, seed, gap):
     return off, n_rows, A, B, C, D
 
 
-CHUNK = 48  # rows per lane of the batched kernel
+CHUNK = 48  # rows per lane of the batched kernel (the default)
 
 
-def oracle_solve(off, n_rows, A, B, C, D):
+def oracle_solve(off, n_rows, A, B, C, D, chunk=CHUNK):
     """x of every system by the oracle, with the engine's partitioning."""
     X = D.copy()
     for s in range(n_rows.size):
         o, n = int(off[s]), int(n_rows[s])
         sl = slice(o, o + n)
         a, b, c, d = (v[sl].reshape(n, 1).copy() for v in (A, B, C, D))
-        Q = (n + CHUNK - 1) // CHUNK
+        Q = (n + chunk - 1) // chunk
         if Q == 1 or Q > 128:
             x = O.thomas(a, b, c, d, sem=O.SEM_GPU)
         else:
@@ -197,6 +197,14 @@ def widened(*arrays):
     return [np.asarray(v, dtype=np.float32).astype(np.float64) for v in arrays]
 
 
+def system_rows(off, n_rows, size):
+    """Mask of the elements that belong to a system (x goes there)."""
+    mask = np.zeros(size, dtype=bool)
+    for o, n in zip(off, n_rows):
+        mask[int(o):int(o) + int(n)] = True
+    return mask
+
+
 def test_batched_f32_matches_oracle_on_widened_values():
     """FP32 storage, FP64 elimination: bit for bit the oracle's solve of the
     exactly widened arrays, and the FP64 batch's; thousands of items, every
@@ -215,9 +223,7 @@ def test_batched_f32_matches_oracle_on_widened_values():
         be.close()
     assert path.startswith("batched"), path
     ref = oracle_solve(off, n_rows, A, B, C, D)
-    mask = np.zeros(A.size, dtype=bool)
-    for o, n in zip(off, n_rows):
-        mask[int(o):int(o) + int(n)] = True
+    mask = system_rows(off, n_rows, A.size)
     np.testing.assert_array_equal(X[mask], ref[mask])
     np.testing.assert_array_equal(X[mask], X64[mask])
     assert np.all(X[~mask] == -7.0)  # gaps untouched
@@ -259,6 +265,54 @@ def test_batched_f32_unsupported_layouts():
             be.solve_batched_f32(f, f, f, f, x, [0, 1], [1, 2])
     finally:
         be.close()
+
+
+def test_batched_chunk_32_matches_oracle():
+    """32-row chunks (ftd_set_batch_chunk): PaScaL with ceil(n/32) partitions,
+    the lane-per-system kernel above 4096 rows."""
+    n_rows = np.concatenate([cases.uneven_lengths(3000, 16, 4096, seed=8), EDGE]).astype(np.int32)
+    off, n_rows, A, B, C, D = general_uneven(n_rows, seed=81, gap=2)
+    note(f"batched chunk 32, {n_rows.size} systems")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        be.set_batch_chunk(32)
+        _, _, X, path = run_batched(be, off, n_rows, A, B, C, D)
+        with pytest.raises(FtdError):
+            be.set_batch_chunk(40)
+    finally:
+        be.close()
+    assert "chunk=32" in path, path
+    np.testing.assert_array_equal(X, oracle_solve(off, n_rows, A, B, C, D, chunk=32))
+
+
+def test_batched_autotune_keeps_the_faster_chunk():
+    n_rows = cases.uneven_lengths(4000, 16, 4096, seed=9)
+    off, n_rows, A, B, C, D = general_uneven(n_rows, seed=91, gap=1)
+    note(f"batched autotune, {n_rows.size} systems")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        tA, tB, tC, tD = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (A, B, C, D))
+        report = be.autotune_batched(tA, tB, tC, tD, off, n_rows, reps=2)
+        torch.cuda.synchronize()
+        # the caller's arrays are left as they were
+        np.testing.assert_array_equal(tC.cpu().numpy(), C)
+        np.testing.assert_array_equal(tD.cpu().numpy(), D)
+        best = report["best"]["chunk"]
+        assert sorted(c["chunk"] for c in report["candidates"]) == [32, 48]
+        assert best in (32, 48)
+        be.solve_batched(tA, tB, tC, tD, off, n_rows)
+        torch.cuda.synchronize()
+        path = be.last_path()
+        X = tD.cpu().numpy()
+        A32, B32, C32, D32 = widened(A, B, C, D)
+        X32, _, path32 = run_batched_f32(be, off, n_rows, A32, B32, C32, D32)
+    finally:
+        be.close()
+    assert f"chunk={best}" in path and f"chunk={best}" in path32, (path, path32)
+    np.testing.assert_array_equal(X, oracle_solve(off, n_rows, A, B, C, D, chunk=best))
+    mask = system_rows(off, n_rows, A.size)
+    ref32 = oracle_solve(off, n_rows, A32, B32, C32, D32, chunk=best)
+    np.testing.assert_array_equal(X32[mask], ref32[mask])
 
 
 def test_batched_toeplitz_c4_shape_sample():
