@@ -1095,7 +1095,9 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         if (j < m) {
             const double aj = ra[j], bj = rb[j], cj = rc[j], dj = rd[j];
             if (PASS == 3) {
-                if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj / bj), fabs(cj / bj)));
+                // max(|a/b|, |c/b|) as max(|a|, |c|) / |b|: the same value
+                // (rounding is monotonic in the numerator), one division
+                if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(bj));
                 if (j == 0) f_d0 = dj;
                 if (j == m - 1) f_dN = dj;
                 f_dmax = fmax(f_dmax, fabs(dj));
@@ -1318,7 +1320,8 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             rg.take(j, first, tile, ntile, dcnt, l, aj, bj, cj, dj);
             if (j < m) {
                 if (PASS == 3) {
-                    if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj / bj), fabs(cj / bj)));
+                    // max(|a/b|, |c/b|) in one division, the same value
+                    if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(bj));
                     if (j == 0) f_d0 = dj;
                     if (j == m - 1) f_dN = dj;
                     f_dmax = fmax(f_dmax, fabs(dj));
