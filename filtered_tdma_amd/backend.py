@@ -213,7 +213,6 @@ class TdmaBackendGPU:
         _lib.check(_lib.load().ftd_set_tuning(self._h, rows_per_lane, max_threads,
                                               systems_per_tile, lanes_per_system))
 
-
     def autotune(self, A, B, C, D, reps: int = 3) -> dict:
         """Time every candidate kernel shape on copies of D and keep the
         fastest for later solves; returns the candidates' times and the
