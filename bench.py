@@ -407,8 +407,8 @@ def cpu_baseline(A, B, C, D):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(systems shard across GPUs with no communication; see the design notes)",
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the systems would shard across GPUs with no communication between them)",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -434,7 +434,7 @@ static double part_rho(const double* A_rho, const double* C_rho, int s, int m) {
 /* Forward sweep. v1: filtered_tdma_kernels.hpp:35-65 / .cuh:69-94 (full).
  * v2 GPU: .cuh:120-155 literally, including J == 0, where the second loop
  * starts at row 1 and eliminates the already-normalised row 1 once more.
- * v2 CPU: filtered_tdma_kernels.hpp:68-110 (the driver keeps J >= 1 here). */
+ * v2 CPU: filtered_tdma_kernels.hpp:68-110 (the reference's solve keeps J >= 1 here). */
 static void ffwd_cols(double* A, const double* B, double* C, double* D, size_t ld,
                       int c0, int c1, int m, int J_full /* last row with a-update */,
                       int gpu_v2) {
@@ -490,7 +490,7 @@ static void ffwd_cols(double* A, const double* B, double* C, double* D, size_t l
  * v1: filtered_tdma_kernels.hpp:117-135,178-189 / .cuh:96-118.
  * v2 GPU: .cuh:157-194: a-update only for j < J seeded from A[J] (J >= 2),
  *   c-update only for j > lo = (m-2)-J.
- * v2 CPU: filtered_tdma_kernels.hpp:138-172 (J <= m-2 by the driver clamp). */
+ * v2 CPU: filtered_tdma_kernels.hpp:138-172 (J <= m-2 by the reference's clamp). */
 static void fbwd_cols(double* A, double* C, double* D, size_t ld, int c0, int c1, int m,
                       int version, int J, int sem) {
     if (version == 1) {

@@ -1,7 +1,8 @@
 // ref_driver.cpp -- drives the reference's own CPU solvers (compiled from the
 // sources under /root/reference by oracle/Makefile.ref) to write golden
-// vectors for the oracle. TEST INFRASTRUCTURE ONLY; it never reaches the GPU
-// box (the reference tree is not there) and no product code calls it.
+// vectors for the oracle. TEST INFRASTRUCTURE ONLY; it needs the reference
+// tree to build, is not used where the GPU tests run, and no product code
+// calls it.
 //
 // Two modes, both reading one input file and writing the solved D:
 //
