@@ -2,10 +2,11 @@
 oracle's restatement of apps/heat_cpu, and the hipGraph replay against the
 eager steps.
 
-Bit-for-bit where every sweep runs the same partitioning in both (the
-filtered kinds always; PaScaL when its sweeps fit the single-lane Thomas
-kernel); the L2 text must equal what the reference heat_cpu prints for the
-same case (tests/golden/heat_l2.json)."""
+Bit-for-bit where every sweep runs the same partitioning in both: the
+directions the caller splits, and undivided ones that fit the single-lane
+Thomas kernel (n <= 64; longer undivided lines run in internal partitions on
+the GPU). For PaScaL the L2 text must equal what the reference heat_cpu
+prints for the same case (tests/golden/heat_l2.json)."""
 import json
 import os
 import sys
@@ -112,20 +113,25 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
         assert np.max(np.abs(f[inner] - thomas[inner])) <= 1e-6, key
 
 
-@pytest.mark.parametrize("n,parts,periodic", [
-    (48, (1, 1, 1), (0, 0, 0)),     # Thomas sweeps (N <= 64)
-    (100, (1, 1, 1), (0, 0, 0)),    # internal PaScaL partitions (N > 64)
-    (64, (2, 4, 8), (0, 0, 0)),     # PaScaL with the caller's partitions
-    (40, (1, 2, 1), (1, 0, 0)),     # periodic x: materialized there, fused in y, z
-    (37, (1, 1, 3), (0, 0, 0)),     # odd sizes, uneven partitions
+@pytest.mark.parametrize("kind,n,parts,periodic", [
+    ("pascal", 48, (1, 1, 1), (0, 0, 0)),     # Thomas sweeps (N <= 64)
+    ("pascal", 100, (1, 1, 1), (0, 0, 0)),    # internal PaScaL partitions (N > 64)
+    ("pascal", 64, (2, 4, 8), (0, 0, 0)),     # PaScaL with the caller's partitions
+    ("pascal", 40, (1, 2, 1), (1, 0, 0)),     # periodic x: materialized there, fused in y, z
+    ("pascal", 37, (1, 1, 3), (0, 0, 0)),     # odd sizes, uneven partitions
+    # filtered kinds: x and y (one partition) fused, z materialized
+    ("filtered", 100, (1, 1, 4), (0, 0, 0)),
+    ("filtered_v2", 48, (1, 1, 4), (0, 0, 0)),
+    ("filtered", 40, (1, 2, 1), (0, 0, 0)),   # y materialized, x and z fused
 ])
-def test_heat_fused_sweeps_equal_materialized(n, parts, periodic):
+def test_heat_fused_sweeps_equal_materialized(kind, n, parts, periodic):
     """Fused sweeps (coefficients generated in the kernel, rhs and x in the
     field) give the materialized build + solve + scatter field bit for bit,
-    eagerly and from a graph."""
+    eagerly and from a graph. The filtered kinds fuse the directions of one
+    caller partition, where their solve is the exact one."""
     dt = dt_policy("order", 0.3, n)[0]
-    note(f"heat fused vs materialized n={n} parts={parts} periodic={periodic}")
-    sims = [SolveTheta(n, n, n, parts=parts, backend="pascal", dt=dt, periodic=periodic)
+    note(f"heat fused vs materialized {kind} n={n} parts={parts} periodic={periodic}")
+    sims = [SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt, periodic=periodic)
             for _ in range(2)]
     try:
         sims[1].set_fused(False)
