@@ -102,15 +102,17 @@ struct Stencil {
 
 // explicit RHS of the factored Crank-Nicolson step (solve_theta.cpp:114-134),
 // marching in z: a thread owns one (i, j) column of a 64 x 4 tile and walks
-// kRhsKB planes with th at k-1, k, k+1 in registers, so each value of th is
+// kb planes with th at k-1, k, k+1 in registers, so each value of th is
 // fetched from HBM about once (one thread per cell would fetch every plane
-// again for its neighbours above and below, from other workgroups).
+// again for its neighbours above and below, from other workgroups). kb is
+// kRhsKB, or shorter where a grid of kRhsKB-plane runs would not cover the
+// CUs (rhs_planes); every cell's value is the same expression either way.
 constexpr int kRhsKB = 32;
 __global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
                                                double* __restrict__ rhs, Dims g, Stencil s,
                                                const double* __restrict__ cx,
                                                const double* __restrict__ cy,
-                                               const double* __restrict__ cz) {
+                                               const double* __restrict__ cz, int kb) {
     // Workgroups go round-robin to the 8 XCDs, each with its own L2. Each
     // XCD takes a contiguous run of tiles, y fastest, so the rows above and
     // below a 64 x 4 tile were just read by its neighbours on the same XCD
@@ -121,9 +123,9 @@ __global__ __launch_bounds__(256) void k_rhs_z(const double* __restrict__ th,
     const int by = T % nby, bx = (T / nby) % nbx, bz = T / (nby * nbx);
     const int i = bx * 64 + threadIdx.x + 1;
     const int j = by * 4 + threadIdx.y + 1;
-    const int k0 = bz * kRhsKB + 1;
+    const int k0 = bz * kb + 1;
     if (i > g.nx || j > g.ny) return;
-    const int k1 = k0 + kRhsKB - 1 < g.nz ? k0 + kRhsKB - 1 : g.nz;
+    const int k1 = k0 + kb - 1 < g.nz ? k0 + kb - 1 : g.nz;
     const size_t sy = (size_t)g.px, sz = (size_t)g.px * g.ny1;
     const double diag = 1.0 + s.xb + s.yb + s.zb;
     const double cxy = s.c0 * cx[i] * cy[j];
@@ -578,6 +580,7 @@ ftd_status herr(ftd_status s, const std::string& m) {
 struct ftd_heat {
     Dims g{};
     int nz = 0;
+    int rhs_kb = kRhsKB;  // planes per z run of k_rhs_z
     int parts[3] = {1, 1, 1};
     int periodic[3] = {0, 0, 0};
     int kind = FTD_KIND_PASCAL;
@@ -612,6 +615,27 @@ namespace {
 // holds, and with one caller partition (P == 1) thereby the number of
 // internal partitions Q; systems_per_tile (power of two; 0 = automatic)
 // the systems of one workgroup (S * Q <= 256)
+// compute units of the current device (1 if it cannot be read)
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+        return 1;
+    return cus;
+}
+
+// planes per z run of k_rhs_z: kRhsKB, halved (down to 4) while the grid of
+// 64 x 4 x kb tiles has fewer workgroups than the device has CUs (64^3: 32
+// tiles of 32 planes would leave most CUs idle)
+int rhs_planes(int nx, int ny, int nz) {
+    const int cus = device_cus();
+    const long xy = (long)((nx + 63) / 64) * ((ny + 3) / 4);
+    int kb = kRhsKB;
+    while (kb > 4 && xy * ((nz + kb - 1) / kb) < cus) kb >>= 1;
+    return kb;
+}
+
 void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
                 int systems_per_tile = 0) {
     ftd_heat::Fused f;
@@ -647,10 +671,8 @@ void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
     // tiles of 16 lines, 0.073 ms per step against 0.087 with 64 tiles of 64;
     // 8-line tiles were slower again). The bits do not depend on the tile.
     if (d == 0 && !systems_per_tile) {
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
-            while (S > 16 && (n_sys + S - 1) / S < cus) S >>= 1;
+        const int cus = device_cus();
+        while (S > 16 && (n_sys + S - 1) / S < cus) S >>= 1;
     }
     auto lds_of = [&](int S_) {
         const size_t red = f.thomas ? 0 : (size_t)3 * 2 * Q * S_;
@@ -729,9 +751,9 @@ ftd_status enqueue_step(ftd_heat* h) {
         hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f,
                            h->cx, h->cy, h->cz);
     }
-    hipLaunchKernelGGL(k_rhs_z,
-                       dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kRhsKB - 1) / kRhsKB)),
-                       dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz);
+    const int kb = h->rhs_kb;
+    hipLaunchKernelGGL(k_rhs_z, dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kb - 1) / kb)),
+                       dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz, kb);
     HH(hipGetLastError());
     const size_t n_in = (size_t)g.nx * g.ny * h->nz;
     const dim3 b1(256), g1((unsigned)((n_in + 255) / 256));
@@ -854,6 +876,7 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
         ftd_heat_destroy(h);
         return herr(FTD_ERR_HIP, std::string("ftd_heat_create: ") + hipGetErrorString(e));
     }
+    h->rhs_kb = rhs_planes(nx, ny, nz);
     // x, y, z sweeps: n_sys = ny*nz, nx*nz, nx*ny
     const int ns[3] = {ny * nz, nx * nz, nx * ny}, nr[3] = {nx, ny, nz};
     for (int d = 0; d < 3; ++d) {
