@@ -16,6 +16,7 @@
 // replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,10 +53,14 @@ inline size_t field_doubles(const Dims& g, int nz) {
 }
 
 // u_ghost = 2 u_BC - u_adjacent, u_BC = -cos(pi a) cos(pi b) on each face
-// (mpi_subdomain.cpp:180-236); face f: 0 = x, 1 = y, 2 = z
-__global__ void k_dirichlet(double* __restrict__ th, Dims g, int face,
+// (mpi_subdomain.cpp:180-236); face f = blockIdx.z: 0 = x, 1 = y, 2 = z, the
+// faces in bit f of `faces`. The faces write disjoint ghost cells (no edges)
+// and read only interior cells, so one launch serves all three.
+__global__ void k_dirichlet(double* __restrict__ th, Dims g, int faces,
                             const double* __restrict__ cx, const double* __restrict__ cy,
                             const double* __restrict__ cz) {
+    const int face = blockIdx.z;
+    if (!((faces >> face) & 1)) return;
     const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int v = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (face == 0) {
@@ -740,17 +745,26 @@ hipError_t allow_lds(const ftd_heat::Fused& f, int d) {
     return hipErrorInvalidValue;
 }
 
+// Dirichlet ghosts of every non-periodic face, one launch (grid z = face)
+void launch_dirichlet(ftd_heat* h, hipStream_t st) {
+    const Dims g = h->g;
+    int faces = 0, mu = 0, mv = 0;
+    for (int f = 0; f < 3; ++f) {
+        if (h->periodic[f]) continue;
+        faces |= 1 << f;
+        mu = std::max(mu, f == 0 ? g.ny : g.nx);
+        mv = std::max(mv, f == 2 ? g.ny : g.nz);
+    }
+    if (faces)
+        hipLaunchKernelGGL(k_dirichlet, dim3((mu + 15) / 16, (mv + 15) / 16, 3), dim3(16, 16), 0, st,
+                           h->th, g, faces, h->cx, h->cy, h->cz);
+}
+
 ftd_status enqueue_step(ftd_heat* h) {
     const Dims g = h->g;
     hipStream_t st = h->stream;
     const dim3 b2(16, 16);
-    // Dirichlet ghosts of the non-periodic faces
-    for (int f = 0; f < 3; ++f) {
-        if (h->periodic[f]) continue;
-        const int u = f == 0 ? g.ny : g.nx, v = f == 2 ? g.ny : g.nz;
-        hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, st, h->th, g, f,
-                           h->cx, h->cy, h->cz);
-    }
+    launch_dirichlet(h, st);
     const int kb = h->rhs_kb;
     hipLaunchKernelGGL(k_rhs_z, dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kb - 1) / kb)),
                        dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz, kb);
@@ -933,12 +947,7 @@ ftd_status ftd_heat_init(ftd_heat* h) {
         const int u = d == 0 ? g.ny1 : g.nx1, v = d == 2 ? g.ny1 : h->nz + 2;
         hipLaunchKernelGGL(k_periodic, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, h->stream, h->th, g, d);
     }
-    for (int d = 0; d < 3; ++d) {
-        if (h->periodic[d]) continue;
-        const int u = d == 0 ? g.ny : g.nx, v = d == 2 ? g.ny : g.nz;
-        hipLaunchKernelGGL(k_dirichlet, dim3((u + 15) / 16, (v + 15) / 16), b2, 0, h->stream, h->th,
-                           g, d, h->cx, h->cy, h->cz);
-    }
+    launch_dirichlet(h, h->stream);
     HH(hipGetLastError());
     HH(hipStreamSynchronize(h->stream));
     h->steps_done = 0;
