@@ -507,22 +507,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    const int s_raw = first + sl;
-    const bool live = s_raw < n_sys;
-    const int s = live ? s_raw : n_sys - 1;
-    const size_t here0 = cell(g, 1, s % g.ny + 1, s / g.ny + 1);
-    int r0, cnt;
-    {
-        const int base = N / Q, rem = N % Q;
-        r0 = q * base + (q < rem ? q : rem);
-        cnt = base + (q < rem ? 1 : 0);
+    // threads past S * Q (a Thomas tile of fewer than 64 lines is launched as
+    // one full wave, so that its copies use 64 lanes) only copy: the Thomas
+    // solve has no barrier
+    if (q < Q) {
+        const int s_raw = first + sl;
+        const bool live = s_raw < n_sys;
+        const int s = live ? s_raw : n_sys - 1;
+        const size_t here0 = cell(g, 1, s % g.ny + 1, s / g.ny + 1);
+        int r0, cnt;
+        {
+            const int base = N / Q, rem = N % Q;
+            r0 = q * base + (q < rem ? q : rem);
+            cnt = base + (q < rem ? 1 : 0);
+        }
+        const int lt = live ? sl : 0;  // dead lanes read a loaded line
+        double* myline = tile + (size_t)lt * LS + r0;
+        RowAt rows{myline, 1, cnt};
+        adi_solve_lane<M, 2, THOMAS>(
+            N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1), 1, live, red,
+            rows, [&](int j, double x) { myline[j] = x; });
     }
-    const int lt = live ? sl : 0;  // dead lanes read a loaded line
-    double* myline = tile + (size_t)lt * LS + r0;
-    RowAt rows{myline, 1, cnt};
-    adi_solve_lane<M, 2, THOMAS>(
-        N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1), 1, live, red,
-        rows, [&](int j, double x) { myline[j] = x; });
     __syncthreads();
     for (int l = w < nw ? w : nl; l < nl; l += nw) {
         const int s2 = first + l;
@@ -703,7 +708,10 @@ hipError_t launch_adi_t(ftd_heat* h, int d, hipStream_t st) {
     const Dims g = h->g;
     const int S = 1 << f.lgS;
     const int n_sys = d == 0 ? g.ny * h->nz : d == 1 ? g.nx * h->nz : g.nx * g.ny;
-    const dim3 grid((n_sys + S - 1) / S), block(S * f.Q);
+    // an x tile of the Thomas kind with fewer than 64 lines runs as one full
+    // wave: the extra threads take part in the copies only
+    const int T = d == 0 && TH && S * f.Q < 64 ? 64 : S * f.Q;
+    const dim3 grid((n_sys + S - 1) / S), block(T);
     const double sa[3] = {h->s.xa, h->s.ya, h->s.za}, sb[3] = {h->s.xb, h->s.yb, h->s.zb},
                  sc[3] = {h->s.xc, h->s.yc, h->s.zc};
     if (d == 0) {
