@@ -65,6 +65,28 @@ def test_heat_gpu_matches_oracle_and_reference_l2(case):
         assert "%g" % l2 == case["l2"]
 
 
+@pytest.mark.parametrize("n", [100, 128])
+def test_heat_gpu_matches_oracle_at_larger_sizes(n):
+    """Every direction split by the caller (P = 2, so the GPU runs the
+    oracle's partitioning): the field equals the oracle's bit for bit at
+    sizes whose right-hand-side runs are 16 (100^3) and 32 (128^3) planes
+    long and whose x tiles are halved to cover the CUs (100^3)."""
+    parts = (2, 2, 2)
+    dt = dt_policy("order", 0.3, n)[0]
+    note(f"heat pascal n={n} parts={parts} against the oracle")
+    sim = SolveTheta(n, n, n, parts=parts, backend="pascal", dt=dt)
+    try:
+        sim.run(3, use_graph=True)
+        l2 = sim.l2()
+        field = sim.field()
+    finally:
+        sim.close()
+    l2o, ref = O.heat_run(n, n, n, parts, "pascal", dt, 3, sem=O.SEM_GPU, return_theta=True)
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_array_equal(field[inner], ref[inner])
+    assert l2 == l2o
+
+
 @pytest.mark.parametrize("kind,parts,periodic", [
     ("filtered", (1, 1, 4), (0, 0, 0)),
     ("filtered_v2", (2, 2, 2), (0, 0, 0)),
