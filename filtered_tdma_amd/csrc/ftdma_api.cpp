@@ -60,6 +60,7 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t want = 0;  // largest size a sizing pass asked for
 };
 
 }  // namespace
@@ -96,10 +97,15 @@ struct ftd_handle {
     int b_plan_chunk = 0;  // rows per chunk of the planned layout
     DevBuf b_dlanes, b_dlong_off, b_dlong_n;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // sizing pass: the solve paths run without launching anything and only
+    // record the scratch they would use (DevBuf::want)
+    int sizing = 0;
 };
 
 namespace {
 
+// Grows a buffer to `bytes` (contents are not kept). The solve paths call
+// the handle form below, which only records the size during a sizing pass.
 ftd_status ensure(DevBuf& b, size_t bytes) {
     if (b.p && b.bytes >= bytes) return FTD_OK;
     if (b.p) {
@@ -116,6 +122,20 @@ ftd_status ensure(DevBuf& b, size_t bytes) {
     b.bytes = bytes;
     return FTD_OK;
 }
+
+ftd_status ensure(ftd_handle* h, DevBuf& b, size_t bytes) {
+    if (h->sizing) {
+        b.want = std::max(b.want, bytes);
+        return FTD_OK;
+    }
+    return ensure(b, bytes);
+}
+
+// a kernel launch of a solve path: skipped during a sizing pass
+#define FTD_LAUNCH(call)               \
+    do {                               \
+        if (!h->sizing) FTD_HIP(call); \
+    } while (0)
 
 // ---------------- PaScaL / Thomas (exact) ----------------
 
@@ -206,10 +226,10 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
         if (cyclic) {
             std::snprintf(h->path, sizeof h->path, "thomas_cyclic_fused M=%d",
                           ftd::thomas_fused_rows(n_row));
-            FTD_HIP(ftd::launch_thomas_cyclic_fused(A, B, C, D, n_sys, n_row, st));
+            FTD_LAUNCH(ftd::launch_thomas_cyclic_fused(A, B, C, D, n_sys, n_row, st));
         } else {
             std::snprintf(h->path, sizeof h->path, "thomas_fused M=%d", ftd::thomas_fused_rows(n_row));
-            FTD_HIP(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, csh, st));
+            FTD_LAUNCH(ftd::launch_thomas_fused(A, B, C, D, n_sys, n_row, csh, st));
         }
         return FTD_OK;
     }
@@ -222,12 +242,12 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
         if (Ms && ftd::stream_eligible(n_sys, A, B, C, D)) {
             std::snprintf(h->path, sizeof h->path, "pascal_stream M=%d Q=%d R=%d", Ms, fs.Q,
                           ftd::stream_ring_depth(Ms, fs.Q));
-            FTD_HIP(ftd::launch_pascal_stream(Ms, fs.Q, A, B, C, D, n_sys, n_row, cyclic, st));
+            FTD_LAUNCH(ftd::launch_pascal_stream(Ms, fs.Q, A, B, C, D, n_sys, n_row, cyclic, st));
             return FTD_OK;
         }
         std::snprintf(h->path, sizeof h->path, "pascal_fused M=%d T=%d S=%d Q=%d", fs.M, fs.T,
                       fs.S, fs.Q);
-        FTD_HIP(ftd::launch_pascal_fused(fs.M, fs.T, A, B, C, D, n_sys, n_row, fs.Q, fs.S, cyclic,
+        FTD_LAUNCH(ftd::launch_pascal_fused(fs.M, fs.T, A, B, C, D, n_sys, n_row, fs.Q, fs.S, cyclic,
                                          csh, st));
         return FTD_OK;
     }
@@ -241,19 +261,19 @@ ftd_status solve_exact(ftd_handle* h, double* A, double* B, double* C, double* D
     }
     if (P == 1 && !cyclic) {
         std::snprintf(h->path, sizeof h->path, "thomas_general");
-        FTD_HIP(ftd::launch_gen_thomas(A, B, C, D, n_sys, n_row, st));
+        FTD_LAUNCH(ftd::launch_gen_thomas(A, B, C, D, n_sys, n_row, st));
         return FTD_OK;
     }
     if (P == 1) {
-        FTD_TRY(ensure(h->E, sizeof(double) * (size_t)n_row * n_sys));
+        FTD_TRY(ensure(h, h->E, sizeof(double) * (size_t)n_row * n_sys));
         std::snprintf(h->path, sizeof h->path, "thomas_cyclic_general");
-        FTD_HIP(ftd::launch_gen_thomas_cyclic(A, B, C, D, (double*)h->E.p, n_sys, n_row, st));
+        FTD_LAUNCH(ftd::launch_gen_thomas_cyclic(A, B, C, D, (double*)h->E.p, n_sys, n_row, st));
         return FTD_OK;
     }
-    FTD_TRY(ensure(h->R, sizeof(double) * (size_t)6 * P * n_sys));
-    FTD_TRY(ensure(h->E, sizeof(double) * (size_t)2 * P * n_sys));
+    FTD_TRY(ensure(h, h->R, sizeof(double) * (size_t)6 * P * n_sys));
+    FTD_TRY(ensure(h, h->E, sizeof(double) * (size_t)2 * P * n_sys));
     std::snprintf(h->path, sizeof h->path, "pascal_general P=%d", P);
-    FTD_HIP(ftd::launch_gen_pascal(A, B, C, D, n_sys, n_row, P, cyclic, (double*)h->R.p,
+    FTD_LAUNCH(ftd::launch_gen_pascal(A, B, C, D, n_sys, n_row, P, cyclic, (double*)h->R.p,
                                    (double*)h->E.p, st));
     return FTD_OK;
 }
@@ -304,35 +324,35 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
             // network's registers without spilling; shorter partitions take
             // the fused kernel below)
             const size_t nj = (size_t)P * n_sys;
-            FTD_TRY(ensure(h->jsys, sizeof(int) * nj));
+            FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
             h->jsys_count = nj;
             std::snprintf(h->path, sizeof h->path, "filtered_learned_stream M=%d Q=%d R=%d", Ms, P,
                           R);
-            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 3, A, B, C, D, n_sys, n_row, cyclic, Jp,
+            FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 3, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 nullptr, nullptr, h->eps_value, h->eps_direct,
                                                 (int*)h->jsys.p, st));
             return FTD_OK;
         }
         if (version == 2) {
             int nt = 0;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
-            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
-            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+            FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
+            FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
+            FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                           h->eps_direct, csh, Jp, st));
             std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d", Ms, P, R);
-            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
+            FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 nullptr, nullptr, 0.0, 0, nullptr, st));
         } else {
             const int tiles = n_sys / 32;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
-            FTD_TRY(ensure(h->xbuf, xbytes));
+            FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
+            FTD_TRY(ensure(h, h->xbuf, xbytes));
             std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d", Ms, P, R);
-            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
+            FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 (double*)h->xbuf.p, (double*)h->partial.p, 0.0, 0,
                                                 nullptr, st));
-            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
+            FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
                                           h->eps_value, h->eps_direct, csh, Jp, st));
-            FTD_HIP(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
+            FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
                                                 (double*)h->xbuf.p, nullptr, 0.0, 0, nullptr, st));
         }
         return FTD_OK;
@@ -343,11 +363,11 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         // MLP, checked against eps_, truncated correction)
         if (!fs.M) return fail(FTD_ERR_UNSUPPORTED, "learned J needs a fused kernel shape");
         const size_t nj = (size_t)P * n_sys;
-        FTD_TRY(ensure(h->jsys, sizeof(int) * nj));
+        FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
         h->jsys_count = nj;
         std::snprintf(h->path, sizeof h->path, "filtered_learned_fused M=%d T=%d S=%d", fs.M,
                       fs.T, fs.S);
-        FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 3, A, B, C, D, n_sys, n_row, P, fs.S,
+        FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 3, A, B, C, D, n_sys, n_row, P, fs.S,
                                            cyclic, csh, Jp, nullptr, nullptr, h->eps_value,
                                            h->eps_direct, (int*)h->jsys.p, st));
         return FTD_OK;
@@ -356,25 +376,25 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         const int tiles = (n_sys + fs.S - 1) / fs.S;
         if (version == 2) {
             int nt = 0;
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
-            FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
-            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+            FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
+            FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
+            FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                           h->eps_direct, csh, Jp, st));
             std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d", fs.M, fs.T,
                           fs.S);
-            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
+            FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
                                                cyclic, csh, Jp, nullptr, nullptr, 0.0, 0, nullptr, st));
         } else {
-            FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
-            FTD_TRY(ensure(h->xbuf, xbytes));
+            FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
+            FTD_TRY(ensure(h, h->xbuf, xbytes));
             std::snprintf(h->path, sizeof h->path, "filtered_v1_fused M=%d T=%d S=%d", fs.M, fs.T,
                           fs.S);
-            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
+            FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
                                                cyclic, csh, Jp, (double*)h->xbuf.p,
                                                (double*)h->partial.p, 0.0, 0, nullptr, st));
-            FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
+            FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
                                           h->eps_value, h->eps_direct, csh, Jp, st));
-            FTD_HIP(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
+            FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
                                                cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, 0.0, 0,
                                                nullptr, st));
         }
@@ -384,22 +404,22 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     if (csh) return fail(FTD_ERR_UNSUPPORTED, "complex right-hand sides need a fused kernel shape");
     // general path: sweeps in place, interface, J, final
     const int tiles = (n_sys + 255) / 256;
-    FTD_TRY(ensure(h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
-    FTD_TRY(ensure(h->xbuf, xbytes));
+    FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
+    FTD_TRY(ensure(h, h->xbuf, xbytes));
     std::snprintf(h->path, sizeof h->path, "filtered_v%d_general", version);
     int nt = 0;
     if (version == 2) {
-        FTD_HIP(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
-        FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+        FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
+        FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                       h->eps_direct, csh, Jp, st));
     }
-    FTD_HIP(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
-    FTD_HIP(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
+    FTD_LAUNCH(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
+    FTD_LAUNCH(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
                                            (double*)h->partial.p, &nt, st));
     if (version == 1)
-        FTD_HIP(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
+        FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                       h->eps_direct, csh, Jp, st));
-    FTD_HIP(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
+    FTD_LAUNCH(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
     return FTD_OK;
 }
 
@@ -423,6 +443,48 @@ ftd_status do_solve(ftd_handle* h, double* A, double* B, double* C, double* D, i
         FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->last_ms = ms;
     }
+    return FTD_OK;
+}
+
+// Sizes the handle's scratch for every solve that its dimensions, kind,
+// tuning and J mode can take (ftd_solve, ftd_solve_cyclic and
+// ftd_solve_complex, with or without 16-byte aligned arrays, which select
+// the streamed or the fused kernels) and allocates it, so that those solves
+// allocate nothing. Waits for the device before it replaces a buffer that
+// queued work may still use.
+ftd_status reserve(ftd_handle* h) {
+    if (h->n_sys == 0) return FTD_OK;
+    char path[sizeof h->path];
+    std::memcpy(path, h->path, sizeof path);
+    const size_t jcount = h->jsys_count;
+    DevBuf* const bufs[] = {&h->partial, &h->xbuf, &h->R, &h->E, &h->jsys};
+    for (DevBuf* b : bufs) b->want = 0;
+    double* const aligned = reinterpret_cast<double*>(uintptr_t(4096));
+    double* const unaligned = reinterpret_cast<double*>(uintptr_t(4104));
+    const bool filtered = h->kind != FTD_KIND_PASCAL && h->P > 1;
+    const bool complex_ok = h->n_sys <= (1 << 29);
+    h->sizing = 1;
+    for (double* p : {aligned, unaligned})
+        for (int cyclic = 0; cyclic < 2; ++cyclic) {
+            // sizing passes launch nothing; a shape they reject is rejected
+            // again, with its message, by the solve itself
+            if (filtered) {
+                (void)solve_filtered(h, p, p, p, p, cyclic);
+                if (!cyclic && complex_ok) (void)solve_filtered(h, p, p, p, p, 0, 1);
+            } else {
+                (void)solve_exact(h, p, p, p, p, h->n_sys, h->n_row, cyclic);
+                if (!cyclic && complex_ok)
+                    (void)solve_exact(h, p, p, p, p, 2 * h->n_sys, h->n_row, 0, 1);
+            }
+        }
+    h->sizing = 0;
+    std::memcpy(h->path, path, sizeof path);
+    h->jsys_count = jcount;
+    bool grow = false;
+    for (DevBuf* b : bufs) grow = grow || (b->p && b->want > b->bytes);
+    if (grow) FTD_HIP(hipDeviceSynchronize());
+    for (DevBuf* b : bufs)
+        if (b->want) FTD_TRY(ensure(*b, b->want));
     return FTD_OK;
 }
 
@@ -475,6 +537,12 @@ ftd_status ftd_create(ftd_handle** out, int kind, int n_sys, int n_row, int part
     if (e != hipSuccess) {
         ftd_destroy(h);
         return hip_fail(e, "ftd_create");
+    }
+    const ftd_status st = reserve(h);
+    if (st != FTD_OK) {
+        const std::string msg = g_err;
+        ftd_destroy(h);
+        return fail(st, msg);
     }
     *out = h;
     return FTD_OK;
@@ -568,7 +636,7 @@ ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int
         h->t_M = h->t_T = h->t_S = h->t_Q = 0;
         h->force_general = rows_per_lane == -1;
         h->no_stream = rows_per_lane == -2;
-        return FTD_OK;
+        return reserve(h);
     }
     h->force_general = 0;
     h->no_stream = 0;
@@ -582,7 +650,7 @@ ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads, int
     h->t_T = max_threads;
     h->t_S = systems_per_tile;
     h->t_Q = lanes_per_system;
-    return FTD_OK;
+    return reserve(h);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,8 +717,20 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
     const int saved_general = h->force_general, saved_prof = h->profiling;
     std::vector<TuneCand> cands;
     tune_candidates(h, h->n_row, cands);
+    // every solve runs on fresh copies of A, C and D (the general kernels
+    // use A and C as scratch), so the caller's arrays are left as they were
     double* work = nullptr;
-    FTD_HIP(hipMalloc(&work, bytes));
+    FTD_HIP(hipMalloc(&work, 3 * bytes));
+    const size_t nel = bytes / sizeof(double);
+    double* const wa = work;
+    double* const wc = work + nel;
+    double* const wd = work + 2 * nel;
+    auto restore = [&]() {
+        hipError_t e = hipMemcpyAsync(wa, A, bytes, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(wc, C, bytes, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(wd, D, bytes, hipMemcpyDeviceToDevice, h->stream);
+        return e;
+    };
     std::string rep = "{\"candidates\": [";
     int best = -1;
     double best_ms = 0.0;
@@ -658,17 +738,17 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
     h->profiling = 0;
     for (size_t k = 0; k < cands.size(); ++k) {
         apply_cand(h, cands[k]);
-        hipError_t e = hipMemcpyAsync(work, D, bytes, hipMemcpyDeviceToDevice, h->stream);
-        ftd_status st = e == hipSuccess ? do_solve(h, A, B, C, work, h->n_sys, h->n_row, 0)
+        hipError_t e = restore();
+        ftd_status st = e == hipSuccess ? do_solve(h, wa, B, wc, wd, h->n_sys, h->n_row, 0)
                                         : FTD_ERR_HIP;
         if (st != FTD_OK) continue;  // this shape does not hold the solve
         const std::string path = h->path;
         float total = 0.0f;
         for (int r = 0; r < reps && st == FTD_OK; ++r) {
-            e = hipMemcpyAsync(work, D, bytes, hipMemcpyDeviceToDevice, h->stream);
+            e = restore();
             if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
             if (e != hipSuccess) break;
-            st = do_solve(h, A, B, C, work, h->n_sys, h->n_row, 0);
+            st = do_solve(h, wa, B, wc, wd, h->n_sys, h->n_row, 0);
             if (st == FTD_OK) e = hipEventRecord(h->ev1, h->stream);
             if (e == hipSuccess && st == FTD_OK) e = hipEventSynchronize(h->ev1);
             float ms = 0.0f;
@@ -705,6 +785,7 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
         return fail(FTD_ERR_UNSUPPORTED, "no candidate shape holds this solve");
     }
     apply_cand(h, cands[best]);
+    FTD_TRY(reserve(h));
     char buf[256];
     std::snprintf(buf, sizeof buf, "], \"best\": {\"index\": %d, \"ms\": %.6f, \"path\": \"%s\"}}",
                   best, best_ms, best_path.c_str());
@@ -972,7 +1053,7 @@ ftd_status ftd_set_j_mode(ftd_handle* h, int mode) {
     if (!h) return fail(FTD_ERR_INVALID, "null handle");
     if (mode != 0 && mode != 1) return fail(FTD_ERR_INVALID, "J mode must be 0 or 1");
     h->j_mode = mode;
-    return FTD_OK;
+    return reserve(h);
 }
 
 ftd_status ftd_last_J_systems(ftd_handle* h, int* out, int64_t n) {

@@ -24,6 +24,15 @@
  * synchronisation, except ftd_last_J, solves with profiling enabled and
  * ftd_solve_batched with a layout the handle has not planned yet.
  * Calls on one handle must not overlap (the handle owns scratch memory).
+ *
+ * Scratch memory: ftd_create allocates all the device scratch that
+ * ftd_solve, ftd_solve_cyclic, ftd_solve_complex, ftd_solve_f32 and
+ * ftd_solve_f32c of the handle's dimensions can need, for any array
+ * alignment; ftd_set_tuning, ftd_set_j_mode and ftd_autotune size it again
+ * for their new choice (and may wait for the device to replace a buffer).
+ * Those solves therefore allocate nothing, and a handle's first solve may be
+ * captured into a hipGraph. Only ftd_solve_nr with dimensions other than the
+ * handle's and ftd_solve_batched with a layout not planned yet may allocate.
  */
 #ifndef FTDMA_H
 #define FTDMA_H
@@ -145,14 +154,14 @@ ftd_status ftd_set_tuning(ftd_handle* h, int rows_per_lane, int max_threads,
  * automatic choice, the automatic choice without the streamed kernels and
  * every compiled register-resident shape (rows per lane, workgroup size,
  * systems per workgroup, PaScaL sub-partitions) that holds a solve of the
- * handle's dimensions, each `reps` times on a fresh device copy of D (D
- * itself is not written; A and C may be, as by any solve), and keeps the
- * fastest as the handle's tuning (what ftd_set_tuning sets). PaScaL keeps
- * the handle's P partitions when P > 1 and the filtered kinds always keep
- * theirs, so results stay bit-identical; with P == 1 the sub-partition count
- * is tuned. Allocates one n_row x n_sys scratch array for the duration and
- * waits on the host. `report` (optional) receives a JSON text of every
- * candidate's mean time and the choice. */
+ * handle's dimensions, each `reps` times on fresh device copies of A, C and
+ * D (the caller's arrays are not written), and keeps the fastest as the
+ * handle's tuning (what ftd_set_tuning sets). PaScaL keeps the handle's P
+ * partitions when P > 1 and the filtered kinds always keep theirs, so
+ * results stay bit-identical; with P == 1 the sub-partition count is tuned.
+ * Allocates three n_row x n_sys arrays for the duration and waits on the
+ * host. `report` (optional) receives a JSON text of every candidate's mean
+ * time and the choice. */
 ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const double* D,
                         int reps, char* report, size_t report_len);
 

@@ -5,6 +5,11 @@
 // replaced by the number of partitions each system's rows are split into on
 // one GPU. Errors throw std::runtime_error (the reference exits).
 //
+// Like the reference, whose solve always runs the *_profile variants
+// (tdma_backend_gpu.cu:35-46), every solve is timed by default: it waits for
+// its HIP events, and last_gpu_ms() returns its device time.
+// set_profiling(false) drops the events and the wait (last_gpu_ms() is then 0).
+//
 //   TdmaBackendGPU be(TdmaBackendGPU::parse("filtered_v2"), n_sys, n_row,
 //                     /*partitions=*/8, /*eps_constant=*/dt);
 //   be.set_eps_constant(dt);
@@ -34,6 +39,11 @@ public:
         : kind_(kind), partitions_(partitions) {
         check(ftd_create(&h_, static_cast<int>(kind), n_sys, n_row, partitions, eps_constant,
                          stream));
+        const ftd_status s = ftd_set_profiling(h_, 1);
+        if (s != FTD_OK) {
+            ftd_destroy(h_);
+            check(s);
+        }
     }
     ~TdmaBackendGPU() { ftd_destroy(h_); }
     TdmaBackendGPU(const TdmaBackendGPU&) = delete;
@@ -52,6 +62,10 @@ public:
         check(ftd_set_rho(h_, d_A, d_B, d_C));
     }
     void set_eps_constant(double eps_constant) { check(ftd_set_eps_constant(h_, eps_constant)); }
+
+    /// Time every solve (default, as the reference) or not.
+    void set_profiling(bool enable) { check(ftd_set_profiling(h_, enable ? 1 : 0)); }
+    void set_stream(void* stream) { check(ftd_set_stream(h_, stream)); }
 
     Kind kind() const { return kind_; }
     double last_comm_ms() const { return ftd_last_comm_ms(h_); }
