@@ -120,3 +120,50 @@ def test_complex_channel_poisson_modes():
             assert J[P - 1] == 31 and J[2 * P - 1] == 31, J
         # the pinned zero mode
         assert X[0, 0] == 0.0
+
+
+def test_channel_modes_full_size():
+    """The channel pressure systems at full size: 128 x 128 modes, n_row =
+    129, tanh grid (gamma = 2), pinned zero mode, complex128 RHS ~ N(0, 1).
+    pascal (one caller partition), filtered_v2 and learned J (P = 8,
+    eps_constant = 1e-6), each part bit for bit against the oracle's real
+    solves; learned J's per-system J and fallback flags too."""
+    A, B, C, Z = cases.channel_poisson(128, 128, 129, seed=0)
+    P, eps_c = 8, 1e-6
+    note("channel 128x128 modes pascal P=1")
+    X, _, path, _ = run_complex(Kind.PASCAL, A, B, C, Z, 1)
+    for part, x in ((Z.real, X.real), (Z.imag, X.imag)):
+        d = np.ascontiguousarray(part)
+        if path.startswith("thomas"):
+            ref = O.thomas(A, B, C, d, sem=O.SEM_GPU)
+        else:
+            ref = O.pascal(A, B, C, d, path_Q(path), sem=O.SEM_GPU)
+        np.testing.assert_array_equal(x, ref)
+    assert X[0, 0] == 0.0
+    note("channel 128x128 modes filtered_v2 P=8")
+    X, J, path, _ = run_complex(Kind.FILTERED_V2, A, B, C, Z, P, rho=True, eps_constant=eps_c)
+    Jref = []
+    for part, x in ((Z.real, X.real), (Z.imag, X.imag)):
+        ref, Jp = O.filtered(A, B, C, np.ascontiguousarray(part), P, 2, eps_c, sem=O.SEM_GPU,
+                             rho="auto", return_J=True)
+        Jref += list(Jp)
+        np.testing.assert_array_equal(x, ref)
+    assert J == Jref
+    note("channel 128x128 modes learned J P=8")
+    n_row, n_sys = Z.shape
+    be = TdmaBackendGPU(Kind.FILTERED, n_sys, n_row, P, eps_c)
+    try:
+        be.set_j_mode(1)
+        tA, tB, tC = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (A, B, C))
+        tZ = torch.from_numpy(np.ascontiguousarray(Z)).cuda()
+        be.solve_complex(tA, tB, tC, tZ)
+        torch.cuda.synchronize()
+        X = tZ.cpu().numpy()
+        Jl, fb = be.last_J_systems(True)
+    finally:
+        be.close()
+    for par, part, x in ((0, Z.real, X.real), (1, Z.imag, X.imag)):
+        xr, Jr = O.filtered_learned(A, B, C, np.ascontiguousarray(part), P, eps_c)
+        np.testing.assert_array_equal(x, xr)
+        np.testing.assert_array_equal(Jl[:, par::2], Jr & 0xFFFF)
+        np.testing.assert_array_equal(fb[:, par::2], Jr >> 16)

@@ -189,3 +189,25 @@ def test_heat_plans_keep_fused_equal_materialized(n, plans):
     finally:
         for sim in sims:
             sim.close()
+
+
+def test_heat_256_adi_step_matches_oracle():
+    """The 256^3 ADI step of the uneven-batch/ADI workload with the automatic
+    plans (one caller partition per direction; lines of 256 rows solved in
+    8 internal partitions of 32 rows, x tiles and right-hand-side runs sized
+    for the device), three steps from a hipGraph, against the oracle with
+    the same partitioning (parts = (8, 8, 8)): field and L2 bit for bit."""
+    n, dt, steps = 256, 1e-4, 3
+    note(f"heat 256^3 automatic plans vs oracle parts=(8, 8, 8), {steps} graph steps")
+    sim = SolveTheta(n, n, n, parts=(1, 1, 1), backend="pascal", dt=dt)
+    try:
+        sim.run(steps, use_graph=True)
+        l2 = sim.l2()
+        field = sim.field()
+    finally:
+        sim.close()
+    l2o, ref = O.heat_run(n, n, n, (8, 8, 8), "pascal", dt, steps, sem=O.SEM_GPU,
+                          return_theta=True, nthreads=16)
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_array_equal(field[inner], ref[inner])
+    assert l2 == l2o

@@ -421,3 +421,41 @@ def test_f32_storage_bitwise_vs_oracle_on_widened_data(n_sys, n_row, P, d32):
     np.testing.assert_array_equal(x, ref)
     assert relerr(x, O.thomas(A32.astype(np.float64), B32.astype(np.float64),
                               C32.astype(np.float64), Dw)) <= 1e-12
+
+
+@pytest.mark.parametrize("d32", [False, True])
+def test_f32_storage_full_size_sampled_columns(d32):
+    """FP32 storage at the full 262144 x 512, P = 8 shape with r ~ U[5, 500]
+    (the streamed FP32 kernel walks many tiles per workgroup): 2048 sampled
+    systems bit for bit against the oracle's FP64 PaScaL of the widened
+    stored data, and within 1e-12 of its serial Thomas."""
+    n_sys, n_row, P = 262144, 512, 8
+    note(f"f32 storage full size d32={d32}")
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 5.0, 500.0, seed=0)
+    A32, B32, C32 = (x.astype(np.float32) for x in (A, B, C))
+    D32 = D.astype(np.float32) if d32 else None
+    del A, B, C
+    be = TdmaBackendGPU("pascal", n_sys, n_row, P)
+    try:
+        tA, tB, tC = (torch.from_numpy(x).cuda() for x in (A32, B32, C32))
+        if d32:
+            tD = torch.from_numpy(D32).cuda()
+            tX = torch.empty((n_row, n_sys), dtype=torch.float64, device="cuda")
+            be.solve_f32(tA, tB, tC, tD, tX)
+        else:
+            tX = dev(D)
+            be.solve_f32c(tA, tB, tC, tX)
+        torch.cuda.synchronize()
+        path = be.last_path()
+        cols = np.unique(np.concatenate([np.arange(0, n_sys, 131), np.arange(n_sys - 32, n_sys)]))
+        x = tX.cpu().numpy()[:, cols]
+        del tA, tB, tC, tX
+    finally:
+        be.close()
+    if d32:
+        assert path.startswith("pascal_stream_f32"), path
+    sub = [np.ascontiguousarray(a[:, cols]).astype(np.float64) for a in (A32, B32, C32)]
+    Dw = np.ascontiguousarray((D32 if d32 else D)[:, cols]).astype(np.float64)
+    ref = O.pascal(*sub, Dw, path_Q(path), sem=O.SEM_GPU)
+    np.testing.assert_array_equal(x, ref)
+    assert relerr(x, O.thomas(*sub, Dw)) <= 1e-12
