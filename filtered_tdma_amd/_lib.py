@@ -19,6 +19,7 @@ EXPORTS = (
     "ftd_last_path", "ftd_last_error_string", "ftd_solve_f32c", "ftd_solve_f32",
     "ftd_solve_batched", "ftd_solve_batched_f32", "ftd_set_batch_chunk", "ftd_autotune_batched",
     "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
+    "ftd_solve_static",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
     "ftd_heat_set_plan",
@@ -78,6 +79,8 @@ def load() -> ctypes.CDLL:
     L.ftd_set_j_mode.argtypes = [vp, c_int]
     L.ftd_autotune.argtypes = [vp, dp, dp, dp, dp, c_int, ctypes.c_char_p, ctypes.c_size_t]
     L.ftd_last_J_systems.argtypes = [vp, ip, ctypes.c_int64]
+    L.ftd_solve_static.argtypes = [c_int, c_int, c_int, c_int, c_double, c_int, dp, dp, dp, dp,
+                                   vp]
     L.ftd_last_path.argtypes = [vp]
     L.ftd_last_path.restype = ctypes.c_char_p
     L.ftd_last_error_string.restype = ctypes.c_char_p
@@ -116,7 +119,8 @@ def load() -> ctypes.CDLL:
                  "ftd_set_profiling", "ftd_last_J", "ftd_set_tuning", "ftd_solve_f32c",
                  "ftd_solve_f32", "ftd_solve_batched", "ftd_solve_batched_f32",
                  "ftd_set_batch_chunk", "ftd_autotune_batched",
-                 "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune"):
+                 "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
+                 "ftd_solve_static"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

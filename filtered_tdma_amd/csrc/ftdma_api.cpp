@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1065,6 +1066,116 @@ ftd_status ftd_last_J_systems(ftd_handle* h, int* out, int64_t n) {
                            h->stream));
     FTD_HIP(hipStreamSynchronize(h->stream));
     return FTD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ftd_solve_static: the solve of a handle's kind with its scratch in a static
+// device array instead of hipMalloc'd buffers, for callers that cannot
+// allocate. The array is a host-side stand-in in host code: its device
+// address is taken once per device with hipGetSymbolAddress, and only that
+// address is used.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+constexpr size_t kStaticScratch = (size_t)FTD_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once)
+ftd_status static_scratch(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(FTD_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(FTD_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch_ptr[dev];
+    return FTD_OK;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+ftd_status ftd_solve_static(int kind, int n_sys, int n_row, int partitions, double eps_constant,
+                            int cyclic, double* A, double* B, double* C, double* D,
+                            void* stream) {
+    if (kind < FTD_KIND_FILTERED || kind > FTD_KIND_PASCAL)
+        return fail(FTD_ERR_INVALID, "kind must be FTD_KIND_FILTERED, _FILTERED_V2 or _PASCAL");
+    if (n_sys < 0 || n_row < 1 || partitions < 1)
+        return fail(FTD_ERR_INVALID, "need n_sys >= 0, n_row >= 1, partitions >= 1");
+    if (partitions > n_row) return fail(FTD_ERR_INVALID, "more partitions than rows");
+    if (!A || !B || !C || !D) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (n_sys == 0) return FTD_OK;
+    // a handle on the stack: its buffers will point into the static array
+    ftd_handle h;
+    h.kind = kind;
+    h.n_sys = n_sys;
+    h.n_row = n_row;
+    h.P = partitions;
+    h.eps_value = eps_constant;
+    h.stream = (hipStream_t)stream;
+    const bool filtered = kind != FTD_KIND_PASCAL && partitions > 1;
+    // what this solve would use, for these arrays (their alignment picks the
+    // streamed or the fused kernels)
+    h.sizing = 1;
+    const ftd_status sz = filtered ? solve_filtered(&h, A, B, C, D, cyclic ? 1 : 0)
+                                   : solve_exact(&h, A, B, C, D, n_sys, n_row, cyclic ? 1 : 0);
+    h.sizing = 0;
+    if (sz != FTD_OK) return sz;
+    const size_t rho_b = round256(sizeof(double) * 2 * ((size_t)n_row + 1));
+    const size_t J_b = round256(sizeof(int) * 2 * (size_t)partitions);
+    DevBuf* const bufs[] = {&h.partial, &h.xbuf, &h.R, &h.E};
+    size_t total = (filtered ? rho_b + J_b : 0);
+    for (DevBuf* b : bufs) total += round256(b->want);
+    if (total > kStaticScratch)
+        return fail(FTD_ERR_UNSUPPORTED,
+                    "ftd_solve_static needs " + std::to_string(total) +
+                        " bytes of scratch for this solve; the static buffer holds " +
+                        std::to_string(kStaticScratch) + " (FTD_STATIC_SCRATCH_BYTES)");
+    int dev = 0;
+    FTD_HIP(hipGetDevice(&dev));
+    unsigned char* base = nullptr;
+    FTD_TRY(static_scratch(dev, &base));
+    size_t pos = 0;
+    if (filtered) {
+        h.rho = (double*)(base + pos);
+        pos += rho_b;
+        h.J = (int*)(base + pos);
+        pos += J_b;
+    }
+    for (DevBuf* b : bufs) {
+        if (!b->want) continue;
+        b->p = base + pos;
+        b->bytes = b->want;
+        pos += round256(b->want);
+    }
+    // ftd_set_rho, then the solve; every scratch value a kernel reads is
+    // written earlier in the same call (column-0 bounds, J, tile maxima,
+    // boundary values), so nothing needs clearing
+    ftd_status st = FTD_OK;
+    if (filtered) {
+        const hipError_t e = ftd::launch_set_rho(A, B, C, n_sys, n_row, h.rho, h.stream);
+        if (e != hipSuccess) st = hip_fail(e, "set_rho");
+    }
+    if (st == FTD_OK)
+        st = filtered ? solve_filtered(&h, A, B, C, D, cyclic ? 1 : 0)
+                      : solve_exact(&h, A, B, C, D, n_sys, n_row, cyclic ? 1 : 0);
+    // the buffers are the static array's: the handle must not free them
+    for (DevBuf* b : bufs) b->p = nullptr;
+    h.rho = nullptr;
+    h.J = nullptr;
+    return st;
 }
 
 const char* ftd_last_path(const ftd_handle* h) { return h ? h->path : "none"; }

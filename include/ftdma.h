@@ -237,6 +237,29 @@ ftd_status ftd_autotune_batched(ftd_handle* h, const int64_t* row_offsets, const
                                 int n_systems, double* A, double* B, const double* C,
                                 const double* D, int reps, char* report, size_t report_len);
 
+/* Size of the static device scratch array behind ftd_solve_static, per GPU. */
+#define FTD_STATIC_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one solve with no handle and no allocation, equal bit
+ * for bit to ftd_create(kind, n_sys, n_row, partitions, eps_constant,
+ * stream) + ftd_set_rho(A, B, C) + ftd_solve (cyclic == 0) or
+ * ftd_solve_cyclic (cyclic != 0) + ftd_destroy on the same arrays. The
+ * solve's scratch (column-0 bounds, J per partition, per-tile maxima,
+ * boundary values, reduced rows) lives in one static device array of
+ * FTD_STATIC_SCRATCH_BYTES per GPU, held from the library's load to the
+ * process's end; a solve needing more returns FTD_ERR_UNSUPPORTED with the
+ * size in ftd_last_error_string. Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race
+ * and give undefined results. Nothing in the array needs to start at zero
+ * (every value a kernel reads is written earlier in the same call).
+ * Ordered on `stream` (a hipStream_t, NULL = the null stream); no host
+ * synchronisation. */
+ftd_status ftd_solve_static(int kind, int n_sys, int n_row, int partitions, double eps_constant,
+                            int cyclic, double* A, double* B, double* C, double* D,
+                            void* stream);
+
 /* Extension (diagnostics): the path the last solve took, as a short
  * static string, e.g. "pascal_stream M=64 Q=8 R=16",
  * "pascal_fused M=32 T=256 S=16 Q=16" or "filtered_v1_general". */

@@ -81,3 +81,16 @@ def test_cpp_wrapper_compiles():
                           os.path.join(ROOT, "include"), "-"], input=src, capture_output=True,
                          text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_static_solve_rejects_oversize_before_any_device_call(lib):
+    """ftd_solve_static sizes its scratch on the host and refuses a solve that
+    exceeds the static array before touching the device (the pointers below
+    are never dereferenced)."""
+    fake = ctypes.c_void_p(1 << 20)
+    st = lib.ftd_solve_static(0, 600000, 512, 16, 1e-3, 0, fake, fake, fake, fake, None)
+    assert st == 2
+    msg = lib.ftd_last_error_string().decode()
+    assert "bytes of scratch" in msg and str(64 << 20) in msg, msg
+    assert lib.ftd_solve_static(7, 10, 10, 1, 0.0, 0, fake, fake, fake, fake, None) == 1
+    assert lib.ftd_solve_static(2, 10, 10, 1, 0.0, 0, None, fake, fake, fake, None) == 1
