@@ -443,6 +443,187 @@ __device__ __forceinline__ void thomas_item(const Stage& st, const TS* A, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reduced systems in LDS (B == 1): rows r of a system's reduced system are
+// (rA[r], 1, rC[r] | rD[r]). Statement for statement the oracle's
+// thomas_cols / modthomas_cols / update_cols with B == 1 (rows 0 and 1 of a
+// group "divided by b = 1" are left as they are, which is exact).
+// ---------------------------------------------------------------------------
+
+// para_range over [0, n) in P parts
+__device__ __forceinline__ void split_rows(int n, int P, int p, int& s, int& m) {
+    const int base = n / P, rem = n % P;
+    s = p * base + (p < rem ? p : rem);
+    m = base + (p < rem ? 1 : 0);
+}
+
+// groups of a reduced system of NR rows: 1 below 16 rows, else the largest
+// G with 2 G^2 <= NR (groups of at least 2G >= 4 rows)
+__host__ __device__ __forceinline__ int red_groups(int NR) {
+    if (NR < 16) return 1;
+    int G = 1;
+    while (2 * (G + 1) * (G + 1) <= NR) ++G;
+    return G;
+}
+
+// Thomas on rows r0 .. r0 + NR - 1, x into rD; the rows' values are read
+// into registers 8 at a time, so the LDS latency is paid once per 8 rows
+// and the serial chain is the arithmetic only
+__device__ __forceinline__ void red_thomas(double* rA, double* rC, double* rD, int r0, int NR) {
+    double cp = rC[r0], dp = rD[r0];
+    int k = 1;
+    for (; k + 8 <= NR; k += 8) {
+        double ak[8], ck[8], dk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ak[u] = rA[r0 + k + u];
+            ck[u] = rC[r0 + k + u];
+            dk[u] = rD[r0 + k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double inv = 1.0 / (1.0 - ak[u] * cp);
+            dp = inv * (dk[u] - ak[u] * dp);
+            cp = inv * ck[u];
+            rD[r0 + k + u] = dp;
+            rC[r0 + k + u] = cp;
+        }
+    }
+    for (; k < NR; ++k) {
+        const double ak = rA[r0 + k];
+        const double inv = 1.0 / (1.0 - ak * cp);
+        dp = inv * (rD[r0 + k] - ak * dp);
+        cp = inv * rC[r0 + k];
+        rD[r0 + k] = dp;
+        rC[r0 + k] = cp;
+    }
+    double dn = dp;
+    k = NR - 2;
+    for (; k >= 7; k -= 8) {
+        double ck[8], dk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ck[u] = rC[r0 + k - u];
+            dk[u] = rD[r0 + k - u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            dn = dk[u] - ck[u] * dn;
+            rD[r0 + k - u] = dn;
+        }
+    }
+    for (; k >= 0; --k) {
+        dn = rD[r0 + k] - rC[r0 + k] * dn;
+        rD[r0 + k] = dn;
+    }
+}
+
+// modified Thomas of one group of m >= 4 rows at b (pascal_tdma.cpp:131-183
+// with B == 1): forward j = 2..m-1, backward j = m-3..1, then the row-0
+// reduction; the rows' values are read 4 at a time ahead of their use
+__device__ __forceinline__ void red_modthomas(double* rA, double* rC, double* rD, int b, int m) {
+    double am = rA[b + 1], cm = rC[b + 1], dm = rD[b + 1];
+    int j = 2;
+    for (; j + 4 <= m; j += 4) {
+        double aj[4], cj[4], dj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            aj[u] = rA[b + j + u];
+            cj[u] = rC[b + j + u];
+            dj[u] = rD[b + j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double inv = 1.0 / (1.0 - aj[u] * cm);
+            dm = inv * (dj[u] - aj[u] * dm);
+            cm = inv * cj[u];
+            am = -inv * aj[u] * am;
+            rD[b + j + u] = dm;
+            rC[b + j + u] = cm;
+            rA[b + j + u] = am;
+        }
+    }
+    for (; j < m; ++j) {
+        const double aj = rA[b + j], cj = rC[b + j], dj = rD[b + j];
+        const double inv = 1.0 / (1.0 - aj * cm);
+        dm = inv * (dj - aj * dm);
+        cm = inv * cj;
+        am = -inv * aj * am;
+        rD[b + j] = dm;
+        rC[b + j] = cm;
+        rA[b + j] = am;
+    }
+    double dn = rD[b + m - 2], an = rA[b + m - 2], cn = rC[b + m - 2];
+    j = m - 3;
+    for (; j >= 4; j -= 4) {
+        double aj[4], cj[4], dj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            aj[u] = rA[b + j - u];
+            cj[u] = rC[b + j - u];
+            dj[u] = rD[b + j - u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            dn = dj[u] - cj[u] * dn;
+            an = aj[u] - cj[u] * an;
+            cn = -cj[u] * cn;
+            rD[b + j - u] = dn;
+            rA[b + j - u] = an;
+            rC[b + j - u] = cn;
+        }
+    }
+    for (; j >= 1; --j) {
+        const double cj = rC[b + j];
+        dn = rD[b + j] - cj * dn;
+        an = rA[b + j] - cj * an;
+        cn = -cj * cn;
+        rD[b + j] = dn;
+        rA[b + j] = an;
+        rC[b + j] = cn;
+    }
+    const double c0 = rC[b], c1 = rC[b + 1], d1 = rD[b + 1], a1 = rA[b + 1];
+    const double r = 1.0 / (1.0 - a1 * c0);
+    rD[b] = r * (rD[b] - c0 * d1);
+    rA[b] = r * rA[b];
+    rC[b] = -r * c0 * c1;
+}
+
+// Thomas on the 2G rows of the groups' ends (row 0 and row m - 1 of each
+// group of the NR rows at r0), gathered by index; x into their rD
+__device__ __forceinline__ void red_thomas_ends(double* rA, double* rC, double* rD, int r0, int NR,
+                                                int G) {
+    auto row = [&](int k) {
+        int s, m;
+        split_rows(NR, G, k >> 1, s, m);
+        return r0 + s + ((k & 1) ? m - 1 : 0);
+    };
+    int rp = row(0);
+    double cp = rC[rp], dp = rD[rp];
+    for (int k = 1; k < 2 * G; ++k) {
+        const int rk = row(k);
+        const double ak = rA[rk];
+        const double inv = 1.0 / (1.0 - ak * cp);
+        dp = inv * (rD[rk] - ak * dp);
+        cp = inv * rC[rk];
+        rD[rk] = dp;
+        rC[rk] = cp;
+    }
+    double dn = dp;
+    for (int k = 2 * G - 2; k >= 0; --k) {
+        const int rk = row(k);
+        dn = rD[rk] - rC[rk] * dn;
+        rD[rk] = dn;
+    }
+}
+
+// interior update of one group: x_j = d_j - (a_j x_0 + c_j x_{m-1}), j = 1..m-2
+__device__ __forceinline__ void red_update(const double* rA, const double* rC, double* rD, int b,
+                                           int m) {
+    const double x0 = rD[b], xN = rD[b + m - 1];
+    for (int j = 1; j < m - 1; ++j) rD[b + j] = rD[b + j] - (rA[b + j] * x0 + rC[b + j] * xN);
+}
+
 // one chunk of a chunked system: modified Thomas (rows 0, 1 divided by b)
 template <int M>
 struct PascalStep {
@@ -506,59 +687,29 @@ __device__ __forceinline__ void pascal_item(const Stage& st, const TS* A, const 
         rD[2 * t + 1] = s.dN;
     }
     group_lds_sync();
-    if (active && q == 0) {
-        // reduced system of this system: rows 2*lead .. 2*(lead+Q)-1 (up to
-        // 256), taken 8 rows at a time: the rows' values are read into
-        // registers first, so the LDS latency is paid once per 8 rows and
-        // the serial chain is the arithmetic only
-        const int r0 = 2 * lead, NR = 2 * Q;
-        double cp = rC[r0], dp = rD[r0];
-        int k = 1;
-        for (; k + 8 <= NR; k += 8) {
-            double ak[8], ck[8], dk[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ak[u] = rA[r0 + k + u];
-                ck[u] = rC[r0 + k + u];
-                dk[u] = rD[r0 + k + u];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double inv = 1.0 / (1.0 - ak[u] * cp);
-                dp = inv * (dk[u] - ak[u] * dp);
-                cp = inv * ck[u];
-                rD[r0 + k + u] = dp;
-                rC[r0 + k + u] = cp;
-            }
-        }
-        for (; k < NR; ++k) {
-            const double ak = rA[r0 + k];
-            const double inv = 1.0 / (1.0 - ak * cp);
-            dp = inv * (rD[r0 + k] - ak * dp);
-            cp = inv * rC[r0 + k];
-            rD[r0 + k] = dp;
-            rC[r0 + k] = cp;
-        }
-        double dn = dp;
-        k = NR - 2;
-        for (; k >= 7; k -= 8) {
-            double ck[8], dk[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ck[u] = rC[r0 + k - u];
-                dk[u] = rD[r0 + k - u];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                dn = dk[u] - ck[u] * dn;
-                rD[r0 + k - u] = dn;
-            }
-        }
-        for (; k >= 0; --k) {
-            dn = rD[r0 + k] - rC[r0 + k] * dn;
-            rD[r0 + k] = dn;
-        }
+    // The reduced system of this system, rows r0 .. r0 + NR - 1 of the
+    // item's LDS (NR = 2Q, up to 256). Short ones (G == 1): Thomas in the
+    // lane of chunk 0. Longer ones: PaScaL over G groups of rows, one lane
+    // each (modified Thomas in LDS), the 2G rows of the groups' ends by
+    // Thomas in the lane of chunk 0, then each group's interior update --
+    // a serial chain of about NR / G + 2G rows instead of NR.
+    const int NR = 2 * Q, r0 = 2 * lead;
+    const int G = red_groups(NR);
+    const bool grouped = active && G > 1 && q < G;
+    int gs = 0, gm = 0;
+    if (grouped) {
+        split_rows(NR, G, q, gs, gm);
+        red_modthomas(rA, rC, rD, r0 + gs, gm);
     }
+    group_lds_sync();
+    if (active && q == 0) {
+        if (G == 1)
+            red_thomas(rA, rC, rD, r0, NR);
+        else
+            red_thomas_ends(rA, rC, rD, r0, NR, G);
+    }
+    group_lds_sync();
+    if (grouped) red_update(rA, rC, rD, r0 + gs, gm);
     group_lds_sync();
     vm_drain();  // the next item's leading blocks, before this item's stores
     if (active) {

@@ -316,8 +316,63 @@ static int check_parts(int n_row, int P) {
     return 0;
 }
 
+/* The 2P-row reduced system solved itself by PaScaL over G groups of rows
+ * (para_range split, B == 1): modified Thomas per group, the 2G-row system
+ * of the groups' end rows by Thomas, then the groups' interior update --
+ * the order of the uneven-batch kernel's reduced solve for long systems
+ * (ftdma_batched.hip). A2 holds 4 arrays of 2G rows and R2 6G rows. */
+static void reduced_nested_cols(double* Ar, double* Br, double* Cr, double* Dr, size_t ld,
+                                int c0, int c1, int n_rd, int G, int sem, double* A2,
+                                double* R2) {
+    double* B2 = A2 + ld * (size_t)(2 * G);
+    double* C2 = B2 + ld * (size_t)(2 * G);
+    double* D2 = C2 + ld * (size_t)(2 * G);
+    for (int g = 0; g < G; ++g) {
+        int s, m;
+        part_rows(n_rd, G, g, &s, &m);
+        double* R = R2 + (size_t)6 * g * ld;
+        modthomas_cols(ROW(Ar, s), ROW(Br, s), ROW(Cr, s), ROW(Dr, s), ld, c0, c1, m, sem, R);
+        for (int i = c0; i < c1; ++i) {
+            A2[(size_t)(2 * g) * ld + i] = R[0 * ld + i];
+            C2[(size_t)(2 * g) * ld + i] = R[1 * ld + i];
+            D2[(size_t)(2 * g) * ld + i] = R[2 * ld + i];
+            A2[(size_t)(2 * g + 1) * ld + i] = R[3 * ld + i];
+            C2[(size_t)(2 * g + 1) * ld + i] = R[4 * ld + i];
+            D2[(size_t)(2 * g + 1) * ld + i] = R[5 * ld + i];
+            B2[(size_t)(2 * g) * ld + i] = 1.0;
+            B2[(size_t)(2 * g + 1) * ld + i] = 1.0;
+        }
+    }
+    thomas_cols(A2, B2, C2, D2, ld, c0, c1, 2 * G, sem);
+    for (int g = 0; g < G; ++g) {
+        int s, m;
+        part_rows(n_rd, G, g, &s, &m);
+        update_cols(ROW(Ar, s), ROW(Cr, s), ROW(Dr, s), ld, c0, c1, m,
+                    D2 + (size_t)(2 * g) * ld, D2 + (size_t)(2 * g + 1) * ld);
+    }
+}
+
+static int pascal_impl(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
+                       int P, int cyclic, int sem, int nthreads, int G);
+
 int ftdo_pascal(double* A, const double* B, double* C, double* D,
                 int n_sys, int n_row, int P, int cyclic, int sem, int nthreads) {
+    return pascal_impl(A, B, C, D, n_sys, n_row, P, cyclic, sem, nthreads, 1);
+}
+
+int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
+                       int n_sys, int n_row, int P, int G, int sem, int nthreads) {
+    if (G < 1) return fail(-1, "pascal_nested: need G >= 1");
+    if (G > 1 && P > 1) {
+        int s, m;
+        part_rows(2 * P, G, G - 1, &s, &m);
+        if (m < 3) return fail(-2, "pascal_nested: every group needs at least 3 reduced rows");
+    }
+    return pascal_impl(A, B, C, D, n_sys, n_row, P, 0, sem, nthreads, G);
+}
+
+static int pascal_impl(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
+                       int P, int cyclic, int sem, int nthreads, int G) {
     int rc = check_parts(n_row, P);
     if (rc) return rc;
     if (P == 1) {
@@ -327,13 +382,18 @@ int ftdo_pascal(double* A, const double* B, double* C, double* D,
     }
     const size_t ld = (size_t)n_sys;
     const int n_rd = 2 * P;
-    double* buf = (double*)malloc(sizeof(double) * ld * (size_t)(4 * n_rd + 6 * P) + 8);
+    const int nested = G > 1 && !cyclic;
+    const size_t extra = nested ? (size_t)(8 * G + 6 * G) : 0;
+    double* buf =
+        (double*)malloc(sizeof(double) * ld * ((size_t)(4 * n_rd + 6 * P) + extra) + 8);
     if (!buf) return fail(-3, "pascal: out of memory");
     double* Ar = buf;
     double* Br = Ar + ld * n_rd;
     double* Cr = Br + ld * n_rd;
     double* Dr = Cr + ld * n_rd;
     double* Rp = Dr + ld * n_rd; /* 6 rows per partition */
+    double* A2 = Rp + ld * (size_t)(6 * P); /* nested: 4 arrays of 2G rows, then 6G rows */
+    double* R2 = A2 + ld * (size_t)(8 * G);
     set_threads(nthreads);
     const int nc = nchunks(n_sys);
 #pragma omp parallel for schedule(static)
@@ -357,7 +417,10 @@ int ftdo_pascal(double* A, const double* B, double* C, double* D,
         }
         /* the cyclic reduced solve needs an E scratch of 2P rows: reuse Rp
          * (6P >= 2P rows), which is no longer read after the copy above */
-        reduced_cols(Ar, Br, Cr, Dr, Rp, ld, c0, c1, n_rd, cyclic, sem);
+        if (nested)
+            reduced_nested_cols(Ar, Br, Cr, Dr, ld, c0, c1, n_rd, G, sem, A2, R2);
+        else
+            reduced_cols(Ar, Br, Cr, Dr, Rp, ld, c0, c1, n_rd, cyclic, sem);
         for (int p = 0; p < P; ++p) {
             int s, m;
             part_rows(n_row, P, p, &s, &m);

@@ -46,6 +46,12 @@ int ftdo_thomas_cyclic(double* A, const double* B, double* C, double* D,
 int ftdo_pascal(double* A, const double* B, double* C, double* D,
                 int n_sys, int n_row, int P, int cyclic, int sem, int nthreads);
 
+/* PaScaL with P partitions whose 2P-row reduced system is solved by PaScaL
+ * itself over G groups of rows (G = 1: Thomas, as ftdo_pascal); the order
+ * of the uneven-batch kernel's reduced solve. Non-periodic. */
+int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
+                       int n_sys, int n_row, int P, int G, int sem, int nthreads);
+
 /* Column-0 spectral bounds, n_row+1 entries each (tdma_backend.cpp:33-44). */
 void ftdo_set_rho(const double* A, const double* B, const double* C,
                   int n_sys, int n_row, double* A_rho, double* C_rho);

@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
         L.ftdo_thomas.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int]
         L.ftdo_thomas_cyclic.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int]
         L.ftdo_pascal.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int, c_int]
+        L.ftdo_pascal_nested.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int,
+                                         c_int]
         L.ftdo_set_rho.argtypes = [dp, dp, dp, c_int, c_int, dp, dp]
         L.ftdo_set_rho.restype = None
         L.ftdo_cal_J.argtypes = [c_double, c_double, c_double, c_int, c_int]
@@ -61,7 +63,8 @@ def lib() -> ctypes.CDLL:
         L.ftdo_mlp_J.argtypes = [fp, c_int, c_int, fp]
         L.ftdo_lg.argtypes = [c_double]
         L.ftdo_lg.restype = ctypes.c_float
-        for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_cal_J,
+        for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_pascal_nested,
+                  L.ftdo_cal_J,
                   L.ftdo_filtered, L.ftdo_filtered_learned, L.ftdo_mlp_J):
             f.restype = c_int
         _lib = L
@@ -99,6 +102,15 @@ def pascal(A, B, C, D, P, sem=SEM_GPU, cyclic=False, nthreads=0, inplace=False):
     (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
     _check(lib().ftdo_pascal(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, int(cyclic), sem,
                              nthreads))
+    return D
+
+
+def pascal_nested(A, B, C, D, P, G, sem=SEM_GPU, nthreads=0, inplace=False):
+    """PaScaL with P partitions whose 2P-row reduced system is solved by
+    PaScaL over G groups of rows (G = 1: Thomas, the same as pascal)."""
+    (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
+    _check(lib().ftdo_pascal_nested(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, G, sem,
+                                    nthreads))
     return D
 
 

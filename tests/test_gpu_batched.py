@@ -2,7 +2,9 @@
 
 A system of N <= 48 rows is plain Thomas in one lane, a longer one (up to
 128 * 48 = 6144 rows) PaScaL with Q = ceil(N/48) partitions inside one
-128-lane work item, a longer one still the lane-per-system Thomas kernel. Each is compared bit for bit
+128-lane work item (its 2Q-row reduced system solved by PaScaL over groups
+of rows once it has 16 rows or more), a longer one still the lane-per-system
+Thomas kernel. Each is compared bit for bit
 with the oracle's solve of the same partitioning (GPU semantics), and with
 serial Thomas within 1e-13 relative."""
 import sys
@@ -45,8 +47,21 @@ def general_uneven(n_rows, seed, gap):
 CHUNK = 48  # rows per lane of the batched kernel (the default)
 
 
+def red_groups(NR):
+    """Groups the kernel solves a reduced system of NR rows in: 1 below 16
+    rows, else the largest G with 2 G^2 <= NR."""
+    if NR < 16:
+        return 1
+    G = 1
+    while 2 * (G + 1) ** 2 <= NR:
+        G += 1
+    return G
+
+
 def oracle_solve(off, n_rows, A, B, C, D, chunk=CHUNK):
-    """x of every system by the oracle, with the engine's partitioning."""
+    """x of every system by the oracle, with the engine's partitioning:
+    Q = ceil(N / chunk) chunks, the 2Q-row reduced system in red_groups(2Q)
+    groups."""
     X = D.copy()
     for s in range(n_rows.size):
         o, n = int(off[s]), int(n_rows[s])
@@ -56,7 +71,7 @@ def oracle_solve(off, n_rows, A, B, C, D, chunk=CHUNK):
         if Q == 1 or Q > 128:
             x = O.thomas(a, b, c, d, sem=O.SEM_GPU)
         else:
-            x = O.pascal(a, b, c, d, Q, sem=O.SEM_GPU)
+            x = O.pascal_nested(a, b, c, d, Q, red_groups(2 * Q), sem=O.SEM_GPU)
         X[sl] = x[:, 0]
     return X
 

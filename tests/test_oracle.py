@@ -164,6 +164,23 @@ def test_pascal_matches_thomas_any_P(P, cyclic):
         np.testing.assert_array_equal(x, ref)
 
 
+@pytest.mark.parametrize("P,G", [(8, 2), (20, 4), (86, 9), (128, 8)])
+def test_pascal_nested_matches_banded_and_reduces_to_pascal(P, G):
+    """The reduced system solved by PaScaL over G groups (the uneven-batch
+    kernel's order for long systems) is still an exact solve: within 1e-13
+    of LAPACK's banded solver; with G = 1 it is PaScaL bit for bit."""
+    A, B, C, D = cases.general_batch(20, 48 * P, seed=13)
+    x = O.pascal_nested(A, B, C, D, P, G)
+    assert relerr(x, banded_solve(A, B, C, D)) <= 1e-13
+    np.testing.assert_array_equal(O.pascal_nested(A, B, C, D, P, 1), O.pascal(A, B, C, D, P))
+
+
+def test_pascal_nested_rejects_short_groups():
+    A, B, C, D = cases.general_batch(4, 40, seed=1)
+    with pytest.raises(RuntimeError):
+        O.pascal_nested(A, B, C, D, 2, 2)  # 4 reduced rows in 2 groups of 2
+
+
 def test_pascal_rejects_short_partitions():
     A, B, C, D = cases.general_batch(4, 10, seed=1)
     with pytest.raises(RuntimeError):
