@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--kinds", default="pascal,filtered_v2,filtered,filtered_learned")
+    ap.add_argument("--kinds", default="pascal,filtered_v2,filtered,filtered_learned,"
+                    "filtered_v2_guard,filtered_guard,filtered_learned_guard")
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
     ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 heat (C1) leg")
     ap.add_argument("--no-c4", action="store_true",
@@ -48,6 +49,8 @@ def parse():
                     help="skip the complex channel-Poisson (C5) leg")
     ap.add_argument("--no-f4", action="store_true",
                     help="skip the FFT pressure-step (F4) leg")
+    ap.add_argument("--no-long", action="store_true",
+                    help="skip the 262144 x 1024, P = 2 (512-row partitions) leg")
     return ap.parse_args()
 
 
@@ -57,12 +60,16 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     the kernels run on."""
     from filtered_tdma_amd import TdmaBackendGPU
 
-    learned = kind == "filtered_learned"
-    be = TdmaBackendGPU("filtered" if learned else kind, N_SYS, N_ROW, P, 0.0)
+    guard = kind.endswith("_guard")
+    base_kind = kind[:-len("_guard")] if guard else kind
+    learned = base_kind == "filtered_learned"
+    be = TdmaBackendGPU("filtered" if learned else base_kind, N_SYS, N_ROW, P, 0.0)
     be.set_eps(EPS)
     be.set_rho_device(tA, tB, tC)
     if learned:
         be.set_j_mode(1)
+    if guard:
+        be.set_guard(True)
     stream = torch.cuda.current_stream()
     for k in range(warmup):
         copies[k % len(copies)].copy_(D0)
@@ -81,9 +88,13 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     wall = time.perf_counter() - t0
     ev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     path = be.last_path()
-    if learned:
-        Js, fb = be.last_J_systems()
-        J = {"mean_J": float(Js.mean()), "fallback_fraction": float(fb.mean())}
+    if learned or guard:
+        rec = be.last_J_records()
+        J = {"mean_J": float((rec & 0xFFFF).mean())}
+        if learned:
+            J["fallback_fraction"] = float(((rec >> 16) & 1).mean())
+        if guard:
+            J["guarded_system_fraction"] = float(((rec >> 17) & 1).max(axis=0).mean())
     else:
         J = be.last_J() if kind != "pascal" else None
     be.close()
@@ -95,13 +106,17 @@ def solve_once(kind, tA, tB, tC, tD):
     are not modified by the kernels this shape takes); returns x."""
     from filtered_tdma_amd import TdmaBackendGPU
 
-    learned = kind == "filtered_learned"
-    be = TdmaBackendGPU("filtered" if learned else kind, N_SYS, N_ROW, P, 0.0)
+    guard = kind.endswith("_guard")
+    base_kind = kind[:-len("_guard")] if guard else kind
+    learned = base_kind == "filtered_learned"
+    be = TdmaBackendGPU("filtered" if learned else base_kind, N_SYS, N_ROW, P, 0.0)
     try:
         be.set_eps(EPS)
         be.set_rho_device(tA, tB, tC)
         if learned:
             be.set_j_mode(1)
+        if guard:
+            be.set_guard(True)
         x = tD.clone()
         be.solve(tA, tB, tC, x)
     finally:
@@ -147,12 +162,14 @@ def time_f32(tA32, tB32, tC32, tD, tD32, copies, steps, warmup, d32, torch):
 def accuracy(x, ref, eps, torch):
     """Per system: |x - ref|_inf / |ref|_inf (systems are the columns of the
     [n_row, n_sys] arrays); the largest, and the fractions of systems above
-    1e-12 and above eps_."""
+    1e-12 and above eps_; and the absolute |x - ref|_inf, the measure eps_
+    bounds (its largest, the fraction of systems above eps_)."""
     num = (x - ref).abs().amax(dim=0)
     den = ref.abs().amax(dim=0).clamp_min(1e-300)
     rel = num / den
     return {"max_rel": float(rel.max()), "frac_gt_1e-12": float((rel > 1e-12).double().mean()),
-            "frac_gt_eps": float((rel > eps).double().mean()), "eps": eps}
+            "frac_gt_eps": float((rel > eps).double().mean()), "eps": eps,
+            "max_abs": float(num.max()), "frac_abs_gt_eps": float((num > eps).double().mean())}
 
 
 def c3_error(x_dev, A32, B32, C32, Dw, path):
@@ -346,15 +363,21 @@ def c5_channel(steps, warmup, torch):
     eps_small = eps_c / float(m_min * P_f) ** 2  # eps_ of the shortest partitions
     learned = {"partitions": P_f, "eps_constant": eps_c}
     x_ref = zc[0].clone()
-    for mode, name in ((0, "cal_J"), (1, "learned")):
+    for mode, guard, name in ((0, 0, "cal_J"), (1, 0, "learned"), (0, 1, "cal_J_guard"),
+                              (1, 1, "learned_guard")):
         bf = TdmaBackendGPU("filtered", n_sys, n_row, P_f, eps_c)
         bf.set_rho_device(tA, tB, tC)
         bf.set_j_mode(mode)
+        bf.set_guard(bool(guard))
         ms = timed(lambda k: bf.solve_complex(tA, tB, tC, zc[k]), [(zc, tZ0)])
         acc = accuracy(zc[0], x_ref, eps_small, torch)
-        if mode:
-            J, fb = bf.last_J_systems(True)
-            stats = {"mean_J": float(J.mean()), "fallback_fraction": float(fb.mean())}
+        if mode or guard:
+            rec = bf.last_J_records(True)
+            stats = {"mean_J": float((rec & 0xFFFF).mean())}
+            if mode:
+                stats["fallback_fraction"] = float(((rec >> 16) & 1).mean())
+            if guard:
+                stats["guarded_system_fraction"] = float(((rec >> 17) & 1).max(axis=0).mean())
         else:
             stats = {"mean_J": float(np.mean(bf.last_J(2 * P_f)))}
         learned[name] = {"ms": ms, "accuracy_vs_exact": acc, "path": bf.last_path(), **stats}
@@ -415,6 +438,86 @@ def f4_poisson(steps, warmup, torch, cpu=True):
         out["cpu_oracle_s"] = time.perf_counter() - t
         out["max_abs_diff_vs_oracle"] = float(max(np.max(np.abs(g - r)) for g, r in zip(got, ref)))
     return out
+
+
+def long_partitions(steps, warmup, torch):
+    """The reference's H200 per-rank slab: 262144 systems of 1024 rows in
+    P = 2 caller partitions of 512 rows (the published weak-scaling runs use
+    512-row slabs per rank). PaScaL solves them exactly over internal
+    partitions that fit a lane (exact to rounding); the filtered kinds keep
+    the caller's partitions, whose sweeps are the bits of the method, and run
+    the general kernels (partitions walked in global memory, in place)."""
+    from filtered_tdma_amd import TdmaBackendGPU, cases
+
+    n_sys, n_row, P = 262144, 1024, 2
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, R_LO, R_HI, seed=SEED)
+    tA0, tB, tC0 = (torch.from_numpy(x).cuda() for x in (A, B, C))
+    D0 = torch.from_numpy(D).cuda()
+    del A, B, C, D
+    rows = n_sys * n_row
+    out = {"workload": f"n_sys={n_sys} n_row={n_row} P={P}, a=c=-r b=1+2r r~U[{R_LO},{R_HI}], "
+                       f"D~U[-1,1] seed {SEED}, eps_={EPS}"}
+    x_ref = None
+    for kind in ("pascal", "filtered_v2", "filtered"):
+        be = TdmaBackendGPU(kind, n_sys, n_row, P, 0.0)
+        be.set_eps(EPS)
+        be.set_rho_device(tA0, tB, tC0)
+        # the general kernels use A and C as scratch: every solve gets fresh copies
+        tA, tC, x = torch.empty_like(tA0), torch.empty_like(tC0), torch.empty_like(D0)
+        stream = torch.cuda.current_stream()
+        ms = []
+        for k in range(warmup + steps):
+            tA.copy_(tA0)
+            tC.copy_(tC0)
+            x.copy_(D0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            be.solve(tA, tB, tC, x)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if k >= warmup:
+                ms.append(e0.elapsed_time(e1))
+        path = be.last_path()
+        be.close()
+        t = statistics.median(ms)
+        res = {"ms": t, "rows_per_s": rows / (t * 1e-3),
+               "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (t * 1e-3) / (HBM_PEAK_GBS * 1e9),
+               "path": path}
+        if kind == "pascal":
+            x_ref = x.clone()
+        else:
+            res["accuracy_vs_pascal"] = accuracy(x, x_ref, EPS, torch)
+        out[kind] = res
+    return out
+
+
+def cpu_baseline_reference(ranks=16, reps=20):
+    """The reference's own CPU PaScaL (libs/pascal_tdma/cpu through
+    apps/heat_cpu's TdmaBackend, built from its sources into oracle/_ref)
+    under mpiexec on the host cores: every rank owns 512 / ranks rows of the
+    262144 systems (the reference's z-slab split), median of `reps` solves.
+    None when the build or MPICH is not there."""
+    import subprocess
+
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(drv) and os.path.exists(mpiexec)):
+        return None
+    cmd = [mpiexec, "-n", str(ranks), drv, "time", "pascal", str(N_SYS), str(N_ROW), str(reps),
+           str(R_LO), str(R_HI), str(EPS)]
+    try:
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd="/tmp")
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
+    if res.returncode != 0 or not lines:
+        return None
+    r = json.loads(lines[-1])
+    return {"value": r["rows_per_s"], "unit": "rows/s", "cores": ranks, "kind": "reference",
+            "host_nproc": os.cpu_count(),
+            "sample": f"the reference's CPU PaScaL (TdmaBackend, libs/pascal_tdma/cpu) under "
+                      f"mpiexec -n {ranks}, full {N_SYS}x{N_ROW} batch split in {ranks} z-slabs, "
+                      f"median of {reps} solves ({r['median_s']:.3f} s each)"}
 
 
 def cpu_baseline(A, B, C, D):
@@ -513,7 +616,8 @@ def main():
         # one solve each, against FP64 PaScaL
         tA3, tB3, tC3 = (torch.from_numpy(x).cuda() for x in (A3, B3, C3))
         ref3 = solve_once("pascal", tA3, tB3, tC3, tD3)
-        for kind in ("filtered_v2", "filtered", "filtered_learned"):
+        for kind in ("filtered_v2", "filtered", "filtered_learned", "filtered_v2_guard",
+                     "filtered_guard", "filtered_learned_guard"):
             x3 = solve_once(kind, tA3, tB3, tC3, tD3)
             acc = accuracy(x3, ref3, EPS, torch)
             results.setdefault("c3_filtered_fp64", {})[kind] = {"accuracy_vs_pascal": acc}
@@ -557,6 +661,15 @@ def main():
         print(f"[bench] f4 pressure step 256^3: {f4['ms_per_step']:.4f} ms "
               f"(div ratio {f4['max_div_after_over_before']:.1e})", file=sys.stderr, flush=True)
 
+    longp = None
+    if not args.no_long:
+        del tA, tB, tC, D0
+        torch.cuda.empty_cache()
+        longp = long_partitions(5, 2, torch)
+        print("[bench] long partitions 262144x1024 P=2: " + ", ".join(
+            f"{k} {v['ms']:.3f} ms ({v['path']})" for k, v in longp.items() if k != "workload"),
+            file=sys.stderr, flush=True)
+
     head = results["pascal"] if "pascal" in results else next(iter(results.values()))
     achieved = BYTES_PER_ROW * rows / (head["event_ms_mean"] * 1e-3) / 1e9
     traffic = None
@@ -599,8 +712,10 @@ def main():
         line["c5"] = c5
     if f4 is not None:
         line["f4"] = f4
+    if longp is not None:
+        line["long_partitions"] = longp
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(A, B, C, D)
+        line["cpu_baseline"] = cpu_baseline_reference() or cpu_baseline(A, B, C, D)
     print(json.dumps(line), flush=True)
 
 

@@ -19,7 +19,7 @@ EXPORTS = (
     "ftd_last_path", "ftd_last_error_string", "ftd_solve_f32c", "ftd_solve_f32",
     "ftd_solve_batched", "ftd_solve_batched_f32", "ftd_set_batch_chunk", "ftd_autotune_batched",
     "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
-    "ftd_solve_static",
+    "ftd_solve_static", "ftd_set_guard",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
     "ftd_heat_set_plan",
@@ -77,6 +77,7 @@ def load() -> ctypes.CDLL:
                                        dp, dp, c_int, ctypes.c_char_p, ctypes.c_size_t]
     L.ftd_solve_complex.argtypes = [vp, dp, dp, dp, dp]
     L.ftd_set_j_mode.argtypes = [vp, c_int]
+    L.ftd_set_guard.argtypes = [vp, c_int]
     L.ftd_autotune.argtypes = [vp, dp, dp, dp, dp, c_int, ctypes.c_char_p, ctypes.c_size_t]
     L.ftd_last_J_systems.argtypes = [vp, ip, ctypes.c_int64]
     L.ftd_solve_static.argtypes = [c_int, c_int, c_int, c_int, c_double, c_int, dp, dp, dp, dp,
@@ -120,7 +121,7 @@ def load() -> ctypes.CDLL:
                  "ftd_solve_f32", "ftd_solve_batched", "ftd_solve_batched_f32",
                  "ftd_set_batch_chunk", "ftd_autotune_batched",
                  "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
-                 "ftd_solve_static"):
+                 "ftd_solve_static", "ftd_set_guard"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

@@ -124,16 +124,27 @@ class TdmaBackendGPU:
         (FILTERED kind), checked against eps_."""
         _lib.check(_lib.load().ftd_set_j_mode(self._h, int(mode)))
 
-    def last_J_systems(self, complex_rhs: bool = False):
-        """(J, fallback) arrays of shape [P, n_sys] (n_sys doubled for a
-        complex solve) from the last learned-mode solve."""
+    def set_guard(self, enable: bool) -> None:
+        """Filtered kinds: solve exactly the systems where the terms the
+        filtered solve drops exceed eps_/4 (v1 sweeps for every kind)."""
+        _lib.check(_lib.load().ftd_set_guard(self._h, int(bool(enable))))
+
+    def last_J_records(self, complex_rhs: bool = False):
+        """The per (partition, system) records of the last learned or guarded
+        solve, shape [P, n_sys] (n_sys doubled for a complex solve):
+        J | fallback << 16 | guard << 17."""
         import numpy as np
 
         n = self.partitions * self.n_sys * (2 if complex_rhs else 1)
         out = np.empty(n, dtype=np.int32)
         _lib.check(_lib.load().ftd_last_J_systems(
             self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
-        out = out.reshape(self.partitions, -1)
+        return out.reshape(self.partitions, -1)
+
+    def last_J_systems(self, complex_rhs: bool = False):
+        """(J, fallback) arrays of shape [P, n_sys] (n_sys doubled for a
+        complex solve) from the last learned-mode solve."""
+        out = self.last_J_records(complex_rhs)
         return out & 0xFFFF, (out >> 16) & 1
 
     def last_J(self, n=None) -> list:

@@ -80,6 +80,7 @@ struct ftd_handle {
     int force_general = 0;                   // use the general kernels only
     int no_stream = 0;                       // automatic choice without the streamed kernels
     int j_mode = 0;                          // 0: cal_J (reference), 1: learned J per system
+    int guard = 0;                           // filtered kinds: exact solve where terms exceed eps_/4
     DevBuf jsys;                             // learned mode: J (| fallback << 16) per lane
     size_t jsys_count = 0;
     double* rho = nullptr;                   // A_rho[n_row+1] then C_rho[n_row+1]
@@ -286,11 +287,15 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     const int n_sys = h->n_sys << csh, n_row = h->n_row, P = h->P;
     const int version = (h->kind == FTD_KIND_FILTERED_V2) ? 2 : 1;
     hipStream_t st = h->stream;
-    h->jsys_count = 0;  // J records belong to the last solve, if it was a learned one
+    h->jsys_count = 0;  // J records belong to the last solve, if it kept them
     int s_, mmax, mmin;
     part_of(n_row, P, 0, s_, mmax);
     part_of(n_row, P, P - 1, s_, mmin);
     if (mmin < 3) return fail(FTD_ERR_UNSUPPORTED, "every partition needs at least 3 rows");
+    // the guard: systems whose dropped terms exceed eps_/4 are solved exactly
+    const int g = h->guard;
+    if (g && cyclic) return fail(FTD_ERR_UNSUPPORTED, "the guard takes non-periodic solves");
+    const char* gs = g ? " guard" : "";
 
     // fused shape: smallest compiled M holding the largest partition
     FusedShape fs;
@@ -314,24 +319,31 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
 
     int* Jp = h->J;
     const size_t xbytes = sizeof(double) * (size_t)2 * P * n_sys;
-    const int Ms = (h->force_general || h->t_M || h->no_stream || csh)
+    const int Ms = (h->force_general || h->t_M || h->no_stream || csh ||
+                    (g && !ftd::stream_guard_supported(P)))
                        ? 0
                        : ftd::stream_rows(mmax, P);
     const bool learned = h->j_mode == 1 && version == 1;
+    // per (partition, system) records: learned J, and the guard's
+    const size_t nj = (size_t)P * n_sys;
+    const double ev = g ? h->eps_value : 0.0;
+    const int ed = g ? h->eps_direct : 0;
     if (Ms && ftd::stream_eligible(n_sys, A, B, C, D) && (!learned || Ms == 64)) {
         const int R = ftd::stream_ring_depth(Ms, P);
+        if (learned || g) {
+            FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
+            h->jsys_count = nj;
+        }
+        int* const Js = (learned || g) ? (int*)h->jsys.p : nullptr;
         if (learned) {
             // learned J: one streamed pass (the 64-row variants hold the
             // network's registers without spilling; shorter partitions take
             // the fused kernel below)
-            const size_t nj = (size_t)P * n_sys;
-            FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
-            h->jsys_count = nj;
-            std::snprintf(h->path, sizeof h->path, "filtered_learned_stream M=%d Q=%d R=%d", Ms, P,
-                          R);
+            std::snprintf(h->path, sizeof h->path, "filtered_learned_stream M=%d Q=%d R=%d%s", Ms,
+                          P, R, gs);
             FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 3, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                nullptr, nullptr, h->eps_value, h->eps_direct,
-                                                (int*)h->jsys.p, st));
+                                                   nullptr, nullptr, h->eps_value, h->eps_direct,
+                                                   Js, g, st));
             return FTD_OK;
         }
         if (version == 2) {
@@ -339,70 +351,78 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
             FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
             FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
             FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                          h->eps_direct, csh, Jp, st));
-            std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d", Ms, P, R);
+                                             h->eps_direct, csh, Jp, st));
+            std::snprintf(h->path, sizeof h->path, "filtered_v2_stream M=%d Q=%d R=%d%s", Ms, P, R,
+                          gs);
             FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 2, 0, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                nullptr, nullptr, 0.0, 0, nullptr, st));
+                                                   nullptr, nullptr, ev, ed, Js, g, st));
         } else {
             const int tiles = n_sys / 32;
             FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h, h->xbuf, xbytes));
-            std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d", Ms, P, R);
+            std::snprintf(h->path, sizeof h->path, "filtered_v1_stream M=%d Q=%d R=%d%s", Ms, P, R,
+                          gs);
             FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 1, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                (double*)h->xbuf.p, (double*)h->partial.p, 0.0, 0,
-                                                nullptr, st));
+                                                   (double*)h->xbuf.p, (double*)h->partial.p, 0.0,
+                                                   0, nullptr, 0, st));
             FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
-                                          h->eps_value, h->eps_direct, csh, Jp, st));
+                                             h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 2, A, B, C, D, n_sys, n_row, cyclic, Jp,
-                                                (double*)h->xbuf.p, nullptr, 0.0, 0, nullptr, st));
+                                                   (double*)h->xbuf.p, nullptr, ev, ed, Js, g, st));
         }
         return FTD_OK;
     }
     if (fs.M && csh && fs.S < 2) fs.M = 0;  // the two parts of a system share a tile
-    if (h->j_mode == 1 && version == 1) {
+    if ((learned || g) && fs.M) {
+        FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
+        h->jsys_count = nj;
+    }
+    if (learned) {
         // learned J: one pass (v1 sweeps, interface, J per system from the
         // MLP, checked against eps_, truncated correction)
         if (!fs.M) return fail(FTD_ERR_UNSUPPORTED, "learned J needs a fused kernel shape");
-        const size_t nj = (size_t)P * n_sys;
-        FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
-        h->jsys_count = nj;
-        std::snprintf(h->path, sizeof h->path, "filtered_learned_fused M=%d T=%d S=%d", fs.M,
-                      fs.T, fs.S);
+        std::snprintf(h->path, sizeof h->path, "filtered_learned_fused M=%d T=%d S=%d%s", fs.M,
+                      fs.T, fs.S, gs);
         FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 3, A, B, C, D, n_sys, n_row, P, fs.S,
-                                           cyclic, csh, Jp, nullptr, nullptr, h->eps_value,
-                                           h->eps_direct, (int*)h->jsys.p, st));
+                                              cyclic, csh, Jp, nullptr, nullptr, h->eps_value,
+                                              h->eps_direct, (int*)h->jsys.p, g, st));
         return FTD_OK;
     }
     if (fs.M) {
         const int tiles = (n_sys + fs.S - 1) / fs.S;
+        int* const Js = g ? (int*)h->jsys.p : nullptr;
         if (version == 2) {
             int nt = 0;
             FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)((n_sys + 255) / 256) * (P << csh)));
             FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
             FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                          h->eps_direct, csh, Jp, st));
-            std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d", fs.M, fs.T,
-                          fs.S);
+                                             h->eps_direct, csh, Jp, st));
+            std::snprintf(h->path, sizeof h->path, "filtered_v2_fused M=%d T=%d S=%d%s", fs.M, fs.T,
+                          fs.S, gs);
             FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 2, 0, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, csh, Jp, nullptr, nullptr, 0.0, 0, nullptr, st));
+                                                  cyclic, csh, Jp, nullptr, nullptr, ev, ed, Js, g,
+                                                  st));
         } else {
             FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
             FTD_TRY(ensure(h, h->xbuf, xbytes));
-            std::snprintf(h->path, sizeof h->path, "filtered_v1_fused M=%d T=%d S=%d", fs.M, fs.T,
-                          fs.S);
+            std::snprintf(h->path, sizeof h->path, "filtered_v1_fused M=%d T=%d S=%d%s", fs.M, fs.T,
+                          fs.S, gs);
             FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 1, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, csh, Jp, (double*)h->xbuf.p,
-                                               (double*)h->partial.p, 0.0, 0, nullptr, st));
+                                                  cyclic, csh, Jp, (double*)h->xbuf.p,
+                                                  (double*)h->partial.p, 0.0, 0, nullptr, 0, st));
             FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, tiles, P, n_row, h->rho,
-                                          h->eps_value, h->eps_direct, csh, Jp, st));
+                                             h->eps_value, h->eps_direct, csh, Jp, st));
             FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 2, A, B, C, D, n_sys, n_row, P, fs.S,
-                                               cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, 0.0, 0,
-                                               nullptr, st));
+                                                  cyclic, csh, Jp, (double*)h->xbuf.p, nullptr, ev,
+                                                  ed, Js, g, st));
         }
         return FTD_OK;
     }
 
     if (csh) return fail(FTD_ERR_UNSUPPORTED, "complex right-hand sides need a fused kernel shape");
+    if (g)
+        return fail(FTD_ERR_UNSUPPORTED,
+                    "the guard runs on the fused and streamed kernels (partitions of at most 64 rows)");
     // general path: sweeps in place, interface, J, final
     const int tiles = (n_sys + 255) / 256;
     FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
@@ -412,14 +432,14 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     if (version == 2) {
         FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
         FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                      h->eps_direct, csh, Jp, st));
+                                         h->eps_direct, csh, Jp, st));
     }
     FTD_LAUNCH(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
     FTD_LAUNCH(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
-                                           (double*)h->partial.p, &nt, st));
+                                              (double*)h->partial.p, &nt, st));
     if (version == 1)
         FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
-                                      h->eps_direct, csh, Jp, st));
+                                         h->eps_direct, csh, Jp, st));
     FTD_LAUNCH(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
     return FTD_OK;
 }
@@ -551,13 +571,14 @@ ftd_status ftd_create(ftd_handle** out, int kind, int n_sys, int n_row, int part
 
 ftd_status ftd_destroy(ftd_handle* h) {
     if (!h) return FTD_OK;
-    if (h->rho) hipFree(h->rho);
-    if (h->J) hipFree(h->J);
+    // teardown: errors of the frees are not reported (the handle is gone)
+    if (h->rho) (void)hipFree(h->rho);
+    if (h->J) (void)hipFree(h->J);
     for (DevBuf* b : {&h->partial, &h->xbuf, &h->R, &h->E, &h->b_dlanes, &h->b_dlong_off,
                       &h->b_dlong_n, &h->jsys})
-        if (b->p) hipFree(b->p);
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
+        if (b->p) (void)hipFree(b->p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
     delete h;
     return FTD_OK;
 }
@@ -669,7 +690,7 @@ struct TuneCand {
     int M, T, S, Q;
 };
 
-void tune_candidates(const ftd_handle* h, int n_row, std::vector<TuneCand>& out) {
+static void tune_candidates(const ftd_handle* h, int n_row, std::vector<TuneCand>& out) {
     out.push_back({0, 0, 0, 0, 0});
     out.push_back({1, 0, 0, 0, 0});
     static const int Ms[] = {8, 16, 32, 64};
@@ -696,7 +717,7 @@ void tune_candidates(const ftd_handle* h, int n_row, std::vector<TuneCand>& out)
         }
 }
 
-void apply_cand(ftd_handle* h, const TuneCand& c) {
+static void apply_cand(ftd_handle* h, const TuneCand& c) {
     h->force_general = 0;
     h->no_stream = c.mode == 1;
     h->t_M = c.mode == 2 ? c.M : 0;
@@ -758,7 +779,7 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
             total += ms;
         }
         if (e != hipSuccess || st != FTD_OK) {
-            hipFree(work);
+            (void)hipFree(work);
             apply_cand(h, saved);
             h->force_general = saved_general;
             h->profiling = saved_prof;
@@ -778,9 +799,13 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
             best_path = path;
         }
     }
-    hipFree(work);
+    const hipError_t ef = hipFree(work);
     h->profiling = saved_prof;
     h->force_general = saved_general;
+    if (ef != hipSuccess) {
+        apply_cand(h, saved);
+        return hip_fail(ef, "hipFree");
+    }
     if (best < 0) {
         apply_cand(h, saved);
         return fail(FTD_ERR_UNSUPPORTED, "no candidate shape holds this solve");
@@ -854,7 +879,7 @@ ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const fl
 namespace {
 
 // the plan of this layout on the handle (made and uploaded once per layout)
-ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
+static ftd_status batch_layout(ftd_handle* h, const int64_t* row_offsets, const int* n_rows,
                         int n_systems) {
     if (!row_offsets || !n_rows) return fail(FTD_ERR_INVALID, "null layout array");
     const size_t n = (size_t)n_systems;
@@ -1054,6 +1079,13 @@ ftd_status ftd_set_j_mode(ftd_handle* h, int mode) {
     if (!h) return fail(FTD_ERR_INVALID, "null handle");
     if (mode != 0 && mode != 1) return fail(FTD_ERR_INVALID, "J mode must be 0 or 1");
     h->j_mode = mode;
+    return reserve(h);
+}
+
+ftd_status ftd_set_guard(ftd_handle* h, int enable) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (enable != 0 && enable != 1) return fail(FTD_ERR_INVALID, "guard must be 0 or 1");
+    h->guard = enable;
     return reserve(h);
 }
 

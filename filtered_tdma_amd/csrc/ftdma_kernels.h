@@ -37,6 +37,8 @@ hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float*
 // wave): M rows per partition for Q partitions, or 0 when no variant holds
 // them; eligible when n_sys % 32 == 0 and the arrays are 16-byte aligned
 int stream_rows(int m_max, int Q);
+// the guarded filtered variants of the streamed kernel exist for Q <= 8
+bool stream_guard_supported(int Q);
 int stream_ring_depth(int M, int Q);
 bool stream_eligible(int n_sys, const void* A, const void* B, const void* C, const void* D);
 // FP32 storage of A, B, C, D, x in FP64 to X: same partitions and bits as
@@ -49,12 +51,15 @@ hipError_t launch_pascal_stream_f32(int M, int Q, const float* A, const float* B
 hipError_t launch_pascal_stream(int M, int Q, const double* A, const double* B, const double* C,
                                 double* D, int n_sys, int n_row, int cyclic, hipStream_t st);
 // pass 3: learned J per (system, partition), eps_ from eps_value/eps_direct,
-// J records to Jsys [Q x n_sys]
+// J records to Jsys [Q x n_sys]. guard = 1 (non-periodic; version 2 pass 0,
+// or version 1 pass 2 or 3): v1 sweeps, and the systems whose dropped terms
+// exceed eps_/4 solved exactly; records (J | fallback << 16 | guard << 17)
+// to Jsys
 hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const double* A,
                                   const double* B, const double* C, double* D, int n_sys,
                                   int n_row, int cyclic, const int* Jp, double* xbuf,
                                   double* partial, double eps_value, int eps_direct, int* Jsys,
-                                  hipStream_t st);
+                                  int guard, hipStream_t st);
 
 // uneven batches: one lane per chunk of <= 64 rows of a system, 64 lanes per
 // work item (a wave); kind 0 = single-chunk systems (plain Thomas), kind 1 =
@@ -100,12 +105,13 @@ hipError_t launch_thomas_cyclic_fused(const double* A, const double* B, const do
                                       int n_sys, int n_row, hipStream_t st);
 
 int filtered_fused_supported(int M, int maxt);
-size_t filtered_fused_lds(int P, int S);
+size_t filtered_fused_lds(int P, int S, int guard);
+// guard: as launch_filtered_stream
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
                                  int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
                                  double* partial, double eps_value, int eps_direct, int* Jsys,
-                                 hipStream_t st);
+                                 int guard, hipStream_t st);
 
 hipError_t launch_gen_thomas(const double* A, const double* B, double* C, double* D, int n_sys,
                              int n_row, hipStream_t st);

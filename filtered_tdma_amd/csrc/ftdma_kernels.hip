@@ -1044,7 +1044,73 @@ __device__ inline int mlp_J(float f0, float f1, float f2, float f3, float f4) {
     return (int)(cj > lim ? lim : (cj < -lim ? -lim : cj)) + kMlpMargin;
 }
 
-template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE, int CSH>
+// ---------------------------------------------------------------------------
+// Guard (GRD = 1; no counterpart in the reference): besides the truncation
+// by J, the filtered solve drops row 0's coupling to its own partition's last
+// unknown (c0', pack_reduced) and row m-1's to its own row 0 (a'_{m-1},
+// k_solve_both). With the interface values x0, xN the guard evaluates those
+// terms and the first truncated term of each side; where any exceeds eps_/4
+// in any partition of a system, the system is solved exactly: its 2P-row
+// reduced system (c0' kept) by Thomas in LDS, then the full correction.
+// Sweeps are v1's (full) under the guard, for every kind.
+// ---------------------------------------------------------------------------
+
+// the guard's exact reduced solve of one system (column sl of the [P][S]
+// LDS arrays): rows 2q = row 0 of partition q (a0', c0', d0' in gA, gC0,
+// gD0), 2q + 1 = row m-1 (aN, cN, dN in gAN, gCN, gDN); x into gD0 / gDN
+__device__ inline void guard_reduced_thomas(double* gA, double* gC0, double* gD0, double* gAN,
+                                            double* gCN, double* gDN, int P, int S, int sl) {
+    double cp = gC0[sl], dp = gD0[sl];
+    for (int q = 0; q < P; ++q) {
+        const int o = q * S + sl;
+        if (q > 0) {
+            const double ak = gA[o];
+            const double inv = 1.0 / (1.0 - ak * cp);
+            dp = inv * (gD0[o] - ak * dp);
+            cp = inv * gC0[o];
+            gC0[o] = cp;
+            gD0[o] = dp;
+        }
+        const double ak = gAN[o];
+        const double inv = 1.0 / (1.0 - ak * cp);
+        dp = inv * (gDN[o] - ak * dp);
+        cp = inv * gCN[o];
+        gCN[o] = cp;
+        gDN[o] = dp;
+    }
+    double xn = dp;
+    for (int q = P - 1; q >= 0; --q) {
+        const int o = q * S + sl;
+        if (q < P - 1) {
+            xn = gDN[o] - gCN[o] * xn;
+            gDN[o] = xn;
+        }
+        xn = gD0[o] - gC0[o] * xn;
+        gD0[o] = xn;
+    }
+}
+
+// the guard's test of one (system, partition): the dropped couplings and the
+// first truncated terms for J, against eps_/4
+template <int M>
+__device__ __forceinline__ bool guard_flag(const double (&a)[M], const double (&c)[M], int m,
+                                           int J, double c0p, double aN, double x0, double xN,
+                                           double eps) {
+    const double thr = eps * 0.25;
+    const int kL = (J < m - 2 ? J : m - 2) + 1;
+    const int kR = ((m - 1) - J > 1 ? (m - 1) - J : 1) - 1;
+    double akL = 0.0, ckR = 0.0;
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        if (j == kL) akL = a[j];
+        if (j == kR) ckR = c[j];
+    }
+    const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
+    const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
+    return fabs(c0p * xN) > thr || fabs(aN * x0) > thr || ta > thr || tc > thr;
+}
+
+template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE, int CSH, int GRD>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_fused(const double* __restrict__ A,
                                                          const double* __restrict__ B,
                                                          const double* __restrict__ C,
@@ -1073,7 +1139,8 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     // complex right-hand sides: the real and imaginary parts keep their own J
     // (the reference solves them as two real systems)
     const int par = CSH ? (i & 1) : 0;
-    const int J = (VERSION == 2 || PASS == 2) ? Jp[par * P + p] : m;
+    static_assert(!GRD || VERSION == 1, "the guard runs the v1 sweeps");
+    const int J = (VERSION == 2 || PASS == 2 || (GRD && PASS == 0)) ? Jp[par * P + p] : m;
 
     double a[M], c[M], d[M];
     double ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
@@ -1229,7 +1296,44 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
         const bool fallback = ta > eps || tc > eps;
         Jf = fallback ? m - 1 : Jl;
-        if (Jsys && live) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+        if (!GRD && Jsys && live) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+        if (GRD && fallback) Jf |= 1 << 16;  // the record keeps the fallback flag
+    }
+    if (GRD) {
+        const int fb = Jf & (1 << 16);
+        Jf &= 0xFFFF;
+        const double Nl = (double)(m * P);
+        const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        const double c0p = -rr * c[0] * c[1];
+        const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps);
+        const int PS = P * S, o = p * S + sl;
+        double* gCN = lds;  // the exchange arrays hold cN, dN, a0', d0'
+        double* gDN = gCN + PS;
+        double* gA = gDN + PS;
+        double* gD0 = gA + PS;
+        double* gC0 = gD0 + PS;
+        double* gAN = gC0 + PS;
+        int* gF = (int*)(gAN + PS);
+        if (PASS == 2) {
+            gCN[o] = cN;
+            gDN[o] = dN;
+            gA[o] = a0p;
+            gD0[o] = d0p;
+        }
+        gC0[o] = c0p;
+        gAN[o] = aN;
+        gF[o] = flag ? 1 : 0;
+        __syncthreads();
+        int sys = 0;
+        for (int q = 0; q < P; ++q) sys |= gF[q * S + sl];
+        if (sys && p == 0) guard_reduced_thomas(gA, gC0, gD0, gAN, gCN, gDN, P, S, sl);
+        __syncthreads();
+        if (sys) {
+            x0 = gD0[o];
+            xN = gDN[o];
+            Jf = m - 1;
+        }
+        if (Jsys && live) Jsys[(size_t)p * ld + i] = Jf | fb | (sys ? (1 << 17) : 0);
     }
 
     // k_final_pass: truncated correction
@@ -1266,7 +1370,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
 //           an LDS block, issued ahead of the sweep), x stored (M stores per
 //           tile).
 // ---------------------------------------------------------------------------
-template <int M, int Q, int R, int VERSION, int PASS, int WPE>
+template <int M, int Q, int R, int VERSION, int PASS, int WPE, int GRD>
 __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_stream(
     const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
     double* D, int n_sys, int n_row, int n_tiles, int cyclic, const int* __restrict__ Jp,
@@ -1274,8 +1378,12 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     int* __restrict__ Jsys) {
     constexpr int S = 32, NW = Q / 2;
     static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
+    static_assert(!GRD || VERSION == 1, "the guard runs the v1 sweeps");
     __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
-    __shared__ double ex[4][Q][S];  // C, D of row m-1; A, D of row 0 (packed)
+    // C, D of row m-1; A, D of row 0 (packed); under the guard also c0' and
+    // a'_{m-1}, and the guard flags
+    __shared__ double ex[GRD ? 6 : 4][Q][S];
+    __shared__ int gflag[GRD ? Q : 1][S];
     __shared__ __attribute__((aligned(16))) double xin[PASS == 2 ? NW : 1][128];
 
     const int t = threadIdx.x;
@@ -1286,7 +1394,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const size_t ld = (size_t)n_sys;
     int r0, m0;
     part_of(n_row, Q, p, r0, m0);
-    int J0 = (VERSION == 2 || PASS == 2) ? Jp[p] : m0;
+    int J0 = (VERSION == 2 || PASS == 2 || (GRD && PASS == 0)) ? Jp[p] : m0;
     asm volatile("" : "+v"(J0));  // J is loaded before the ring starts
     // PASS 2: rows 4w..4w+3 of xbuf (x0, xN of partitions 2w, 2w+1), two
     // systems per lane, as the ring's sub-blocks
@@ -1383,6 +1491,13 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             // the 2M row DMAs of this tile were issued after the xbuf DMA
             wait_vmcnt<(2 * M < 63 ? 2 * M : 63)>();
             xin_read((lds_cdptr_t)&xin[w][(l >> 5) * 64 + sl], x0, xN);
+            if (GRD) {
+                lds_barrier();  // the previous tile's readers of ex[] are done
+                ex[0][p][sl] = cN;
+                ex[1][p][sl] = dN;
+                ex[2][p][sl] = a0p;
+                ex[3][p][sl] = d0p;
+            }
         } else {
             lds_barrier();  // the previous tile's readers of ex[] are done
             ex[0][p][sl] = cN;
@@ -1427,7 +1542,36 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
                 const bool fallback = ta > eps || tc > eps;
                 Jf = fallback ? m - 1 : Jl;
-                Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+                if (!GRD) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
+                if (GRD && fallback) Jf |= 1 << 16;  // the record keeps the fallback flag
+            }
+            if (GRD) {
+                // the guard (as k_filtered_fused): the dropped couplings and the
+                // first truncated terms against eps_/4, the exact reduced solve
+                // of a flagged system in LDS by its partition-0 lane
+                const int fb = Jf & (1 << 16);
+                Jf &= 0xFFFF;
+                const double Nl = (double)(m * Q);
+                const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+                const double c0p = -rr * c[0] * c[1];
+                const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps);
+                ex[4][p][sl] = c0p;
+                ex[5][p][sl] = aN;
+                gflag[p][sl] = flag ? 1 : 0;
+                lds_barrier();
+                int sys = 0;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) sys |= gflag[q][sl];
+                if (sys && p == 0)
+                    guard_reduced_thomas(&ex[2][0][0], &ex[4][0][0], &ex[3][0][0], &ex[5][0][0],
+                                         &ex[0][0][0], &ex[1][0][0], Q, S, sl);
+                lds_barrier();
+                if (sys) {
+                    x0 = ex[3][p][sl];
+                    xN = ex[1][p][sl];
+                    Jf = m - 1;
+                }
+                Jsys[(size_t)p * ld + i] = Jf | fb | (sys ? (1 << 17) : 0);
             }
             const int jL = Jf < m - 2 ? Jf : m - 2;
             const int jR = (m - 1) - Jf > 1 ? (m - 1) - Jf : 1;
@@ -1973,37 +2117,50 @@ int filtered_fused_supported(int M, int maxt) {
     return 0;
 }
 
-size_t filtered_fused_lds(int P, int S) { return sizeof(double) * 4 * (size_t)P * (size_t)S; }
+size_t filtered_fused_lds(int P, int S, int guard) {
+    return guard ? (sizeof(double) * 6 + sizeof(int)) * (size_t)P * (size_t)S
+                 : sizeof(double) * 4 * (size_t)P * (size_t)S;
+}
 
 hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const double* A,
                                  const double* B, const double* C, double* D, int n_sys, int n_row,
                                  int P, int S, int cyclic, int csh, const int* Jp, double* xbuf,
                                  double* partial, double eps_value, int eps_direct, int* Jsys,
-                                 hipStream_t st) {
+                                 int guard, hipStream_t st) {
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * P);
-    const size_t lds = (pass == 2) ? 0 : filtered_fused_lds(P, S);
-#define LAUNCH(m_, t_, v_, p_, pf_, w_)                                                        \
+    // the guard runs after the first pass of v1 only
+    if (guard && (cyclic || pass == 1)) return hipErrorInvalidValue;
+    const size_t lds = guard ? filtered_fused_lds(P, S, 1)
+                             : (pass == 2) ? 0 : filtered_fused_lds(P, S, 0);
+#define LAUNCH(m_, t_, v_, p_, pf_, w_, g_)                                                   \
     do {                                                                                     \
         if (csh)                                                                             \
-            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 1>), grid, block, lds, \
-                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial,  \
-                               eps_value, eps_direct, Jsys);                                 \
+            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 1, g_>), grid, block, \
+                               lds, st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf,   \
+                               partial, eps_value, eps_direct, Jsys);                        \
         else                                                                                 \
-            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 0>), grid, block, lds, \
-                               st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf, partial,  \
-                               eps_value, eps_direct, Jsys);                                 \
+            hipLaunchKernelGGL((k_filtered_fused<m_, t_, v_, p_, pf_, w_, 0, g_>), grid, block, \
+                               lds, st, A, B, C, D, n_sys, n_row, P, lgS, cyclic, Jp, xbuf,   \
+                               partial, eps_value, eps_direct, Jsys);                        \
     } while (0)
 #define X(m_, t_, pf_, w_)                                \
     if (M == m_ && maxt == t_) {                          \
-        if (version == 2)                                 \
-            LAUNCH(m_, t_, 2, 0, pf_, w_);                \
+        if (guard) {                                      \
+            if (version == 2)                             \
+                LAUNCH(m_, t_, 1, 0, pf_, 1, 1);          \
+            else if (pass == 3)                           \
+                LAUNCH(m_, t_, 1, 3, pf_, 1, 1);          \
+            else                                          \
+                LAUNCH(m_, t_, 1, 2, pf_, 1, 1);          \
+        } else if (version == 2)                          \
+            LAUNCH(m_, t_, 2, 0, pf_, w_, 0);             \
         else if (pass == 1)                               \
-            LAUNCH(m_, t_, 1, 1, pf_, w_);                \
+            LAUNCH(m_, t_, 1, 1, pf_, w_, 0);             \
         else if (pass == 3)                               \
-            LAUNCH(m_, t_, 1, 3, pf_, w_);                \
+            LAUNCH(m_, t_, 1, 3, pf_, w_, 0);             \
         else                                              \
-            LAUNCH(m_, t_, 1, 2, pf_, w_);                \
+            LAUNCH(m_, t_, 1, 2, pf_, w_, 0);             \
         return hipGetLastError();                         \
     }
     FTD_FILTERED_VARIANTS(X)
@@ -2023,6 +2180,8 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
     X(32, 4, 8, 2)             \
     X(32, 16, 8, 2)
 #endif
+
+bool stream_guard_supported(int Q) { return Q <= 8; }
 
 int stream_rows(int m_max, int Q) {
     int best = 0;
@@ -2146,28 +2305,40 @@ hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const dou
                                   const double* B, const double* C, double* D, int n_sys,
                                   int n_row, int cyclic, const int* Jp, double* xbuf,
                                   double* partial, double eps_value, int eps_direct, int* Jsys,
-                                  hipStream_t st) {
+                                  int guard, hipStream_t st) {
     if (!stream_eligible(n_sys, A, B, C, D)) return hipErrorInvalidValue;
-    if (pass == 3 && !Jsys) return hipErrorInvalidValue;
+    if ((pass == 3 || guard) && !Jsys) return hipErrorInvalidValue;
+    if (guard && (cyclic || pass == 1)) return hipErrorInvalidValue;
     const int n_tiles = n_sys / 32;
-#define LAUNCH(m_, q_, r_, w_, v_, p_)                                                        \
+#define LAUNCH(m_, q_, r_, w_, v_, p_, g_)                                                    \
     do {                                                                                     \
-        const void* k = (const void*)k_filtered_stream<m_, q_, r_, v_, p_, w_>;              \
+        const void* k = (const void*)k_filtered_stream<m_, q_, r_, v_, p_, w_, g_>;          \
         const dim3 grid(resident_grid(k, q_ * 32, n_tiles)), block(q_ * 32);                 \
-        hipLaunchKernelGGL((k_filtered_stream<m_, q_, r_, v_, p_, w_>), grid, block, 0, st, A, \
-                           B, C, D, n_sys, n_row, n_tiles, cyclic, Jp, xbuf, partial,        \
+        hipLaunchKernelGGL((k_filtered_stream<m_, q_, r_, v_, p_, w_, g_>), grid, block, 0, st, \
+                           A, B, C, D, n_sys, n_row, n_tiles, cyclic, Jp, xbuf, partial,     \
                            eps_value, eps_direct, Jsys);                                     \
     } while (0)
+    // the guard's variants: one wave per SIMD (its LDS flags and exact reduced
+    // solve make them larger), and none for 16 partitions (their ring fills the LDS)
 #define X(m_, q_, r_, w_)                         \
     if (M == m_ && Q == q_) {                     \
-        if (version == 2)                         \
-            LAUNCH(m_, q_, r_, w_, 2, 0);         \
+        if (guard) {                              \
+            if (q_ > 8)                           \
+                return hipErrorInvalidValue;      \
+            else if (version == 2)                \
+                LAUNCH(m_, q_, r_, 1, 1, 0, (q_ <= 8));  \
+            else if (pass == 3)                   \
+                LAUNCH(m_, q_, r_, 1, 1, 3, (q_ <= 8));  \
+            else                                  \
+                LAUNCH(m_, q_, r_, 1, 1, 2, (q_ <= 8));  \
+        } else if (version == 2)                  \
+            LAUNCH(m_, q_, r_, w_, 2, 0, 0);      \
         else if (pass == 1)                       \
-            LAUNCH(m_, q_, r_, w_, 1, 1);         \
+            LAUNCH(m_, q_, r_, w_, 1, 1, 0);      \
         else if (pass == 3)                       \
-            LAUNCH(m_, q_, r_, w_, 1, 3);         \
+            LAUNCH(m_, q_, r_, w_, 1, 3, 0);      \
         else                                      \
-            LAUNCH(m_, q_, r_, w_, 1, 2);         \
+            LAUNCH(m_, q_, r_, w_, 1, 2, 0);      \
         return hipGetLastError();                 \
     }
     FTD_STREAM_VARIANTS(X)

@@ -175,10 +175,27 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
  * is the reference's cal_J. Register-resident kernels only. */
 ftd_status ftd_set_j_mode(ftd_handle* h, int mode);
 
-/* Extension: the J used per (partition, system) by the last learned-mode
- * solve, index p * n_sys + i (n_sys doubled for complex solves), with bit 16
- * set where the eps_ check fell back to the full correction. Blocking copy
- * of n values. */
+/* Extension (no counterpart in the reference): the guard of the filtered
+ * kinds. Besides the truncation by J, the reference's filtered solve drops
+ * row 0's coupling to its own partition's last unknown (pack_reduced,
+ * libs/filtered_tdma/cpu/filtered_tdma_kernels.hpp:178-189) and row m-1's
+ * to its own row 0 (solve_interfaces, :197-218), an error of order
+ * q^(m-1) max|x| that eps_ does not bound. With enable = 1 every kind runs
+ * the v1 (full) sweeps, evaluates per (partition, system) those dropped
+ * terms and the first term the truncation by J drops on each side, and
+ * solves a system exactly (its 2P-row reduced system by Thomas, then the
+ * full correction) where any of them exceeds eps_/4, so that x stays within
+ * about eps_ of the exact solve. Off by default (the reference's bits).
+ * Non-periodic solves on the fused and streamed kernels (partitions of at
+ * most 64 rows); others return FTD_ERR_UNSUPPORTED. ftd_last_J_systems
+ * reports which systems it solved exactly. */
+ftd_status ftd_set_guard(ftd_handle* h, int enable);
+
+/* Extension: the J used per (partition, system) by the last learned-mode or
+ * guarded solve, index p * n_sys + i (n_sys doubled for complex solves),
+ * with bit 16 set where the learned J's eps_ check fell back to the full
+ * correction and bit 17 where the guard solved the system exactly.
+ * Blocking copy of n values. */
 ftd_status ftd_last_J_systems(ftd_handle* h, int* out, int64_t n);
 
 /* Extension (complex right-hand sides; the reference's channel pressure

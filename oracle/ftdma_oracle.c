@@ -950,3 +950,127 @@ int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
     free(feat);
     return 0;
 }
+
+/* ------------------------------ guard mode --------------------------------
+ * No counterpart in the reference. Besides the truncation by J, the filtered
+ * kinds drop two couplings of each partition: row 0's to the partition's own
+ * last unknown (pack_reduced, filtered_tdma_kernels.hpp:178-189) and row
+ * m-1's to its own row 0 (solve_interfaces, :197-218). The guard evaluates,
+ * per (partition, system) and with the interface values x0, xN, the terms
+ * those drops leave out -- |c0' xN|, |a'_{m-1} x0| -- and the first term the
+ * truncation leaves out on each side -- |a'_{kL} x0|, |c'_{kR} xN|. A system
+ * where any of them exceeds eps_/4 (in any partition) is solved exactly: its
+ * 2P-row reduced system, c0' kept, by Thomas (B == 1, row 0 "divided by 1"
+ * left as it is), then the full correction. Every kind runs the v1 (full)
+ * sweeps under the guard.
+ * J[p * n_sys + i]: the J of (partition p, system i) before the guard (v1,
+ * v2: the partition's J in every column; learned: the J after its own
+ * check). rec_out[p * n_sys + i] = the J used | guard << 17. Non-periodic. */
+int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
+                        int n_sys, int n_row, int P, const int* J,
+                        double eps_value, int eps_direct, int* rec_out, int nthreads) {
+    int rc = check_parts(n_row, P);
+    if (rc) return rc;
+    if (P < 2) return fail(-1, "filtered_guard: needs at least 2 partitions");
+    if (P > 512) return fail(-1, "filtered_guard: at most 512 partitions");
+    if (!J) return fail(-1, "filtered_guard: J needed");
+    const size_t ld = (size_t)n_sys;
+    int sv[512], mv[512];
+    for (int p = 0; p < P; ++p) part_rows(n_row, P, p, &sv[p], &mv[p]);
+    /* reduced rows a0', c0', d0', aN, cN, dN per (partition, system) */
+    double* R = (double*)malloc(sizeof(double) * ld * (size_t)(6 * P) + 8);
+    if (!R) return fail(-3, "filtered_guard: out of memory");
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < P; ++p) {
+        const int s = sv[p], m = mv[p];
+        ffwd_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, 0, n_sys, m, m, 0);
+        fbwd_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, 0, n_sys, m, 1, 0, FTDO_SEM_GPU);
+        double* Rp = R + (size_t)(6 * p) * ld;
+        const double* A0 = ROW(A, s);
+        const double* A1 = ROW(A, s + 1);
+        const double* C0 = ROW(C, s);
+        const double* C1 = ROW(C, s + 1);
+        for (size_t i = 0; i < ld; ++i) {
+            /* pack_reduced's r, and the c0' it drops (pascal_tdma.cpp:177-183) */
+            const double r = 1.0 / (1.0 - A1[i] * C0[i]);
+            Rp[0 * ld + i] = A0[i];
+            Rp[1 * ld + i] = -r * C0[i] * C1[i];
+            Rp[2 * ld + i] = ROW(D, s)[i];
+            Rp[3 * ld + i] = ROW(A, s + m - 1)[i];
+            Rp[4 * ld + i] = ROW(C, s + m - 1)[i];
+            Rp[5 * ld + i] = ROW(D, s + m - 1)[i];
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n_sys; ++ii) {
+        const size_t i = (size_t)ii;
+        double x0[512], xN[512];
+        int guard = 0;
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            const double* Rp = R + (size_t)(6 * p) * ld;
+            const double a0p = Rp[0 * ld + i], c0p = Rp[1 * ld + i], d0p = Rp[2 * ld + i];
+            const double aN = Rp[3 * ld + i], cN = Rp[4 * ld + i], dN = Rp[5 * ld + i];
+            /* interface 2x2 solves (filtered_tdma_kernels.cuh:284-303) */
+            double v0 = d0p, vN = dN;
+            if (p > 0) {
+                const double* Rl = R + (size_t)(6 * (p - 1)) * ld;
+                v0 = (d0p - a0p * Rl[5 * ld + i]) / (1.0 - Rl[4 * ld + i] * a0p);
+            }
+            if (p < P - 1) {
+                const double* Rr = R + (size_t)(6 * (p + 1)) * ld;
+                vN = (dN - cN * Rr[2 * ld + i]) / (1.0 - Rr[0 * ld + i] * cN);
+            }
+            x0[p] = v0;
+            xN[p] = vN;
+            const double Nl = (double)(m * P);
+            const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+            const double thr = eps * 0.25;
+            const int Jp = J[(size_t)p * ld + i];
+            const int kL = (Jp < m - 2 ? Jp : m - 2) + 1;
+            const int kR = ((m - 1) - Jp > 1 ? (m - 1) - Jp : 1) - 1;
+            const double ta = kL <= m - 2 ? fabs(ROW(A, s + kL)[i] * v0) : 0.0;
+            const double tc = kR >= 1 ? fabs(ROW(C, s + kR)[i] * vN) : 0.0;
+            const double t0 = fabs(c0p * vN), tN = fabs(aN * v0);
+            if (t0 > thr || tN > thr || ta > thr || tc > thr) guard = 1;
+        }
+        if (guard) {
+            /* the exact reduced solve: rows [row 0 of p, row m-1 of p] */
+            double cp[1024], dp[1024];
+            cp[0] = R[1 * ld + i];
+            dp[0] = R[2 * ld + i];
+            for (int k = 1; k < 2 * P; ++k) {
+                const double* Rp = R + (size_t)(6 * (k >> 1)) * ld;
+                const double ak = Rp[((k & 1) ? 3 : 0) * ld + i];
+                const double ck = Rp[((k & 1) ? 4 : 1) * ld + i];
+                const double dk = Rp[((k & 1) ? 5 : 2) * ld + i];
+                const double inv = 1.0 / (1.0 - ak * cp[k - 1]);
+                dp[k] = inv * (dk - ak * dp[k - 1]);
+                cp[k] = inv * ck;
+            }
+            for (int k = 2 * P - 2; k >= 0; --k) dp[k] = dp[k] - cp[k] * dp[k + 1];
+            for (int p = 0; p < P; ++p) {
+                x0[p] = dp[2 * p];
+                xN[p] = dp[2 * p + 1];
+            }
+        }
+        for (int p = 0; p < P; ++p) {
+            const int s = sv[p], m = mv[p];
+            const int Jf = guard ? m - 1 : J[(size_t)p * ld + i];
+            const int left_end = Jf < m - 2 ? Jf : m - 2;
+            const int right_beg = (m - 1) - Jf > 1 ? (m - 1) - Jf : 1;
+            for (int j = 1; j < m - 1; ++j) {
+                double x = ROW(D, s + j)[i];
+                if (j <= left_end) x = x - ROW(A, s + j)[i] * x0[p];
+                if (j >= right_beg) x = x - ROW(C, s + j)[i] * xN[p];
+                ROW(D, s + j)[i] = x;
+            }
+            ROW(D, s)[i] = x0[p];
+            ROW(D, s + m - 1)[i] = xN[p];
+            if (rec_out) rec_out[(size_t)p * ld + i] = Jf | (guard << 17);
+        }
+    }
+    free(R);
+    return 0;
+}

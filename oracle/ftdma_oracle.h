@@ -91,6 +91,17 @@ int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
 int ftdo_mlp_J(const float* weights, int hidden, int margin, const float* features);
 float ftdo_lg(double x);
 
+/* Guard mode of the filtered kinds (no counterpart in the reference): v1
+ * sweeps and interface; per (partition, system) the terms the filtered
+ * solve leaves out (|c0' xN|, |a'_{m-1} x0| and the first truncated term of
+ * each side for J[p * n_sys + i]) against eps_/4; a system where any
+ * exceeds it is solved exactly (2P-row reduced Thomas, full correction).
+ * rec_out[p * n_sys + i] = J used | guard << 17. Non-periodic, 2..512
+ * partitions. */
+int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
+                        int n_sys, int n_row, int P, const int* J,
+                        double eps_value, int eps_direct, int* rec_out, int nthreads);
+
 /* Heat ADI harness (apps/heat_cpu on a parts[0] x parts[1] x parts[2] process
  * grid): nx, ny, nz interior cells, kind 0/1/2 = filtered / filtered_v2 /
  * pascal (a direction with one part is plain Thomas), periodic[3] flags, Nt

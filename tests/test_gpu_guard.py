@@ -1,0 +1,125 @@
+"""The guard of the filtered kinds (ftd_set_guard) against the oracle's
+restatement (ftdo_filtered_guard): x and the per (partition, system)
+records bit for bit, for v1, v2 and learned J on the streamed and the fused
+kernels, real and complex right-hand sides; and the property the guard is
+for: every system within eps_ of the exact (PaScaL) solve, where the
+reference's filtered solve is not."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from filtered_tdma_amd import FtdError, Kind, TdmaBackendGPU, cases
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def run_guard(kind, A, B, C, D, P, eps_c, learned=False, complex_rhs=False):
+    n_row, n_sys = A.shape
+    be = TdmaBackendGPU(kind, n_sys, n_row, P, eps_c)
+    try:
+        be.set_guard(True)
+        if learned:
+            be.set_j_mode(1)
+        tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
+        be.set_rho_device(tA, tB, tC)
+        if complex_rhs:
+            be.solve_complex(tA, tB, tC, tD)
+        else:
+            be.solve(tA, tB, tC, tD)
+        torch.cuda.synchronize()
+        return tD.cpu().numpy(), be.last_J_records(complex_rhs), be.last_path()
+    finally:
+        be.close()
+
+
+def oracle_guard(kind, A, B, C, D, P, eps_c, learned=False):
+    """x and the records the guarded solve should give: the kind's J per
+    (partition, system) by the oracle, then ftdo_filtered_guard."""
+    n_sys = D.shape[1]
+    fb = np.zeros((P, n_sys), dtype=np.int32)
+    if learned:
+        _, Jr = O.filtered_learned(A, B, C, D, P, eps_c)
+        J, fb = Jr & 0xFFFF, (Jr >> 16) & 1
+    else:
+        version = 2 if kind == Kind.FILTERED_V2 else 1
+        _, Jp = O.filtered(A, B, C, D, P, version, eps_c, sem=O.SEM_GPU, rho="auto",
+                           return_J=True)
+        J = np.repeat(np.asarray(Jp, dtype=np.int32)[:, None], n_sys, axis=1)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, eps_c)
+    return x, rec | (fb << 16)
+
+
+CASES = [
+    # (kind, learned, n_sys, n_row, P, r range, eps_constant)
+    (Kind.FILTERED, False, 1024, 512, 8, (0.5, 20.0), 1e-3),      # streamed, two passes
+    (Kind.FILTERED_V2, False, 1024, 512, 8, (0.5, 20.0), 1e-3),   # streamed, one pass
+    (Kind.FILTERED, True, 1024, 512, 8, (0.5, 20.0), 1e-3),       # streamed learned
+    (Kind.FILTERED, False, 300, 256, 8, (0.05, 3.0), 1e-2),       # fused (n_sys % 32 != 0)
+    (Kind.FILTERED_V2, False, 300, 200, 4, (5.0, 500.0), 1e-4),   # fused, rho to 0.499
+    (Kind.FILTERED, True, 257, 128, 8, (0.05, 3.0), 1e-6),        # fused learned, m = 16
+    (Kind.FILTERED_V2, False, 512, 256, 16, (0.5, 20.0), 1e-3),   # 16 partitions: fused
+]
+
+
+@pytest.mark.parametrize("kind,learned,n_sys,n_row,P,rr,eps_c", CASES)
+def test_guard_bitwise_and_within_eps(kind, learned, n_sys, n_row, P, rr, eps_c):
+    note(f"guard {kind.name} learned={learned} n_sys={n_sys} n_row={n_row} P={P} r={rr}")
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, rr[0], rr[1], seed=31 + P)
+    x, rec, path = run_guard(kind, A, B, C, D, P, eps_c, learned)
+    assert path.endswith(" guard"), path
+    xr, recr = oracle_guard(kind, A, B, C, D, P, eps_c, learned)
+    np.testing.assert_array_equal(x, xr, err_msg=path)
+    np.testing.assert_array_equal(rec, recr, err_msg=path)
+    exact = O.pascal(A, B, C, D, P)
+    m_max = -(-n_row // P)
+    eps = eps_c / float(m_max * P) ** 2  # the smallest eps_ of the partitions
+    assert np.max(np.abs(x - exact)) <= eps, (np.max(np.abs(x - exact)), eps)
+
+
+def test_guard_complex_parts_guarded_separately():
+    """Complex right-hand sides: each part is its own system for the guard
+    (the reference solves the parts as two real systems)."""
+    A, B, C, Z = cases.channel_poisson(32, 32, 129, seed=4)
+    P, eps_c = 8, 1e-6
+    n_row, n_sys = Z.shape
+    note("guard complex channel modes 32x32x129, P=8")
+    X, rec, path = run_guard(Kind.FILTERED_V2, A, B, C, Z, P, eps_c, complex_rhs=True)
+    for par, part, x in ((0, Z.real, X.real), (1, Z.imag, X.imag)):
+        d = np.ascontiguousarray(part)
+        xr, recr = oracle_guard(Kind.FILTERED_V2, A, B, C, d, P, eps_c)
+        np.testing.assert_array_equal(x, xr, err_msg=path)
+        np.testing.assert_array_equal(rec[:, par::2], recr, err_msg=path)
+
+
+def test_guard_refuses_periodic_and_long_partitions():
+    A, B, C, D, _ = cases.toeplitz_batch(64, 96, 0.5, 20.0, seed=1)
+    be = TdmaBackendGPU(Kind.FILTERED, 64, 96, 4, 1e-3)
+    try:
+        be.set_guard(True)
+        tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
+        with pytest.raises(FtdError):
+            be.solve_cyclic(tA, tB, tC, tD)
+    finally:
+        be.close()
+    A, B, C, D, _ = cases.toeplitz_batch(64, 300, 0.5, 20.0, seed=1)
+    be = TdmaBackendGPU(Kind.FILTERED_V2, 64, 300, 2, 1e-3)
+    try:
+        be.set_guard(True)
+        tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
+        with pytest.raises(FtdError):
+            be.solve(tA, tB, tC, tD)
+    finally:
+        be.close()

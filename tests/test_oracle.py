@@ -265,6 +265,61 @@ def test_filtered_error_bounded_by_decoupling(version):
     assert err > 0  # the dropped couplings are not exactly zero here
 
 
+# ------------------------------------------------------------- guard mode
+
+def _v1_J(A, B, C, D, P, eps, eps_direct):
+    _, J = O.filtered(A, B, C, D, P, 1, eps, eps_direct=eps_direct, rho="auto", return_J=True)
+    return np.repeat(J[:, None], D.shape[1], axis=1)
+
+
+def test_guard_never_triggered_is_v1_with_its_J():
+    """An unreachable threshold: no system is guarded, and the result is the
+    v1 solve with the same J per partition, bit for bit."""
+    A, B, C, D, _ = cases.toeplitz_batch(30, 256, 0.5, 20.0, seed=21)
+    J = _v1_J(A, B, C, D, 4, 1e-6, False)
+    x, rec = O.filtered_guard(A, B, C, D, 4, J, 1e300, eps_direct=True)
+    np.testing.assert_array_equal(x, O.filtered(A, B, C, D, 4, 1, 1e-6, rho="auto",
+                                                J_force=J[:, 0]))
+    assert not np.any(rec >> 17)
+    np.testing.assert_array_equal(rec & 0xFFFF, J)
+
+
+def test_guard_always_triggered_is_an_exact_solve():
+    """eps_ = 0: every system is guarded and solved exactly (reduced system
+    with the dropped couplings kept, full correction): LAPACK to 1e-13."""
+    A, B, C, D = cases.general_batch(25, 320, seed=22)
+    J = np.zeros((8, 25), dtype=np.int32)
+    x, rec = O.filtered_guard(A, B, C, D, 8, J, 0.0, eps_direct=True)
+    assert relerr(x, banded_solve(A, B, C, D)) <= 1e-13
+    assert np.all(rec >> 17 == 1)
+    m = 320 // 8
+    np.testing.assert_array_equal(rec & 0xFFFF, m - 1)
+
+
+@pytest.mark.parametrize("r_lo,r_hi,P,eps", [
+    (0.5, 20.0, 8, 1e-10),     # the 262144 x 512 workload's systems (m = 64)
+    (5.0, 500.0, 8, 1e-10),    # weakly dominant (rho to 0.499)
+    (0.05, 3.0, 4, 1e-6),      # strongly dominant: few systems guarded
+])
+def test_guard_keeps_every_system_within_eps_of_exact(r_lo, r_hi, P, eps):
+    """The point of the guard: max |x - x_exact| <= eps_ for every system,
+    where the reference's filtered solve (v1, same J) is off by more."""
+    n_sys, n_row = 200, 512
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, r_lo, r_hi, seed=23)
+    J = _v1_J(A, B, C, D, P, eps, True)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, eps, eps_direct=True)
+    exact = O.pascal(A, B, C, D, P)
+    err = np.max(np.abs(x - exact), axis=0)
+    assert np.all(err <= eps), (err.max(), eps)
+    guarded = (rec >> 17).max(axis=0) == 1
+    # every partition of a system shares its guard decision
+    assert np.all((rec >> 17) == guarded[None, :])
+    ref = O.filtered(A, B, C, D, P, 1, eps, eps_direct=True, rho="auto")
+    if r_hi > 10:
+        assert np.max(np.abs(ref - exact)) > eps  # what the guard fixes
+        assert guarded.mean() > 0.1
+
+
 def test_filtered_gpu_vs_cpu_semantics_same_when_J_full():
     """With rho unset (J = m-1 everywhere) GPU v1 and CPU v1 run the same
     statements."""
