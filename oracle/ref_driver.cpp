@@ -4,7 +4,9 @@
 // tree to build, is not used where the GPU tests run, and no product code
 // calls it.
 //
-// Two modes, both reading one input file and writing the solved D:
+// Three modes. `time` (run under `mpiexec -n P`) times the reference's CPU
+// backend on a generated batch (run_time below); the other two read one
+// input file and write the solved D:
 //
 //   backend : run under `mpiexec -n P`. Rank r takes rows
 //             para_range(1, n_row, P, r) of every system (the z-slab split
@@ -27,6 +29,7 @@
 // Output file: float64 D [n_row*n_sys].
 #include <mpi.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -196,11 +199,93 @@ int run_steps(Input& in, const char* out_path) {
     return write_output(out_path, in.D) ? 0 : 3;
 }
 
+// splitmix64 -> uniform [0, 1) with 53 bits (the distribution of the GPU
+// workloads; the timing does not depend on the exact values)
+double uniform01(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Times the reference's CPU TdmaBackend (apps/heat_cpu/tdma_backend.cpp) on
+// the batched workload: rank r owns rows para_range(1, n_row, P, r) of every
+// system (the z-slab split), a = c = -r_s, b = 1 + 2 r_s with r_s ~ U[r_lo,
+// r_hi] per system, D ~ U[-1, 1]. Each rep restores A, C, D (the solve
+// clobbers them), then times set_rho + solve between barriers. Rank 0 prints
+// one JSON line: the median and the minimum over reps, rows/s from the median.
+int run_time(int argc, char** argv) {
+    int rank, P;
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &P);
+    if (argc != 9) {
+        if (rank == 0)
+            std::fprintf(stderr, "usage: ref_driver time kind n_sys n_row reps r_lo r_hi eps_\n");
+        return 1;
+    }
+    const std::string kname = argv[2];
+    const int n_sys = std::atoi(argv[3]), n_row = std::atoi(argv[4]), reps = std::atoi(argv[5]);
+    const double r_lo = std::atof(argv[6]), r_hi = std::atof(argv[7]), eps = std::atof(argv[8]);
+    int s0, m;
+    rows_of(n_row, P, rank, s0, m);
+    const size_t ns = (size_t)n_sys, nloc = (size_t)m * ns;
+    std::vector<double> A0(nloc), B(nloc), C0(nloc), D0(nloc);
+    for (size_t i = 0; i < ns; ++i) {
+        const double r = r_lo + (r_hi - r_lo) * uniform01(2 * i);
+        for (int j = 0; j < m; ++j) {
+            const size_t k = (size_t)j * ns + i;
+            A0[k] = -r;
+            C0[k] = -r;
+            B[k] = 1.0 + 2.0 * r;
+            D0[k] = 2.0 * uniform01(((uint64_t)(s0 + j) << 32) ^ (2 * i + 1)) - 1.0;
+        }
+    }
+    std::vector<double> A(nloc), C(nloc), D(nloc);
+    int left = rank - 1 < 0 ? MPI_PROC_NULL : rank - 1;
+    int right = rank + 1 >= P ? MPI_PROC_NULL : rank + 1;
+    // eps_ = eps_constant / (n_row_local * nprocs)^2 (filtered_tdma.cpp:50-51)
+    const double eps_constant = eps * (double)m * P * (double)m * P;
+    TdmaBackend be(TdmaBackend::parse(kname), n_sys, m, rank, P, MPI_COMM_WORLD, left, right,
+                   eps_constant);
+    std::vector<double> t(reps);
+    for (int k = 0; k < reps; ++k) {
+        std::copy(A0.begin(), A0.end(), A.begin());
+        std::copy(C0.begin(), C0.end(), C.begin());
+        std::copy(D0.begin(), D0.end(), D.begin());
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double t0 = MPI_Wtime();
+        be.set_rho(A.data(), B.data(), C.data());
+        be.solve(A.data(), B.data(), C.data(), D.data());
+        MPI_Barrier(MPI_COMM_WORLD);
+        t[k] = MPI_Wtime() - t0;
+    }
+    if (rank == 0) {
+        std::vector<double> s(reps);
+        std::copy(t.begin(), t.end(), s.begin());
+        std::sort(s.begin(), s.end());
+        const double med = s[reps / 2];
+        std::printf("{\"kind\": \"%s\", \"ranks\": %d, \"n_sys\": %d, \"n_row\": %d, "
+                    "\"reps\": %d, \"median_s\": %.6f, \"min_s\": %.6f, \"rows_per_s\": %.6e}\n",
+                    kname.c_str(), P, n_sys, n_row, reps, med, s[0],
+                    (double)n_sys * n_row / med);
+        std::fflush(stdout);
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "time") {
+        MPI_Init(&argc, &argv);
+        const int rc = run_time(argc, argv);
+        MPI_Finalize();
+        return rc;
+    }
     if (argc != 4) {
-        std::fprintf(stderr, "usage: ref_driver backend|steps in.bin out.bin\n");
+        std::fprintf(stderr, "usage: ref_driver backend|steps in.bin out.bin\n"
+                             "       ref_driver time kind n_sys n_row reps r_lo r_hi eps_\n");
         return 1;
     }
     const std::string mode = argv[1];
