@@ -283,58 +283,78 @@ __device__ __forceinline__ void stage_fix(const Stage& st) {
         : "memory");
 }
 
-// a, b, c, d of row r of the block in buffer t, for this lane's chunk: one
-// base address per lane and the buffer, row and array as immediate offsets,
-// so that no per-row address lives in a register; inline asm keeps the
-// compiler's own (draining) waits out of the staging
-template <int OFF>
-__device__ __forceinline__ void stage_read_at(lds_cdptr lane, double& a, double& b, double& c,
-                                              double& d) {
+// Row reads of the staged blocks: one base address per lane and the buffer,
+// row and array as immediate offsets, so that no per-row address lives in a
+// register; inline asm (waiting inside the same block) keeps the compiler's
+// own draining waits out of the staging.
+
+// rows 2K and 2K + 1 of buffer T (FP64): they are adjacent in each array's
+// slot, so one 16-byte read per array brings both, and one wait serves two
+// rows
+typedef double ftd_d2 __attribute__((ext_vector_type(2)));
+template <int T, int K>
+__device__ __forceinline__ void stage_read_pair(lds_cdptr lane, ftd_d2& a, ftd_d2& b, ftd_d2& c,
+                                                ftd_d2& d) {
+    constexpr int OFF = T * kBUF + K * 256;
     asm volatile(
-        "ds_read_b64 %0, %4 offset:%5\n\t"
-        "ds_read_b64 %1, %4 offset:%6\n\t"
-        "ds_read_b64 %2, %4 offset:%7\n\t"
-        "ds_read_b64 %3, %4 offset:%8\n\t"
+        "ds_read_b128 %0, %4 offset:%5\n\t"
+        "ds_read_b128 %1, %4 offset:%6\n\t"
+        "ds_read_b128 %2, %4 offset:%7\n\t"
+        "ds_read_b128 %3, %4 offset:%8\n\t"
         "s_waitcnt lgkmcnt(0)"
         : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
         : "v"(lane), "n"(OFF), "n"(OFF + kARR), "n"(OFF + 2 * kARR), "n"(OFF + 3 * kARR)
         : "memory");
 }
 
-template <int T, int R>
-__device__ __forceinline__ void stage_read(lds_cdptr lane, double& a, double& b, double& c,
-                                           double& d) {
-    stage_read_at<T * kBUF + (R >> 1) * 256 + (R & 1) * 8>(lane, a, b, c, d);
-}
-
-// FP32: the four values of row R, widened exactly to FP64
-template <int T, int R>
-__device__ __forceinline__ void stage_read_f32(lds_cdptr lane, double& a, double& b, double& c,
-                                               double& d) {
-    constexpr int OFF = T * kBUF4 + (R >> 2) * 512 + (R & 3) * 4;
-    float fa, fb, fc, fd;
+// rows 4K .. 4K + 3 of buffer T (FP32), widened exactly: one 16-byte read
+// per array, one wait for four rows
+typedef float ftd_f4 __attribute__((ext_vector_type(4)));
+template <int T, int K>
+__device__ __forceinline__ void stage_read_quad_f32(lds_cdptr lane, ftd_f4& a, ftd_f4& b,
+                                                    ftd_f4& c, ftd_f4& d) {
+    constexpr int OFF = T * kBUF4 + K * 512;
     asm volatile(
-        "ds_read_b32 %0, %4 offset:%5\n\t"
-        "ds_read_b32 %1, %4 offset:%6\n\t"
-        "ds_read_b32 %2, %4 offset:%7\n\t"
-        "ds_read_b32 %3, %4 offset:%8\n\t"
+        "ds_read_b128 %0, %4 offset:%5\n\t"
+        "ds_read_b128 %1, %4 offset:%6\n\t"
+        "ds_read_b128 %2, %4 offset:%7\n\t"
+        "ds_read_b128 %3, %4 offset:%8\n\t"
         "s_waitcnt lgkmcnt(0)"
-        : "=&v"(fa), "=&v"(fb), "=&v"(fc), "=&v"(fd)
+        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
         : "v"(lane), "n"(OFF), "n"(OFF + kARR4), "n"(OFF + 2 * kARR4), "n"(OFF + 3 * kARR4)
         : "memory");
-    a = (double)fa;
-    b = (double)fb;
-    c = (double)fc;
-    d = (double)fd;
 }
 
-template <typename TS, int T, int R>
-__device__ __forceinline__ void stage_row(lds_cdptr lane, double& a, double& b, double& c,
-                                          double& d) {
-    if constexpr (sizeof(TS) == 8)
-        stage_read<T, R>(lane, a, b, c, d);
-    else
-        stage_read_f32<T, R>(lane, a, b, c, d);
+// the 8 rows of block B from buffer B & 1 through step(j, a, b, c, d)
+template <typename TS, int B, typename F>
+__device__ __forceinline__ void block_rows(lds_cdptr lane, F& step) {
+    if constexpr (sizeof(TS) == 8) {
+#pragma unroll
+        for (int k = 0; k < kRB / 2; ++k) {
+            ftd_d2 a, b, c, d;
+            switch (k) {
+                case 0: stage_read_pair<B & 1, 0>(lane, a, b, c, d); break;
+                case 1: stage_read_pair<B & 1, 1>(lane, a, b, c, d); break;
+                case 2: stage_read_pair<B & 1, 2>(lane, a, b, c, d); break;
+                default: stage_read_pair<B & 1, 3>(lane, a, b, c, d); break;
+            }
+            step(B * kRB + 2 * k, a.x, b.x, c.x, d.x);
+            step(B * kRB + 2 * k + 1, a.y, b.y, c.y, d.y);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRB / 4; ++k) {
+            ftd_f4 a, b, c, d;
+            if (k == 0)
+                stage_read_quad_f32<B & 1, 0>(lane, a, b, c, d);
+            else
+                stage_read_quad_f32<B & 1, 1>(lane, a, b, c, d);
+            step(B * kRB + 4 * k, (double)a.x, (double)b.x, (double)c.x, (double)d.x);
+            step(B * kRB + 4 * k + 1, (double)a.y, (double)b.y, (double)c.y, (double)d.y);
+            step(B * kRB + 4 * k + 2, (double)a.z, (double)b.z, (double)c.z, (double)d.z);
+            step(B * kRB + 4 * k + 3, (double)a.w, (double)b.w, (double)c.w, (double)d.w);
+        }
+    }
 }
 
 template <typename TS>
@@ -370,15 +390,7 @@ struct Sweep {
                 else
                     stage_fix_f32<B & 1>(st);
             }
-            double a, b, c, d;
-            stage_row<TS, B & 1, 0>(st.lane, a, b, c, d); step(B * kRB + 0, a, b, c, d);
-            stage_row<TS, B & 1, 1>(st.lane, a, b, c, d); step(B * kRB + 1, a, b, c, d);
-            stage_row<TS, B & 1, 2>(st.lane, a, b, c, d); step(B * kRB + 2, a, b, c, d);
-            stage_row<TS, B & 1, 3>(st.lane, a, b, c, d); step(B * kRB + 3, a, b, c, d);
-            stage_row<TS, B & 1, 4>(st.lane, a, b, c, d); step(B * kRB + 4, a, b, c, d);
-            stage_row<TS, B & 1, 5>(st.lane, a, b, c, d); step(B * kRB + 5, a, b, c, d);
-            stage_row<TS, B & 1, 6>(st.lane, a, b, c, d); step(B * kRB + 6, a, b, c, d);
-            stage_row<TS, B & 1, 7>(st.lane, a, b, c, d); step(B * kRB + 7, a, b, c, d);
+            block_rows<TS, B>(st.lane, step);
             if (B + 2 < nblk) stage_issue(st, B & 1, B + 2, l, A, Bm, C, D);
             Sweep<TS, B + 1, NB>::run(st, l, nblk, A, Bm, C, D, step);
         }
@@ -590,30 +602,58 @@ __device__ __forceinline__ void red_modthomas(double* rA, double* rC, double* rD
 }
 
 // Thomas on the 2G rows of the groups' ends (row 0 and row m - 1 of each
-// group of the NR rows at r0), gathered by index; x into their rD
+// group of the NR rows at r0), x into their rD. The groups are walked in
+// order with their starts kept incrementally (no division per row), and both
+// end rows of a group are read together, so the LDS latency is paid once per
+// group
 __device__ __forceinline__ void red_thomas_ends(double* rA, double* rC, double* rD, int r0, int NR,
                                                 int G) {
-    auto row = [&](int k) {
-        int s, m;
-        split_rows(NR, G, k >> 1, s, m);
-        return r0 + s + ((k & 1) ? m - 1 : 0);
-    };
-    int rp = row(0);
-    double cp = rC[rp], dp = rD[rp];
-    for (int k = 1; k < 2 * G; ++k) {
-        const int rk = row(k);
-        const double ak = rA[rk];
-        const double inv = 1.0 / (1.0 - ak * cp);
-        dp = inv * (rD[rk] - ak * dp);
-        cp = inv * rC[rk];
-        rD[rk] = dp;
-        rC[rk] = cp;
+    const int base = NR / G, rem = NR % G;
+    // group 0: its row 0 starts the elimination, its row m - 1 follows
+    int m = base + (0 < rem ? 1 : 0);
+    double cp = rC[r0], dp = rD[r0];
+    {
+        const int rN = r0 + m - 1;
+        const double aN = rA[rN], cN = rC[rN], dN = rD[rN];
+        const double inv = 1.0 / (1.0 - aN * cp);
+        dp = inv * (dN - aN * dp);
+        cp = inv * cN;
+        rD[rN] = dp;
+        rC[rN] = cp;
     }
+    int s = m;
+    for (int g = 1; g < G; ++g) {
+        m = base + (g < rem ? 1 : 0);
+        const int r1 = r0 + s, rN = r0 + s + m - 1;
+        const double a1 = rA[r1], c1 = rC[r1], d1 = rD[r1];
+        const double aN = rA[rN], cN = rC[rN], dN = rD[rN];
+        double inv = 1.0 / (1.0 - a1 * cp);
+        dp = inv * (d1 - a1 * dp);
+        cp = inv * c1;
+        rD[r1] = dp;
+        rC[r1] = cp;
+        inv = 1.0 / (1.0 - aN * cp);
+        dp = inv * (dN - aN * dp);
+        cp = inv * cN;
+        rD[rN] = dp;
+        rC[rN] = cp;
+        s += m;
+    }
+    // backward: x of the last row is dp; then row 0 of the last group, row
+    // m - 1 and row 0 of each earlier group
     double dn = dp;
-    for (int k = 2 * G - 2; k >= 0; --k) {
-        const int rk = row(k);
-        dn = rD[rk] - rC[rk] * dn;
-        rD[rk] = dn;
+    for (int g = G - 1; g >= 0; --g) {
+        m = base + (g < rem ? 1 : 0);
+        s = g * base + (g < rem ? g : rem);
+        const int r1 = r0 + s, rN = r0 + s + m - 1;
+        const double c1 = rC[r1], d1 = rD[r1];
+        if (g < G - 1) {
+            const double cN = rC[rN], dN = rD[rN];
+            dn = dN - cN * dn;
+            rD[rN] = dn;
+        }
+        dn = d1 - c1 * dn;
+        rD[r1] = dn;
     }
 }
 
