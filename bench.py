@@ -124,20 +124,29 @@ def solve_once(kind, tA, tB, tC, tD):
     return x
 
 
-def time_f32(tA32, tB32, tC32, tD, tD32, copies, steps, warmup, d32, torch):
+def time_f32(tA32, tB32, tC32, tD, tD32, copies, steps, warmup, d32, torch, sweeps=None):
     """FP32-storage PaScaL (C3): d32=False -> FP32 A,B,C with FP64 D in place
-    (28 B/row); d32=True -> FP32 A,B,C,D, FP64 x to a separate array (24 B/row)."""
+    (28 B/row); d32=True -> FP32 A,B,C,D, FP64 x to a separate array (24 B/row);
+    sweeps = k -> FP32 elimination with k FP64-residual refinement sweeps
+    (FP32 A,B,C,D, x to FP64: 24 + 32 k B/row)."""
     from filtered_tdma_amd import TdmaBackendGPU
 
     be = TdmaBackendGPU("pascal", N_SYS, N_ROW, P, 0.0)
     stream = torch.cuda.current_stream()
-    for k in range(warmup):
-        if d32:
-            be.solve_f32(tA32, tB32, tC32, tD32, copies[k % len(copies)])
+
+    def one(k):
+        if sweeps is not None:
+            be.solve_f32_refine(tA32, tB32, tC32, tD32, copies[k], sweeps)
+        elif d32:
+            be.solve_f32(tA32, tB32, tC32, tD32, copies[k])
         else:
+            be.solve_f32c(tA32, tB32, tC32, copies[k])
+
+    for k in range(warmup):
+        if not d32 and sweeps is None:
             copies[k % len(copies)].copy_(tD)
-            be.solve_f32c(tA32, tB32, tC32, copies[k % len(copies)])
-    if not d32:
+        one(k % len(copies))
+    if not d32 and sweeps is None:
         for c in copies:
             c.copy_(tD)
     torch.cuda.synchronize()
@@ -146,10 +155,7 @@ def time_f32(tA32, tB32, tC32, tD, tD32, copies, steps, warmup, d32, torch):
     t0 = time.perf_counter()
     for k in range(steps):
         starts[k].record(stream)
-        if d32:
-            be.solve_f32(tA32, tB32, tC32, tD32, copies[k])
-        else:
-            be.solve_f32c(tA32, tB32, tC32, copies[k])
+        one(k)
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -172,15 +178,29 @@ def accuracy(x, ref, eps, torch):
             "max_abs": float(num.max()), "frac_abs_gt_eps": float((num > eps).double().mean())}
 
 
-def c3_error(x_dev, A32, B32, C32, Dw, path):
-    """max over sampled systems of |x - x_ref|_inf/|x_ref|_inf against the FP64
-    oracle of the stored (exactly widened) data."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def thomas_f64(A, B, C, D):
+    """Plain float64 Thomas (numpy, vectorised over the columns): the
+    reference of the FP32-storage legs' error figure."""
+    n_row = A.shape[0]
+    c = np.empty_like(D)
+    d = np.empty_like(D)
+    c[0] = C[0] / B[0]
+    d[0] = D[0] / B[0]
+    for j in range(1, n_row):
+        inv = 1.0 / (B[j] - A[j] * c[j - 1])
+        c[j] = C[j] * inv
+        d[j] = (D[j] - A[j] * d[j - 1]) * inv
+    for j in range(n_row - 2, -1, -1):
+        d[j] -= c[j] * d[j + 1]
+    return d
 
+
+def c3_error(x_dev, A32, B32, C32, Dw, path):
+    """max over sampled systems of |x - x_ref|_inf/|x_ref|_inf against the
+    float64 solve of the stored (exactly widened) data."""
     cols = np.arange(0, N_SYS, 4099)
     sub = [np.ascontiguousarray(a[:, cols]).astype(np.float64) for a in (A32, B32, C32)]
-    ref = O.thomas(*sub, np.ascontiguousarray(Dw[:, cols]))
+    ref = thomas_f64(*sub, np.ascontiguousarray(Dw[:, cols], dtype=np.float64))
     x = x_dev[:, cols]
     return float(np.max(np.max(np.abs(x - ref), axis=0) / np.max(np.abs(ref), axis=0)))
 
@@ -390,54 +410,71 @@ def c5_channel(steps, warmup, torch):
             "path": path, "filtered_J": learned}
 
 
-def f4_poisson(steps, warmup, torch, cpu=True):
+F4_SHAPE = dict(nx=256, ny=256, nz=256, Lx=2.0 * 6.28318530718, Ly=2.0 * 3.14159265359,
+                Lz=2.0, gam=1.5, dt=1e-3)
+
+
+def f4_poisson(steps, warmup, torch):
     """The channel pressure step on the reference's re180 mesh
     (apps/channel_cpu/input/PARA_INPUT_re180_pascal.dat: 256^3, gamma3 = 1.5,
     H = 2, Aspect 2 pi x pi): divergence, 2-D D2Z FFTs of the 256 planes,
-    33024 complex wall-normal PaScaL solves, Z2D FFTs, projection."""
+    33024 complex wall-normal PaScaL solves, Z2D FFTs, projection. Returns
+    the timings and the fields after the last step (for the CPU leg)."""
+    from filtered_tdma_amd import cases
     from filtered_tdma_amd.poisson import ChannelPoisson
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import poisson_oracle as PO
-
-    nx = ny = nz = 256
-    Lx, Ly, Lz, gam, dt = 2.0 * 6.28318530718, 2.0 * 3.14159265359, 2.0, 1.5, 1e-3
-    U, V, W = PO.random_velocity(nx, ny, nz, seed=0)
+    g = F4_SHAPE
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    U, V, W = cases.channel_velocity(nx, ny, nz, seed=0)
     Pf = np.zeros_like(U)
     dev0 = [torch.from_numpy(f).cuda() for f in (U, V, W, Pf)]
     dev = [d.clone() for d in dev0]
     # on torch's stream (the default), so the restores and the steps are ordered
-    ps = ChannelPoisson(nx, ny, nz, Lx, Ly, Lz, gam, "pascal", 1)
+    ps = ChannelPoisson(nx, ny, nz, g["Lx"], g["Ly"], g["Lz"], g["gam"], "pascal", 1)
     ps.set_profiling(True)
     try:
         for _ in range(warmup):
-            ps.solve(*dev, dt)
+            ps.solve(*dev, g["dt"])
         ms = []
         for _ in range(steps):
             for d, d0 in zip(dev, dev0):
                 d.copy_(d0)
-            ps.solve(*dev, dt)
+            ps.solve(*dev, g["dt"])
             ms.append(ps.last_ms())
         path = ps.solver_path()
     finally:
         ps.close()
-    got = [d.cpu().numpy() for d in dev]
-    Ug, Vg = got[0].copy(), got[1].copy()
-    PO.refresh_ghosts(Ug, Vg, nx, ny)
-    ch = PO.Channel(nx, ny, nz, Lx, Ly, Lz, gam)
-    div0 = np.max(np.abs(ch.divergence(U, V, W, dt)))
-    div1 = np.max(np.abs(ch.divergence(Ug, Vg, got[2], dt)))
     out = {"workload": "channel pressure step, re180 mesh 256^3, gamma_z=1.5, pascal, "
                        "hipFFT D2Z/Z2D + complex wall-normal solve + projection",
            "ms_per_step": statistics.mean(ms), "ms_median": statistics.median(ms),
-           "z_solver_path": path, "max_div_after_over_before": div1 / div0}
-    if cpu:
-        ref = [f.copy() for f in (U, V, W, Pf)]
-        t = time.perf_counter()
-        ch.solve(*ref, dt, P=1)
-        out["cpu_oracle_s"] = time.perf_counter() - t
-        out["max_abs_diff_vs_oracle"] = float(max(np.max(np.abs(g - r)) for g, r in zip(got, ref)))
-    return out
+           "z_solver_path": path}
+    return out, ((U, V, W, Pf), [d.cpu().numpy() for d in dev])
+
+
+def cpu_baseline_f4(inputs, got):
+    """CPU baseline of the pressure step: the oracle's numpy restatement of
+    the same step on one host core, timed, and the checker of the GPU
+    fields (largest difference, divergence after / before)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import poisson_oracle as PO
+
+    g = F4_SHAPE
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    U, V, W, Pf = inputs
+    ch = PO.Channel(nx, ny, nz, g["Lx"], g["Ly"], g["Lz"], g["gam"])
+    ref = [f.copy() for f in (U, V, W, Pf)]
+    t = time.perf_counter()
+    ch.solve(*ref, g["dt"], P=1)
+    cpu_s = time.perf_counter() - t
+    Ug, Vg = got[0].copy(), got[1].copy()
+    PO.refresh_ghosts(Ug, Vg, nx, ny)
+    div0 = np.max(np.abs(ch.divergence(U, V, W, g["dt"])))
+    div1 = np.max(np.abs(ch.divergence(Ug, Vg, got[2], g["dt"])))
+    return {"value": cpu_s * 1e3, "unit": "ms per step", "cores": 1, "kind": "port",
+            "sample": "the oracle's numpy pressure step (poisson_oracle), one step",
+            "max_abs_diff_gpu_vs_cpu": float(max(np.max(np.abs(a - b))
+                                                 for a, b in zip(got, ref))),
+            "max_div_after_over_before": float(div1 / div0)}
 
 
 def long_partitions(steps, warmup, torch):
@@ -595,9 +632,11 @@ def main():
         A32, B32, C32, D32 = (x.astype(np.float32) for x in (A3, B3, C3, D3))
         tA32, tB32, tC32, tD32 = (torch.from_numpy(x).cuda() for x in (A32, B32, C32, D32))
         tD3 = torch.from_numpy(D3).cuda()
-        for name, d32, bpr in (("pascal_f32c", False, 28), ("pascal_f32", True, 24)):
+        legs = (("pascal_f32c", False, 28, None), ("pascal_f32", True, 24, None),
+                ("pascal_f32_refine1", True, 56, 1), ("pascal_f32_refine2", True, 88, 2))
+        for name, d32, bpr, sweeps in legs:
             wall, ev_ms, path = time_f32(tA32, tB32, tC32, tD3, tD32, copies, args.steps,
-                                         args.warmup, d32, torch)
+                                         args.warmup, d32, torch, sweeps)
             ms = wall * 1e3 / args.steps
             ev_avg = statistics.mean(ev_ms)
             err = c3_error(copies[args.steps - 1].cpu().numpy(), A32, B32, C32,
@@ -606,8 +645,10 @@ def main():
                 "rows_per_s": rows / (wall / args.steps), "ms_per_step": ms,
                 "event_ms_mean": ev_avg, "bytes_per_row": bpr,
                 "hbm_frac_algorithmic": bpr * rows / (ev_avg * 1e-3) / (HBM_PEAK_GBS * 1e9),
-                "max_rel_err_vs_fp64_oracle": err, "path": path,
-                "workload": "r~U[5,500], FP32 storage, FP64 elimination",
+                "max_rel_err_vs_fp64_solve": err, "path": path,
+                "workload": "r~U[5,500], FP32 storage, " + (
+                    "FP64 elimination" if sweeps is None else
+                    f"FP32 elimination + {sweeps} FP64-residual refinement sweep(s)"),
             }
             print(f"[bench] {name}: {ms:.4f} ms/step (events {ev_avg:.4f} ms) err {err:.2e} {path}",
                   file=sys.stderr, flush=True)
@@ -657,9 +698,9 @@ def main():
 
     f4 = None
     if not args.no_f4:
-        f4 = f4_poisson(min(args.steps, 10), min(args.warmup, 2), torch, cpu=not args.no_cpu)
-        print(f"[bench] f4 pressure step 256^3: {f4['ms_per_step']:.4f} ms "
-              f"(div ratio {f4['max_div_after_over_before']:.1e})", file=sys.stderr, flush=True)
+        f4, f4_fields = f4_poisson(min(args.steps, 10), min(args.warmup, 2), torch)
+        print(f"[bench] f4 pressure step 256^3: {f4['ms_per_step']:.4f} ms", file=sys.stderr,
+              flush=True)
 
     longp = None
     if not args.no_long:
@@ -716,6 +757,11 @@ def main():
         line["long_partitions"] = longp
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_reference() or cpu_baseline(A, B, C, D)
+        if f4 is not None:
+            f4["cpu_baseline"] = cpu_baseline_f4(*f4_fields)
+            print(f"[bench] f4 CPU leg: {f4['cpu_baseline']['value']:.0f} ms per step, "
+                  f"div ratio {f4['cpu_baseline']['max_div_after_over_before']:.1e}",
+                  file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

@@ -17,6 +17,7 @@ EXPORTS = (
     "ftd_solve_cyclic", "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
     "ftd_set_profiling", "ftd_last_J", "ftd_last_gpu_ms", "ftd_last_comm_ms", "ftd_set_tuning",
     "ftd_last_path", "ftd_last_error_string", "ftd_solve_f32c", "ftd_solve_f32",
+    "ftd_solve_f32_refine",
     "ftd_solve_batched", "ftd_solve_batched_f32", "ftd_set_batch_chunk", "ftd_autotune_batched",
     "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
     "ftd_solve_static", "ftd_set_guard",
@@ -68,6 +69,7 @@ def load() -> ctypes.CDLL:
     L.ftd_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
     L.ftd_solve_f32c.argtypes = [vp, dp, dp, dp, dp]
     L.ftd_solve_f32.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.ftd_solve_f32_refine.argtypes = [vp, dp, dp, dp, dp, dp, c_int]
     L.ftd_solve_batched.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ip, c_int, dp, dp, dp,
                                     dp]
     L.ftd_solve_batched_f32.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ip, c_int, dp, dp,
@@ -118,7 +120,8 @@ def load() -> ctypes.CDLL:
     for name in ("ftd_create", "ftd_destroy", "ftd_solve", "ftd_solve_nr", "ftd_solve_cyclic",
                  "ftd_set_rho", "ftd_set_eps_constant", "ftd_set_eps", "ftd_set_stream",
                  "ftd_set_profiling", "ftd_last_J", "ftd_set_tuning", "ftd_solve_f32c",
-                 "ftd_solve_f32", "ftd_solve_batched", "ftd_solve_batched_f32",
+                 "ftd_solve_f32", "ftd_solve_f32_refine", "ftd_solve_batched",
+                 "ftd_solve_batched_f32",
                  "ftd_set_batch_chunk", "ftd_autotune_batched",
                  "ftd_solve_complex", "ftd_set_j_mode", "ftd_last_J_systems", "ftd_autotune",
                  "ftd_solve_static", "ftd_set_guard"):

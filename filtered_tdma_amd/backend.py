@@ -101,6 +101,15 @@ class TdmaBackendGPU:
         a, b, c, d = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C, D))
         _lib.check(_lib.load().ftd_solve_f32(self._h, a, b, c, d, _ptr(X, self.n_row, self.n_sys)))
 
+    def solve_f32_refine(self, A, B, C, D, X, sweeps: int) -> None:
+        """FP32 A, B, C, D; FP32 elimination, FP64 reduced solve and update,
+        then `sweeps` FP64-residual refinement sweeps; x to FP64 X
+        (24 + 32 sweeps B/row)."""
+        a, b, c, d = (_ptr(x, self.n_row, self.n_sys, "float32") for x in (A, B, C, D))
+        _lib.check(_lib.load().ftd_solve_f32_refine(self._h, a, b, c, d,
+                                                    _ptr(X, self.n_row, self.n_sys),
+                                                    int(sweeps)))
+
     def solve_complex(self, A, B, C, Z) -> None:
         """Complex right-hand sides: Z complex128 [n_row, n_sys] (or float64
         [n_row, 2 n_sys], (re, im) interleaved) <- x; A, B, C real."""

@@ -105,6 +105,22 @@ def uneven_batch(n_rows, r_lo: float, r_hi: float, seed: int = 0):
     return off, n_rows, A, B, C, D, r
 
 
+def channel_velocity(nx: int, ny: int, nz: int, seed: int):
+    """Face velocities U, V, W [nz + 2, ny + 2, nx + 2] ~ N(0, 1) for the
+    channel pressure step: periodic ghosts in x and y, no-penetration walls
+    (W on the wall faces k = 1 and nz + 1 is zero)."""
+    rng = np.random.default_rng(seed)
+    sh = (nz + 2, ny + 2, nx + 2)
+    U, V, W = (rng.standard_normal(sh) for _ in range(3))
+    for F in (U, V, W):
+        F[:, :, 0] = F[:, :, nx]
+        F[:, :, nx + 1] = F[:, :, 1]
+        F[:, 0, :] = F[:, ny, :]
+        F[:, ny + 1, :] = F[:, 1, :]
+    W[1], W[nz + 1] = 0.0, 0.0
+    return U, V, W
+
+
 def tanh_faces(n: int, gamma: float, L: float) -> np.ndarray:
     """Face positions x_f[0..n] of a tanh-stretched wall-normal grid
     (apps/channel_cpu/Grid.cpp:16-30)."""

@@ -876,6 +876,34 @@ ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const fl
     return solve_f32_common(h, A, B, C, D, X, 1);
 }
 
+ftd_status ftd_solve_f32_refine(ftd_handle* h, const float* A, const float* B, const float* C,
+                                const float* D, double* X, int sweeps) {
+    if (!h) return fail(FTD_ERR_INVALID, "null handle");
+    if (!A || !B || !C || !D || !X) return fail(FTD_ERR_INVALID, "null array pointer");
+    if (sweeps < 0 || sweeps > FTD_MAX_REFINE_SWEEPS)
+        return fail(FTD_ERR_INVALID, "refinement sweeps must lie in [0, 4]");
+    if (h->kind != FTD_KIND_PASCAL && h->P > 1)
+        return fail(FTD_ERR_UNSUPPORTED, "FP32 storage is available for the exact (PaScaL) solve");
+    if (h->n_sys == 0) return FTD_OK;
+    FusedShape fs;
+    if (h->n_row < 6 || !pick_pascal(h, h->n_row, 0, fs))
+        return fail(FTD_ERR_UNSUPPORTED, "no fused kernel holds this system for FP32 storage");
+    if (h->profiling) FTD_HIP(hipEventRecord(h->ev0, h->stream));
+    std::snprintf(h->path, sizeof h->path, "pascal_ir M=%d T=%d S=%d Q=%d sweeps=%d", fs.M, fs.T,
+                  fs.S, fs.Q, sweeps);
+    for (int k = 0; k <= sweeps; ++k)
+        FTD_HIP(ftd::launch_pascal_ir(fs.M, fs.T, A, B, C, D, X, h->n_sys, h->n_row, fs.Q, fs.S,
+                                      k > 0, h->stream));
+    if (h->profiling) {
+        FTD_HIP(hipEventRecord(h->ev1, h->stream));
+        FTD_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0.0f;
+        FTD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return FTD_OK;
+}
+
 namespace {
 
 // the plan of this layout on the handle (made and uploaded once per layout)

@@ -33,6 +33,13 @@ hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float*
                                    const float* D, double* X, int n_sys, int n_row, int Q, int S,
                                    int cyclic, hipStream_t st);
 
+// FP32 elimination, FP64 reduced solve and update: res = 0 solves T x = D
+// into X; res = 1 is one refinement sweep, X += T32^-1 (D - T X) with the
+// residual in FP64. Same shapes (M, maxt, Q, S) as launch_pascal_fused.
+hipError_t launch_pascal_ir(int M, int maxt, const float* A, const float* B, const float* C,
+                            const float* D, double* X, int n_sys, int n_row, int Q, int S, int res,
+                            hipStream_t st);
+
 // LDS-streamed persistent kernels (S = 32 systems per tile, two partitions per
 // wave): M rows per partition for Q partitions, or 0 when no variant holds
 // them; eligible when n_sys % 32 == 0 and the arrays are 16-byte aligned

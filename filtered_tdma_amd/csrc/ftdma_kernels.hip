@@ -307,6 +307,60 @@ __global__ __launch_bounds__(256) void k_thomas_cyclic_fused(const double* __res
         if (j < cnt) FTD_ST(d[j] + x0 * e[j], D + base + (size_t)j * ld);
 }
 
+// Reduced system of NR rows (B == 1) in LDS, row k of this lane's system at
+// index k * S, solved by Thomas in place (x to Dr). 4 rows at a time: their
+// values are read into registers first and c', d' are carried in registers,
+// so the serial chain is the arithmetic, not an LDS round trip per row.
+__device__ __forceinline__ void reduced_thomas_lds(double* Ar, double* Cr, double* Dr, int NR,
+                                                   int S) {
+    double cp = Cr[0], dp = Dr[0];
+    int k = 1;
+    for (; k + 4 <= NR; k += 4) {
+        double ak[4], ck[4], dk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ak[u] = Ar[(k + u) * S];
+            ck[u] = Cr[(k + u) * S];
+            dk[u] = Dr[(k + u) * S];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double inv = 1.0 / (1.0 - ak[u] * cp);
+            dp = inv * (dk[u] - ak[u] * dp);
+            cp = inv * ck[u];
+            Dr[(k + u) * S] = dp;
+            Cr[(k + u) * S] = cp;
+        }
+    }
+    for (; k < NR; ++k) {
+        const double ak = Ar[k * S];
+        const double inv = 1.0 / (1.0 - ak * cp);
+        dp = inv * (Dr[k * S] - ak * dp);
+        cp = inv * Cr[k * S];
+        Dr[k * S] = dp;
+        Cr[k * S] = cp;
+    }
+    double dn = dp;
+    k = NR - 2;
+    for (; k >= 3; k -= 4) {
+        double ck[4], dk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ck[u] = Cr[(k - u) * S];
+            dk[u] = Dr[(k - u) * S];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            dn = dk[u] - ck[u] * dn;
+            Dr[(k - u) * S] = dn;
+        }
+    }
+    for (; k >= 0; --k) {
+        dn = Dr[k * S] - Cr[k * S] * dn;
+        Dr[k * S] = dn;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // fused PaScaL: modified Thomas per (system, sub-partition) in registers,
 // reduced 2Q-row system in LDS, interior update, single store of x.
@@ -401,55 +455,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         double* Cr = rC + sl;
         double* Dr = rD + sl;
         if (!CYCLIC) {
-            // 4 rows at a time: their values are read into registers first
-            // and c', d' are carried in registers, so the serial chain is
-            // the arithmetic, not an LDS round trip per row
-            double cp = Cr[0], dp = Dr[0];
-            int k = 1;
-            for (; k + 4 <= NR; k += 4) {
-                double ak[4], ck[4], dk[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ak[u] = Ar[(k + u) * S];
-                    ck[u] = Cr[(k + u) * S];
-                    dk[u] = Dr[(k + u) * S];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double inv = 1.0 / (1.0 - ak[u] * cp);
-                    dp = inv * (dk[u] - ak[u] * dp);
-                    cp = inv * ck[u];
-                    Dr[(k + u) * S] = dp;
-                    Cr[(k + u) * S] = cp;
-                }
-            }
-            for (; k < NR; ++k) {
-                const double ak = Ar[k * S];
-                const double inv = 1.0 / (1.0 - ak * cp);
-                dp = inv * (Dr[k * S] - ak * dp);
-                cp = inv * Cr[k * S];
-                Dr[k * S] = dp;
-                Cr[k * S] = cp;
-            }
-            double dn = dp;
-            k = NR - 2;
-            for (; k >= 3; k -= 4) {
-                double ck[4], dk[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ck[u] = Cr[(k - u) * S];
-                    dk[u] = Dr[(k - u) * S];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    dn = dk[u] - ck[u] * dn;
-                    Dr[(k - u) * S] = dn;
-                }
-            }
-            for (; k >= 0; --k) {
-                dn = Dr[k * S] - Cr[k * S] * dn;
-                Dr[k * S] = dn;
-            }
+            reduced_thomas_lds(Ar, Cr, Dr, NR, S);
         } else {
             double* Er = rE + sl;
             for (int k = 0; k < NR; ++k) Er[k * S] = 0.0;
@@ -485,6 +491,148 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
                     x = xN;
                 else
                     x = d[j] - (a[j] * x0 + c[j] * xN);
+                FTD_ST(x, X + base + (size_t)j * ld);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 elimination with FP64 refinement (no counterpart in the reference;
+// the mixed-precision mode of the FP32-storage path). A, B, C, D are float
+// [n_row x n_sys], x float64 in X. One launch is one solve of the partitions
+// of k_pascal_fused (same lanes, same LDS layout):
+//   modified Thomas per partition in FP32 arithmetic (rows 0, 1 divided by b),
+//   the 2Q-row reduced system widened to FP64 and solved in FP64, the interior
+//   update in FP64 on the widened FP32 values.
+// RES = false: the right-hand side is D, and X receives the solution.
+// RES = true : one refinement sweep. The right-hand side is the residual
+//   r_j = D_j - (A_j x_{j-1} + B_j x_j + C_j x_{j+1}) of the current X,
+//   formed in FP64 on the widened operands (row 0 without its A term, row
+//   n_row - 1 without its C term) and rounded to FP32; X_j += delta_j.
+//   X is updated in place: a workgroup holds every partition of its systems,
+//   reads all of their x (the halo rows r0 - 1 and r0 + cnt included) before
+//   the barrier that publishes the reduced rows, and writes after it.
+// ---------------------------------------------------------------------------
+template <int M, int MAXT, int PF, int WPE, bool RES>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pascal_ir(
+    const float* __restrict__ A, const float* __restrict__ B, const float* __restrict__ C,
+    const float* __restrict__ D, double* X, int n_sys, int n_row, int Q, int lgS) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int i_raw = blockIdx.x * S + sl;
+    const bool live = i_raw < n_sys;
+    const int i = live ? i_raw : n_sys - 1;  // dead lanes shadow the last system
+    int r0, cnt;
+    part_of(n_row, Q, q, r0, cnt);
+    const size_t ld = (size_t)n_sys;
+    const size_t base = (size_t)r0 * ld + (size_t)i;
+
+    float a[M], c[M], d[M];
+    float ra[M], rb[M], rc[M], rd[M];  // raw rows; PF of them live at a time
+    double xv[M];                      // RES: x of the partition's rows
+    // RES: x of the rows just outside the partition (0 past the system's ends)
+    double xlo = 0.0, xhi = 0.0;
+    if (RES) {
+        if (r0 > 0) xlo = X[base - ld];
+        if (r0 + cnt < n_row) xhi = X[base + (size_t)cnt * ld];
+    }
+    float aN = 0.0f, cN = 0.0f, dN = 0.0f;
+
+#define FTD_IR_LOAD(j)                                        \
+    do {                                                      \
+        const int jj_ = (j) < cnt ? (j) : cnt - 1;            \
+        const size_t o_ = base + (size_t)jj_ * ld;            \
+        ra[(j)] = FTD_LD(A + o_);                             \
+        rb[(j)] = FTD_LD(B + o_);                             \
+        rc[(j)] = FTD_LD(C + o_);                             \
+        rd[(j)] = FTD_LD(D + o_);                             \
+        if (RES) xv[(j)] = X[o_];                             \
+    } while (0)
+
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) FTD_IR_LOAD(j);
+
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j + PF < M) FTD_IR_LOAD(j + PF);
+        if (j < cnt) {
+            const float aj = ra[j], bj = rb[j], cj = rc[j];
+            float dj;
+            if (RES) {
+                const int g = r0 + j;  // global row
+                const double xm = j == 0 ? xlo : xv[j > 0 ? j - 1 : 0];
+                const double xp = j == cnt - 1 ? xhi : xv[j + 1 < M ? j + 1 : j];
+                double tt = (double)bj * xv[j];
+                if (g > 0) tt = (double)aj * xm + tt;
+                if (g < n_row - 1) tt = tt + (double)cj * xp;
+                dj = (float)((double)rd[j] - tt);
+            } else {
+                dj = rd[j];
+            }
+            if (j < 2) {
+                a[j] = aj / bj;
+                c[j] = cj / bj;
+                d[j] = dj / bj;
+            } else {
+                const float inv = 1.0f / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = -inv * aj * a[j - 1];
+            }
+            if (j == cnt - 1) {
+                aN = a[j];
+                cN = c[j];
+                dN = d[j];
+            }
+        }
+    }
+#undef FTD_IR_LOAD
+#pragma unroll
+    for (int j = M - 3; j >= 1; --j) {
+        if (j <= cnt - 3) {
+            const float cj = c[j];
+            d[j] = d[j] - cj * d[j + 1];
+            a[j] = a[j] - cj * a[j + 1];
+            c[j] = -cj * c[j + 1];
+        }
+    }
+    const float r = 1.0f / (1.0f - a[1] * c[0]);
+    const float d0 = r * (d[0] - c[0] * d[1]);
+    const float a0 = r * a[0];
+    const float c0 = -r * c[0] * c[1];
+
+    const int NR = 2 * Q;
+    double* rA = lds;
+    double* rC = rA + NR * S;
+    double* rD = rC + NR * S;
+    rA[(2 * q) * S + sl] = (double)a0;
+    rC[(2 * q) * S + sl] = (double)c0;
+    rD[(2 * q) * S + sl] = (double)d0;
+    rA[(2 * q + 1) * S + sl] = (double)aN;
+    rC[(2 * q + 1) * S + sl] = (double)cN;
+    rD[(2 * q + 1) * S + sl] = (double)dN;
+    __syncthreads();
+    if (q == 0) reduced_thomas_lds(rA + sl, rC + sl, rD + sl, NR, S);
+    __syncthreads();
+
+    const double x0 = rD[(2 * q) * S + sl];
+    const double xN = rD[(2 * q + 1) * S + sl];
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < cnt) {
+                double x;
+                if (j == 0)
+                    x = x0;
+                else if (j == cnt - 1)
+                    x = xN;
+                else
+                    x = (double)d[j] - ((double)a[j] * x0 + (double)c[j] * xN);
+                if (RES) x = xv[j] + x;
                 FTD_ST(x, X + base + (size_t)j * ld);
             }
         }
@@ -2060,6 +2208,27 @@ hipError_t launch_pascal_fused_f32(int M, int maxt, const float* A, const float*
                                    int cyclic, hipStream_t st) {
     return launch_pascal_fused_t<float, float>(M, maxt, A, B, C, D, X, n_sys, n_row, Q, S, cyclic,
                                                0, st);
+}
+
+hipError_t launch_pascal_ir(int M, int maxt, const float* A, const float* B, const float* C,
+                            const float* D, double* X, int n_sys, int n_row, int Q, int S, int res,
+                            hipStream_t st) {
+    const int lgS = log2i(S);
+    const dim3 grid((n_sys + S - 1) / S), block(S * Q);
+    const size_t lds = pascal_fused_lds(Q, S, 0);
+#define X(m_, t_, pf_, w_)                                                                    \
+    if (M == m_ && maxt == t_) {                                                              \
+        if (res)                                                                              \
+            hipLaunchKernelGGL((k_pascal_ir<m_, t_, pf_, w_, true>), grid, block, lds, st, A, B, \
+                               C, D, X, n_sys, n_row, Q, lgS);                                \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_pascal_ir<m_, t_, pf_, w_, false>), grid, block, lds, st, A,  \
+                               B, C, D, X, n_sys, n_row, Q, lgS);                             \
+        return hipGetLastError();                                                             \
+    }
+    FTD_PASCAL_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 int thomas_fused_rows(int n_row) {

@@ -123,6 +123,20 @@ ftd_status ftd_solve_f32c(ftd_handle* h, const float* A, const float* B, const f
 ftd_status ftd_solve_f32(ftd_handle* h, const float* A, const float* B, const float* C,
                          const float* D, double* X);
 
+/* Extension without a reference counterpart (FP32 storage, mixed-precision
+ * refinement): A, B, C, D float, x (float64) to X. The partitions are
+ * eliminated in FP32 arithmetic; the reduced system and the interior update
+ * run in FP64. Then `sweeps` (0..FTD_MAX_REFINE_SWEEPS) refinement sweeps,
+ * each one launch: the residual D - T x in FP64, rounded to FP32, solved
+ * the same way, added to x. HBM traffic 24 + 32 * sweeps B/row. Exact kinds
+ * only (PASCAL, or one partition). ftd_solve_f32 reaches the FP64 solve of
+ * the stored system in one 24 B/row pass; this entry point is the
+ * FP32-elimination alternative, for comparison and for callers that want
+ * it. */
+#define FTD_MAX_REFINE_SWEEPS 4
+ftd_status ftd_solve_f32_refine(ftd_handle* h, const float* A, const float* B, const float* C,
+                                const float* D, double* X, int sweeps);
+
 /* FilteredTDMACUDA::last_J (filtered_tdma_cuda.cu:83-87), one value per
  * partition: J of the most recent filtered solve (blocking device->host
  * copy of n <= 2 * partitions ints; entries from P on are the imaginary

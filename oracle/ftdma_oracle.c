@@ -433,6 +433,152 @@ static int pascal_impl(double* A, const double* B, double* C, double* D, int n_s
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP32 elimination with FP64 refinement (no counterpart in the reference:  */
+/* the mixed-precision mode of the engine's FP32-storage path). One solve:  */
+/* the GPU-semantics modified Thomas of modthomas_cols in float arithmetic  */
+/* on float rows, the reduced rows widened to double and solved as in       */
+/* ftdo_pascal, the interior update in double on the widened values.        */
+/* ------------------------------------------------------------------------ */
+static void modthomas_f32_cols(float* A, const float* B, float* C, float* D,
+                               size_t ld, int c0, int c1, int m, double* R) {
+    float* A1 = ROW(A, 1); float* C1 = ROW(C, 1); float* D1 = ROW(D, 1);
+    const float* B1 = ROW(B, 1);
+    for (int i = c0; i < c1; ++i) {
+        A[i] = A[i] / B[i]; C[i] = C[i] / B[i]; D[i] = D[i] / B[i];
+        A1[i] = A1[i] / B1[i]; C1[i] = C1[i] / B1[i]; D1[i] = D1[i] / B1[i];
+    }
+    for (int j = 2; j < m; ++j) {
+        float* Aj = ROW(A, j); const float* Bj = ROW(B, j);
+        float* Cj = ROW(C, j); float* Dj = ROW(D, j);
+        const float* Am = ROW(A, j - 1);
+        const float* Cm = ROW(C, j - 1);
+        const float* Dm = ROW(D, j - 1);
+        for (int i = c0; i < c1; ++i) {
+            float inv = 1.0f / (Bj[i] - Aj[i] * Cm[i]);
+            Dj[i] = inv * (Dj[i] - Aj[i] * Dm[i]);
+            Cj[i] = inv * Cj[i];
+            Aj[i] = -inv * Aj[i] * Am[i];
+        }
+    }
+    for (int j = m - 3; j >= 1; --j) {
+        float* Aj = ROW(A, j); float* Cj = ROW(C, j); float* Dj = ROW(D, j);
+        const float* Ap = ROW(A, j + 1);
+        const float* Cp = ROW(C, j + 1);
+        const float* Dp = ROW(D, j + 1);
+        for (int i = c0; i < c1; ++i) {
+            Dj[i] = Dj[i] - Cj[i] * Dp[i];
+            Aj[i] = Aj[i] - Cj[i] * Ap[i];
+            Cj[i] = -Cj[i] * Cp[i];
+        }
+    }
+    const float* AN = ROW(A, m - 1);
+    const float* CN = ROW(C, m - 1);
+    const float* DN = ROW(D, m - 1);
+    for (int i = c0; i < c1; ++i) {
+        float r = 1.0f / (1.0f - A1[i] * C[i]);
+        float d0 = r * (D[i] - C[i] * D1[i]);
+        float a0 = r * A[i];
+        float cc0 = -r * C[i] * C1[i];
+        R[0 * ld + i] = (double)a0; R[1 * ld + i] = (double)cc0; R[2 * ld + i] = (double)d0;
+        R[3 * ld + i] = (double)AN[i]; R[4 * ld + i] = (double)CN[i]; R[5 * ld + i] = (double)DN[i];
+        D[i] = d0; A[i] = a0; C[i] = cc0;
+    }
+}
+
+int ftdo_pascal_ir(const float* A, const float* B, const float* C, const float* D, double* X,
+                   int n_sys, int n_row, int Q, int sweeps, int nthreads) {
+    if (Q < 2) return fail(-1, "pascal_ir: need Q >= 2 partitions");
+    if (sweeps < 0) return fail(-1, "pascal_ir: sweeps must be >= 0");
+    int rc = check_parts(n_row, Q);
+    if (rc) return rc;
+    const size_t ld = (size_t)n_sys;
+    const size_t nn = ld * (size_t)n_row;
+    const int n_rd = 2 * Q;
+    float* fbuf = (float*)malloc(sizeof(float) * nn * 3 + 16);
+    double* buf = (double*)malloc(sizeof(double) * ld * ((size_t)(4 * n_rd + 6 * Q)) + 8);
+    if (!fbuf || !buf) {
+        free(fbuf);
+        free(buf);
+        return fail(-3, "pascal_ir: out of memory");
+    }
+    float* Af = fbuf;
+    float* Cf = Af + nn;
+    float* Df = Cf + nn;
+    double* Ar = buf;
+    double* Br = Ar + ld * n_rd;
+    double* Cr = Br + ld * n_rd;
+    double* Dr = Cr + ld * n_rd;
+    double* Rp = Dr + ld * n_rd;
+    set_threads(nthreads);
+    const int nc = nchunks(n_sys);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nc; ++k) {
+        int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
+        for (int sw = 0; sw <= sweeps; ++sw) {
+            /* right-hand side: D, then the FP64 residual of the current x,
+             * rounded to float (every row's before any x changes) */
+            for (int j = 0; j < n_row; ++j) {
+                for (int i = c0; i < c1; ++i) {
+                    const size_t o = (size_t)j * ld + i;
+                    Af[o] = A[o];
+                    Cf[o] = C[o];
+                    if (sw == 0) {
+                        Df[o] = D[o];
+                    } else {
+                        double t = (double)B[o] * X[o];
+                        if (j > 0) t = (double)A[o] * X[o - ld] + t;
+                        if (j < n_row - 1) t = t + (double)C[o] * X[o + ld];
+                        Df[o] = (float)((double)D[o] - t);
+                    }
+                }
+            }
+            for (int p = 0; p < Q; ++p) {
+                int s, m;
+                part_rows(n_row, Q, p, &s, &m);
+                double* R = Rp + (size_t)6 * p * ld;
+                modthomas_f32_cols(ROW(Af, s), ROW(B, s), ROW(Cf, s), ROW(Df, s), ld, c0, c1, m, R);
+                for (int i = c0; i < c1; ++i) {
+                    Ar[(size_t)(2 * p) * ld + i] = R[0 * ld + i];
+                    Cr[(size_t)(2 * p) * ld + i] = R[1 * ld + i];
+                    Dr[(size_t)(2 * p) * ld + i] = R[2 * ld + i];
+                    Ar[(size_t)(2 * p + 1) * ld + i] = R[3 * ld + i];
+                    Cr[(size_t)(2 * p + 1) * ld + i] = R[4 * ld + i];
+                    Dr[(size_t)(2 * p + 1) * ld + i] = R[5 * ld + i];
+                    Br[(size_t)(2 * p) * ld + i] = 1.0;
+                    Br[(size_t)(2 * p + 1) * ld + i] = 1.0;
+                }
+            }
+            thomas_cols(Ar, Br, Cr, Dr, ld, c0, c1, n_rd, FTDO_SEM_GPU);
+            for (int p = 0; p < Q; ++p) {
+                int s, m;
+                part_rows(n_row, Q, p, &s, &m);
+                const double* x0 = Dr + (size_t)(2 * p) * ld;
+                const double* xN = Dr + (size_t)(2 * p + 1) * ld;
+                for (int j = 0; j < m; ++j) {
+                    const float* Aj = ROW(Af, s + j);
+                    const float* Cj = ROW(Cf, s + j);
+                    const float* Dj = ROW(Df, s + j);
+                    double* Xj = ROW(X, s + j);
+                    for (int i = c0; i < c1; ++i) {
+                        double v;
+                        if (j == 0)
+                            v = x0[i];
+                        else if (j == m - 1)
+                            v = xN[i];
+                        else
+                            v = (double)Dj[i] - ((double)Aj[i] * x0[i] + (double)Cj[i] * xN[i]);
+                        Xj[i] = sw == 0 ? v : Xj[i] + v;
+                    }
+                }
+            }
+        }
+    }
+    free(fbuf);
+    free(buf);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Spectral bounds from column 0 (tdma_backend.cpp:33-44,                   */
 /* filtered_tdma_kernels.cuh:46-67): |A/B|, |C/B| per row, 0 where B == 0,  */
 /* trailing slot n_row = 0.                                                  */

@@ -52,6 +52,16 @@ int ftdo_pascal(double* A, const double* B, double* C, double* D,
 int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
                        int n_sys, int n_row, int P, int G, int sem, int nthreads);
 
+/* FP32 elimination with FP64 refinement (no counterpart in the reference):
+ * float A, B, C, D [n_row x n_sys] (read only), x (double) to X. A solve is
+ * PaScaL over Q partitions with the modified Thomas in float arithmetic (GPU
+ * order: rows 0, 1 divided), the reduced system and the interior update in
+ * double on the widened values; then `sweeps` times: r = D - T x in double
+ * (row 0 without its A term, row n_row-1 without its C term), rounded to
+ * float, solved the same way, added to x. */
+int ftdo_pascal_ir(const float* A, const float* B, const float* C, const float* D, double* X,
+                   int n_sys, int n_row, int Q, int sweeps, int nthreads);
+
 /* Column-0 spectral bounds, n_row+1 entries each (tdma_backend.cpp:33-44). */
 void ftdo_set_rho(const double* A, const double* B, const double* C,
                   int n_sys, int n_row, double* A_rho, double* C_rho);

@@ -63,10 +63,11 @@ def lib() -> ctypes.CDLL:
         L.ftdo_filtered_learned.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int,
                                             c_double, c_int, fp, c_int, c_int, ip, c_int]
         L.ftdo_mlp_J.argtypes = [fp, c_int, c_int, fp]
+        L.ftdo_pascal_ir.argtypes = [fp, fp, fp, fp, dp, c_int, c_int, c_int, c_int, c_int]
         L.ftdo_lg.argtypes = [c_double]
         L.ftdo_lg.restype = ctypes.c_float
         for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_pascal_nested,
-                  L.ftdo_filtered_guard,
+                  L.ftdo_filtered_guard, L.ftdo_pascal_ir,
                   L.ftdo_cal_J,
                   L.ftdo_filtered, L.ftdo_filtered_learned, L.ftdo_mlp_J):
             f.restype = c_int
@@ -115,6 +116,19 @@ def pascal_nested(A, B, C, D, P, G, sem=SEM_GPU, nthreads=0, inplace=False):
     _check(lib().ftdo_pascal_nested(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, G, sem,
                                     nthreads))
     return D
+
+
+def pascal_ir(A, B, C, D, Q, sweeps, nthreads=0):
+    """FP32 elimination with FP64 refinement: A, B, C, D are cast to float32
+    (the stored values); returns x (float64) after the FP32 solve and
+    `sweeps` refinement sweeps, PaScaL over Q partitions."""
+    A, B, C, D = (np.ascontiguousarray(x, dtype=np.float32) for x in (A, B, C, D))
+    n_row, n_sys = A.shape
+    X = np.zeros((n_row, n_sys), dtype=np.float64)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _check(lib().ftdo_pascal_ir(A.ctypes.data_as(fp), B.ctypes.data_as(fp), C.ctypes.data_as(fp),
+                                D.ctypes.data_as(fp), _p(X), n_sys, n_row, Q, sweeps, nthreads))
+    return X
 
 
 def filtered_guard(A, B, C, D, P, J, eps, eps_direct=False, nthreads=0, inplace=False):

@@ -1,8 +1,8 @@
 """CPU restatement of the channel pressure solve (numpy), the checker of
 ftd_poisson_*.
 
-TEST INFRASTRUCTURE ONLY: tests/ and bench.py's CPU leg import it; the
-product never does.
+TEST INFRASTRUCTURE ONLY: tests/ and bench.py's CPU-baseline leg import it;
+the product never does.
 
 Follows apps/channel_cpu/PressureSolver.cpp:
   compute_rhs_          :193-212  (1/dt) div u* with the grid's dx, dy, dz
@@ -133,18 +133,12 @@ class Channel:
 
 
 def random_velocity(nx, ny, nz, seed):
-    """Face velocities ~ N(0, 1) with periodic ghosts in x and y and
-    no-penetration walls (W on the wall faces k = 1 and nz + 1 is zero)."""
-    rng = np.random.default_rng(seed)
-    sh = (nz + 2, ny + 2, nx + 2)
-    U, V, W = (rng.standard_normal(sh) for _ in range(3))
-    for F in (U, V, W):
-        F[:, :, 0] = F[:, :, nx]
-        F[:, :, nx + 1] = F[:, :, 1]
-        F[:, 0, :] = F[:, ny, :]
-        F[:, ny + 1, :] = F[:, 1, :]
-    W[1], W[nz + 1] = 0.0, 0.0
-    return U, V, W
+    """The inputs of the pressure-step tests and bench: the engine's own
+    generator (filtered_tdma_amd.cases.channel_velocity), so that both sides
+    start from the same face velocities."""
+    from filtered_tdma_amd.cases import channel_velocity
+
+    return channel_velocity(nx, ny, nz, seed)
 
 
 def refresh_ghosts(U, V, nx, ny):

@@ -181,6 +181,56 @@ def test_pascal_nested_rejects_short_groups():
         O.pascal_nested(A, B, C, D, 2, 2)  # 4 reduced rows in 2 groups of 2
 
 
+def _relerr(x, ref):
+    return float(np.max(np.max(np.abs(x - ref), axis=0) / np.max(np.abs(ref), axis=0)))
+
+
+@pytest.mark.parametrize("r_lo,r_hi", [(5.0, 500.0), (0.5, 20.0)])
+def test_pascal_ir_refines_to_the_fp64_solve(r_lo, r_hi):
+    """FP32 elimination with FP64 refinement: the FP32 solve alone is off by
+    about kappa * 2^-24 (so it really runs in FP32), each sweep multiplies
+    the error by about the same factor, and two sweeps are within 1e-12 of
+    the FP64 solve of the stored system. A wrong sign or index in the
+    residual, the elimination or the update stops the convergence."""
+    A, B, C, D, _ = cases.toeplitz_batch(256, 512, r_lo, r_hi, seed=3)
+    A32, B32, C32, D32 = (x.astype(np.float32) for x in (A, B, C, D))
+    exact = O.thomas(*(x.astype(np.float64) for x in (A32, B32, C32, D32)))
+    errs = [_relerr(O.pascal_ir(A32, B32, C32, D32, 8, k), exact) for k in range(4)]
+    assert 1e-9 < errs[0] < 1e-4, errs
+    assert errs[1] < errs[0] * 1e-3, errs
+    assert errs[2] <= 1e-12 and errs[3] <= 1e-12, errs
+
+
+def test_pascal_ir_partition_count_only_changes_rounding():
+    A, B, C, D, _ = cases.toeplitz_batch(64, 300, 5.0, 500.0, seed=4)
+    A32, B32, C32, D32 = (x.astype(np.float32) for x in (A, B, C, D))
+    x4 = O.pascal_ir(A32, B32, C32, D32, 4, 2)
+    x9 = O.pascal_ir(A32, B32, C32, D32, 9, 2)
+    assert _relerr(x4, x9) <= 1e-12
+    assert not np.array_equal(O.pascal_ir(A32, B32, C32, D32, 4, 0),
+                              O.pascal_ir(A32, B32, C32, D32, 9, 0))
+
+
+def test_pascal_ir_small_system_vs_dense_solve_in_float32():
+    """One 8-row system: the FP32 solve alone is a float32 solve of the
+    stored matrix (within float32 rounding of numpy's dense float64 solve),
+    and three sweeps reach the float64 dense solve."""
+    rng = np.random.default_rng(5)
+    n = 8
+    a = rng.uniform(-1, 1, n).astype(np.float32)
+    c = rng.uniform(-1, 1, n).astype(np.float32)
+    b = (np.abs(a) + np.abs(c) + rng.uniform(1, 2, n)).astype(np.float32)
+    d = rng.uniform(-1, 1, n).astype(np.float32)
+    T = np.diag(b.astype(np.float64)) + np.diag(a[1:].astype(np.float64), -1) + \
+        np.diag(c[:-1].astype(np.float64), 1)
+    xd = np.linalg.solve(T, d.astype(np.float64))
+    col = lambda v: v.reshape(n, 1)
+    x0 = O.pascal_ir(col(a), col(b), col(c), col(d), 2, 0)[:, 0]
+    x3 = O.pascal_ir(col(a), col(b), col(c), col(d), 2, 3)[:, 0]
+    assert np.max(np.abs(x0 - xd)) <= 1e-5 * np.max(np.abs(xd))
+    assert np.max(np.abs(x3 - xd)) <= 1e-14 * np.max(np.abs(xd))
+
+
 def test_pascal_rejects_short_partitions():
     A, B, C, D = cases.general_batch(4, 10, seed=1)
     with pytest.raises(RuntimeError):
