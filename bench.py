@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--kinds", default="pascal,filtered_v2,filtered,filtered_learned,"
                     "filtered_v2_guard,filtered_guard,filtered_learned_guard")
     ap.add_argument("--no-c3", action="store_true", help="skip the FP32-storage (C3) legs")
+    ap.add_argument("--no-periodic", action="store_true",
+                    help="skip the periodic (solve_cyclic) legs")
     ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 heat (C1) leg")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the uneven-batch and 256^3 ADI-step (C4) legs")
@@ -54,10 +56,10 @@ def parse():
     return ap.parse_args()
 
 
-def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
+def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch, cyclic=False):
     """Wall-clock of `steps` solves (each on its own copy of D) between
     synchronisations, plus per-solve device time from events on the stream
-    the kernels run on."""
+    the kernels run on; cyclic: the periodic solve (solve_cyclic)."""
     from filtered_tdma_amd import TdmaBackendGPU
 
     guard = kind.endswith("_guard")
@@ -71,9 +73,10 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     if guard:
         be.set_guard(True)
     stream = torch.cuda.current_stream()
+    solve = be.solve_cyclic if cyclic else be.solve
     for k in range(warmup):
         copies[k % len(copies)].copy_(D0)
-        be.solve(tA, tB, tC, copies[k % len(copies)])
+        solve(tA, tB, tC, copies[k % len(copies)])
     for c in copies:
         c.copy_(D0)
     torch.cuda.synchronize()
@@ -82,7 +85,7 @@ def time_kind(kind, tA, tB, tC, D0, copies, steps, warmup, torch):
     t0 = time.perf_counter()
     for k in range(steps):
         starts[k].record(stream)
-        be.solve(tA, tB, tC, copies[k])
+        solve(tA, tB, tC, copies[k])
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -626,6 +629,29 @@ def main():
               file=sys.stderr, flush=True)
     del x_pascal
 
+    # periodic systems (solve_cyclic) on the same batch: the wrap-around
+    # interface of the last partition, every kind
+    periodic = None
+    if not args.no_periodic:
+        periodic = {}
+        x_ref = None
+        for kind in ("pascal", "filtered_v2", "filtered"):
+            wall, ev_ms, path, J = time_kind(kind, tA, tB, tC, D0, copies, args.steps,
+                                             args.warmup, torch, cyclic=True)
+            ev_avg = statistics.mean(ev_ms)
+            periodic[kind] = {
+                "ms_per_step": wall * 1e3 / args.steps, "event_ms_mean": ev_avg,
+                "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (ev_avg * 1e-3)
+                / (HBM_PEAK_GBS * 1e9),
+                "path": path, "J": J}
+            if kind == "pascal":
+                x_ref = copies[-1].clone()
+            else:
+                periodic[kind]["accuracy_vs_pascal"] = accuracy(copies[-1], x_ref, EPS, torch)
+            print(f"[bench] periodic {kind}: events {ev_avg:.4f} ms ({path})", file=sys.stderr,
+                  flush=True)
+        del x_ref
+
     # C3: same shape, r ~ U[5, 500] (rho up to 0.499), FP32 storage
     if not args.no_c3:
         A3, B3, C3, D3, _ = cases.toeplitz_batch(N_SYS, N_ROW, 5.0, 500.0, seed=SEED)
@@ -745,6 +771,8 @@ def main():
         },
         "backends": results,
     }
+    if periodic is not None:
+        line["periodic"] = periodic
     if c1 is not None:
         line["c1"] = c1
     if c4 is not None:
