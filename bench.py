@@ -498,10 +498,18 @@ def long_partitions(steps, warmup, torch):
     out = {"workload": f"n_sys={n_sys} n_row={n_row} P={P}, a=c=-r b=1+2r r~U[{R_LO},{R_HI}], "
                        f"D~U[-1,1] seed {SEED}, eps_={EPS}"}
     x_ref = None
-    for kind in ("pascal", "filtered_v2", "filtered"):
-        be = TdmaBackendGPU(kind, n_sys, n_row, P, 0.0)
+    for kind in ("pascal", "filtered_v2", "filtered", "filtered_learned", "filtered_v2_guard",
+                 "filtered_guard"):
+        guard = kind.endswith("_guard")
+        base_kind = kind[:-len("_guard")] if guard else kind
+        learned = base_kind == "filtered_learned"
+        be = TdmaBackendGPU("filtered" if learned else base_kind, n_sys, n_row, P, 0.0)
         be.set_eps(EPS)
         be.set_rho_device(tA0, tB, tC0)
+        if learned:
+            be.set_j_mode(1)
+        if guard:
+            be.set_guard(True)
         # the general kernels use A and C as scratch: every solve gets fresh copies
         tA, tC, x = torch.empty_like(tA0), torch.empty_like(tC0), torch.empty_like(D0)
         stream = torch.cuda.current_stream()
@@ -518,11 +526,18 @@ def long_partitions(steps, warmup, torch):
             if k >= warmup:
                 ms.append(e0.elapsed_time(e1))
         path = be.last_path()
+        rec = be.last_J_records() if (learned or guard) else None
         be.close()
         t = statistics.median(ms)
         res = {"ms": t, "rows_per_s": rows / (t * 1e-3),
                "hbm_frac_algorithmic": BYTES_PER_ROW * rows / (t * 1e-3) / (HBM_PEAK_GBS * 1e9),
                "path": path}
+        if rec is not None:
+            res["mean_J"] = float((rec & 0xFFFF).mean())
+            if learned:
+                res["fallback_fraction"] = float(((rec >> 16) & 1).mean())
+            if guard:
+                res["guarded_system_fraction"] = float(((rec >> 17) & 1).max(axis=0).mean())
         if kind == "pascal":
             x_ref = x.clone()
         else:

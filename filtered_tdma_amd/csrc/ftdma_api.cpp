@@ -377,10 +377,9 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
         h->jsys_count = nj;
     }
-    if (learned) {
+    if (learned && fs.M) {
         // learned J: one pass (v1 sweeps, interface, J per system from the
         // MLP, checked against eps_, truncated correction)
-        if (!fs.M) return fail(FTD_ERR_UNSUPPORTED, "learned J needs a fused kernel shape");
         std::snprintf(h->path, sizeof h->path, "filtered_learned_fused M=%d T=%d S=%d%s", fs.M,
                       fs.T, fs.S, gs);
         FTD_LAUNCH(ftd::launch_filtered_fused(fs.M, fs.T, 1, 3, A, B, C, D, n_sys, n_row, P, fs.S,
@@ -420,27 +419,42 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     }
 
     if (csh) return fail(FTD_ERR_UNSUPPORTED, "complex right-hand sides need a fused kernel shape");
-    if (g)
-        return fail(FTD_ERR_UNSUPPORTED,
-                    "the guard runs on the fused and streamed kernels (partitions of at most 64 rows)");
-    // general path: sweeps in place, interface, J, final
+    // general path: sweeps in place, interface, J, final. Learned J and the
+    // guard take the v1 sweeps and a per-system J (check kernel).
     const int tiles = (n_sys + 255) / 256;
     FTD_TRY(ensure(h, h->partial, sizeof(double) * (size_t)tiles * (P << csh)));
     FTD_TRY(ensure(h, h->xbuf, xbytes));
-    std::snprintf(h->path, sizeof h->path, "filtered_v%d_general", version);
+    const bool per_sys = learned || g;
+    int* Js = nullptr;
+    if (per_sys) {
+        FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
+        h->jsys_count = nj;
+        Js = (int*)h->jsys.p;
+    }
+    if (g) FTD_TRY(ensure(h, h->R, sizeof(double) * (size_t)4 * P * n_sys));
+    if (learned)
+        std::snprintf(h->path, sizeof h->path, "filtered_learned_general%s", gs);
+    else
+        std::snprintf(h->path, sizeof h->path, "filtered_v%d_general%s", version, gs);
     int nt = 0;
     if (version == 2) {
         FTD_LAUNCH(ftd::launch_boundary_maxabs(D, n_sys, n_row, P, csh, (double*)h->partial.p, &nt, st));
         FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                          h->eps_direct, csh, Jp, st));
     }
-    FTD_LAUNCH(ftd::launch_gen_filtered_sweep(version, A, B, C, D, n_sys, n_row, P, Jp, st));
+    FTD_LAUNCH(ftd::launch_gen_filtered_sweep(per_sys ? 1 : version, A, B, C, D, n_sys, n_row, P,
+                                              Jp, learned, h->eps_value, h->eps_direct, Js, st));
     FTD_LAUNCH(ftd::launch_gen_filtered_iface(A, C, D, n_sys, n_row, P, cyclic, (double*)h->xbuf.p,
                                               (double*)h->partial.p, &nt, st));
-    if (version == 1)
+    if (version == 1 && !learned)
         FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                          h->eps_direct, csh, Jp, st));
-    FTD_LAUNCH(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, st));
+    if (per_sys)
+        FTD_LAUNCH(ftd::launch_gen_filtered_check(learned, g, A, C, D, n_sys, n_row, P, Jp,
+                                                  (double*)h->xbuf.p, Js, h->eps_value,
+                                                  h->eps_direct, (double*)h->R.p, st));
+    FTD_LAUNCH(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, Js,
+                                              st));
     return FTD_OK;
 }
 

@@ -126,13 +126,24 @@ hipError_t launch_gen_thomas_cyclic(const double* A, const double* B, double* C,
                                     double* E, int n_sys, int n_row, hipStream_t st);
 hipError_t launch_gen_pascal(double* A, const double* B, double* C, double* D, int n_sys,
                              int n_row, int P, int cyclic, double* R, double* E, hipStream_t st);
+// learned = 1 (version 1): the network's J per (partition, system) to Jsys
 hipError_t launch_gen_filtered_sweep(int version, double* A, const double* B, double* C, double* D,
-                                     int n_sys, int n_row, int P, const int* Jp, hipStream_t st);
+                                     int n_sys, int n_row, int P, const int* Jp, int learned,
+                                     double eps_value, int eps_direct, int* Jsys, hipStream_t st);
+// per-system J after the interface solve: the learned J's check against
+// eps_ and/or the guard (exact reduced solve of flagged systems, scratch
+// gbuf of 4P x n_sys doubles); records J | fallback << 16 | guard << 17 to
+// Jsys, x0 and xN of guarded systems to xbuf
+hipError_t launch_gen_filtered_check(int learned, int guard, const double* A, const double* C,
+                                     const double* D, int n_sys, int n_row, int P, const int* Jp,
+                                     double* xbuf, int* Jsys, double eps_value, int eps_direct,
+                                     double* gbuf, hipStream_t st);
 hipError_t launch_gen_filtered_iface(const double* A, const double* C, const double* D, int n_sys,
                                      int n_row, int P, int cyclic, double* xbuf, double* partial,
                                      int* n_tiles, hipStream_t st);
+// Jsys != nullptr: J per (partition, system) from its low 16 bits
 hipError_t launch_gen_filtered_final(const double* A, const double* C, double* D, int n_sys,
                                      int n_row, int P, const int* Jp, const double* xbuf,
-                                     hipStream_t st);
+                                     const int* Jsys, hipStream_t st);
 
 }  // namespace ftd

@@ -1258,6 +1258,34 @@ __device__ __forceinline__ bool guard_flag(const double (&a)[M], const double (&
     return fabs(c0p * xN) > thr || fabs(aN * x0) > thr || ta > thr || tc > thr;
 }
 
+// The learned J of one (system, partition), checked (PASS 3): the network's
+// Jl where the first term it drops on each side, |a'_{Jl+1} x0| and
+// |c'_{m-2-Jl} xN|, is at most eps_; else the full correction, m - 1.
+// Returns J | (Jl failed) << 16. dropped_exceeds picks the two coefficients
+// by a select chain and multiplies once (products per row would be shared
+// with the final pass and held live until then).
+template <int M>
+__device__ __forceinline__ bool dropped_exceeds(const double (&a)[M], const double (&c)[M], int m,
+                                                int J, double x0, double xN, double eps) {
+    const int kL = (J < m - 2 ? J : m - 2) + 1;              // first a-term J drops
+    const int kR = ((m - 1) - J > 1 ? (m - 1) - J : 1) - 1;  // first c-term J drops
+    double akL = 0.0, ckR = 0.0;
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        if (j == kL) akL = a[j];
+        if (j == kR) ckR = c[j];
+    }
+    const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
+    const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
+    return ta > eps || tc > eps;
+}
+
+template <int M>
+__device__ __forceinline__ int learned_J_checked(const double (&a)[M], const double (&c)[M], int m,
+                                                 int Jl, double x0, double xN, double eps) {
+    return dropped_exceeds<M>(a, c, m, Jl, x0, xN, eps) ? (m - 1) | (1 << 16) : Jl;
+}
+
 template <int M, int MAXT, int VERSION, int PASS, int PF, int WPE, int CSH, int GRD>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_filtered_fused(const double* __restrict__ A,
                                                          const double* __restrict__ B,
@@ -1430,22 +1458,9 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         int Jl = mlp_J((float)fmin(f_rho, 1.0), lg_feature(f_d0), lg_feature(f_dN),
                        lg_feature(f_dmax), lg_feature(eps));
         Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
-        const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;                   // first a-term dropped
-        const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;       // first c-term dropped
-        // pick a'[kL] and c'[kR] first and multiply once: products per row
-        // would be shared with the final pass and held live until then
-        double akL = 0.0, ckR = 0.0;
-#pragma unroll
-        for (int j = 1; j < M; ++j) {
-            if (j == kL) akL = a[j];
-            if (j == kR) ckR = c[j];
-        }
-        const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
-        const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
-        const bool fallback = ta > eps || tc > eps;
-        Jf = fallback ? m - 1 : Jl;
-        if (!GRD && Jsys && live) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
-        if (GRD && fallback) Jf |= 1 << 16;  // the record keeps the fallback flag
+        const int rec = learned_J_checked<M>(a, c, m, Jl, x0, xN, eps);
+        if (!GRD && Jsys && live) Jsys[(size_t)p * ld + i] = rec;
+        Jf = GRD ? rec : (rec & 0xFFFF);  // under the guard the record keeps the fallback flag
     }
     if (GRD) {
         const int fb = Jf & (1 << 16);
@@ -1678,20 +1693,9 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 int Jl = mlp_J((float)fmin(f_rho, 1.0), lg_feature(f_d0), lg_feature(f_dN),
                                lg_feature(f_dmax), lg_feature(eps));
                 Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
-                const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;
-                const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;
-                double akL = 0.0, ckR = 0.0;
-#pragma unroll
-                for (int j = 1; j < M; ++j) {
-                    if (j == kL) akL = a[j];
-                    if (j == kR) ckR = c[j];
-                }
-                const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
-                const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
-                const bool fallback = ta > eps || tc > eps;
-                Jf = fallback ? m - 1 : Jl;
-                if (!GRD) Jsys[(size_t)p * ld + i] = Jf | (fallback ? (1 << 16) : 0);
-                if (GRD && fallback) Jf |= 1 << 16;  // the record keeps the fallback flag
+                const int rec = learned_J_checked<M>(a, c, m, Jl, x0, xN, eps);
+                if (!GRD) Jsys[(size_t)p * ld + i] = rec;
+                Jf = GRD ? rec : (rec & 0xFFFF);  // under the guard the record keeps the flag
             }
             if (GRD) {
                 // the guard (as k_filtered_fused): the dropped couplings and the
@@ -1954,14 +1958,20 @@ __global__ __launch_bounds__(256) void k_gen_pascal_update(const double* __restr
 }
 
 // Filtered sweeps in place: k_fwd_pass_v{1,2} + k_bwd_pass_v{1,2} with the
-// fused pack (filtered_tdma_kernels.cuh:69-194)
-template <int VERSION>
+// fused pack (filtered_tdma_kernels.cuh:69-194). FEAT (learned J, v1
+// sweeps): the network's features are taken from the raw rows as the
+// forward pass reads them; its J of this (system, partition), clamped to
+// [0, m-1], goes to Jsys[p][i] for k_gen_filtered_check.
+template <int VERSION, bool FEAT>
 __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__ A,
                                                             const double* __restrict__ B,
                                                             double* __restrict__ C,
                                                             double* __restrict__ D, int n_sys,
                                                             int n_row, int P,
-                                                            const int* __restrict__ Jp) {
+                                                            const int* __restrict__ Jp,
+                                                            double eps_value, int eps_direct,
+                                                            int* __restrict__ Jsys) {
+    static_assert(!FEAT || VERSION == 1, "the learned J runs the v1 sweeps");
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     if (i >= n_sys) return;
@@ -1973,8 +1983,17 @@ __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__
     double* Cp = C + (size_t)s * ld + i;
     double* Dp = D + (size_t)s * ld + i;
     const int J = (VERSION == 2) ? Jp[p] : m;
+    // features: max |a/b|, |c/b| over rows 1..m-1, raw d of rows 0 and m-1,
+    // max |d| (as k_filtered_fused takes them)
+    double f_rho = 0.0, f_d0 = 0.0, f_dN = 0.0, f_dmax = 0.0;
     for (int j = 0; j < 2; ++j) {
         const size_t o = (size_t)j * ld;
+        if (FEAT) {
+            const double aj = Ap[o], bj = Bp[o], cj = Cp[o], dj = Dp[o];
+            if (j >= 1) f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(bj));
+            if (j == 0) f_d0 = dj;
+            f_dmax = fmax(f_dmax, fabs(dj));
+        }
         const double r0 = 1.0 / Bp[o];
         Ap[o] *= r0;
         Cp[o] *= r0;
@@ -1985,6 +2004,12 @@ __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__
     for (; j <= J && j < m; ++j) {
         const size_t o = (size_t)j * ld;
         const double aj = Ap[o];
+        if (FEAT) {
+            const double cj = Cp[o], dj = Dp[o];
+            f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(Bp[o]));
+            if (j == m - 1) f_dN = dj;
+            f_dmax = fmax(f_dmax, fabs(dj));
+        }
         const double inv = 1.0 / (Bp[o] - aj * cm);
         dm = inv * (Dp[o] - aj * dm);
         cm = inv * Cp[o];
@@ -1992,6 +2017,13 @@ __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__
         Ap[o] = am;
         Cp[o] = cm;
         Dp[o] = dm;
+    }
+    if (FEAT) {
+        const double Nl = (double)(m * P);
+        const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        int Jl = mlp_J((float)fmin(f_rho, 1.0), lg_feature(f_d0), lg_feature(f_dN),
+                       lg_feature(f_dmax), lg_feature(eps));
+        Jsys[(size_t)p * ld + i] = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
     }
     if (VERSION == 2) {
         for (j = (J + 1 > 1 ? J + 1 : 1); j < m; ++j) {
@@ -2072,13 +2104,115 @@ __global__ __launch_bounds__(256) void k_gen_filtered_iface(const double* __rest
         partial[(size_t)blockIdx.x * P + p] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// truncated final correction (k_final_pass, filtered_tdma_kernels.cuh:196-218)
+// Per-system J of the general path (one thread per system, its partitions
+// in turn), after k_gen_filtered_iface wrote x0, xN to xbuf:
+//   LEARNED: the network's J (Jsys, from k_gen_filtered_sweep<1, true>)
+//     checked as learned_J_checked does. Otherwise J = Jp[p].
+//   GRD: guard_flag's test on the values the sweeps left in A, C, D (a', c'
+//     after the backward sweep, a0', d0' of row 0, the forward aN, cN, dN of
+//     row m-1, c0' = -r c0 c1 recomputed); a flagged system is solved
+//     exactly, guard_reduced_thomas over its 2P reduced rows (c', d' kept in
+//     gbuf [4P x n_sys]), its x0, xN replaced in xbuf and J = m - 1.
+// Jsys[p][i] = J | fallback << 16 | guard << 17 for the final pass.
+template <bool LEARNED, bool GRD>
+__global__ __launch_bounds__(256) void k_gen_filtered_check(
+    const double* __restrict__ A, const double* __restrict__ C, const double* __restrict__ D,
+    int n_sys, int n_row, int P, const int* __restrict__ Jp, double* __restrict__ xbuf,
+    int* __restrict__ Jsys, double eps_value, int eps_direct, double* __restrict__ gbuf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sys) return;
+    const size_t ld = (size_t)n_sys;
+    auto at = [&](const double* X, int row) { return X[(size_t)row * ld + i]; };
+    bool sys = false;
+    for (int p = 0; p < P; ++p) {
+        int s, m;
+        part_of(n_row, P, p, s, m);
+        const double x0 = xbuf[(size_t)(2 * p) * ld + i];
+        const double xN = xbuf[(size_t)(2 * p + 1) * ld + i];
+        const double Nl = (double)(m * P);
+        const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
+        int Jf = LEARNED ? Jsys[(size_t)p * ld + i] : Jp[p];
+        int fb = 0;
+        if (LEARNED) {
+            // learned_J_checked on the coefficients in memory
+            auto fails = [&](int J) {
+                const int kL = (J < m - 2 ? J : m - 2) + 1;
+                const int kR = ((m - 1) - J > 1 ? (m - 1) - J : 1) - 1;
+                const double ta = kL <= m - 2 ? fabs(at(A, s + kL) * x0) : 0.0;
+                const double tc = kR >= 1 ? fabs(at(C, s + kR) * xN) : 0.0;
+                return ta > eps || tc > eps;
+            };
+            if (fails(Jf)) {
+                Jf = m - 1;
+                fb = 1 << 16;
+            }
+        }
+        if (GRD) {
+            const double thr = eps * 0.25;
+            const int kL = (Jf < m - 2 ? Jf : m - 2) + 1;
+            const int kR = ((m - 1) - Jf > 1 ? (m - 1) - Jf : 1) - 1;
+            const double ta = kL <= m - 2 ? fabs(at(A, s + kL) * x0) : 0.0;
+            const double tc = kR >= 1 ? fabs(at(C, s + kR) * xN) : 0.0;
+            const double c0 = at(C, s), c1 = at(C, s + 1);
+            const double rr = 1.0 / (1.0 - at(A, s + 1) * c0);
+            const double c0p = -rr * c0 * c1;
+            const double aN = at(A, s + m - 1);
+            sys = sys || fabs(c0p * xN) > thr || fabs(aN * x0) > thr || ta > thr || tc > thr;
+        }
+        Jsys[(size_t)p * ld + i] = Jf | fb;
+    }
+    if (!GRD || !sys) return;
+    // the exact reduced solve: rows 2q (a0', 1, c0' | d0') and 2q + 1
+    // (aN, 1, cN | dN), in guard_reduced_thomas's order
+    auto g = [&](int q, int k) -> double& { return gbuf[(size_t)(4 * q + k) * ld + i]; };
+    double cp = 0.0, dp = 0.0;
+    for (int q = 0; q < P; ++q) {
+        int s, m;
+        part_of(n_row, P, q, s, m);
+        const double c0 = at(C, s), c1 = at(C, s + 1);
+        const double rr = 1.0 / (1.0 - at(A, s + 1) * c0);
+        const double c0p = -rr * c0 * c1;
+        if (q == 0) {
+            cp = c0p;
+            dp = at(D, s);
+        } else {
+            const double ak = at(A, s);
+            const double inv = 1.0 / (1.0 - ak * cp);
+            dp = inv * (at(D, s) - ak * dp);
+            cp = inv * c0p;
+        }
+        g(q, 0) = cp;
+        g(q, 1) = dp;
+        const double ak = at(A, s + m - 1);
+        const double inv = 1.0 / (1.0 - ak * cp);
+        dp = inv * (at(D, s + m - 1) - ak * dp);
+        cp = inv * at(C, s + m - 1);
+        g(q, 2) = cp;
+        g(q, 3) = dp;
+    }
+    double xn = dp;
+    for (int q = P - 1; q >= 0; --q) {
+        int s, m;
+        part_of(n_row, P, q, s, m);
+        if (q < P - 1) xn = g(q, 3) - g(q, 2) * xn;
+        xbuf[(size_t)(2 * q + 1) * ld + i] = xn;
+        xn = g(q, 1) - g(q, 0) * xn;
+        xbuf[(size_t)(2 * q) * ld + i] = xn;
+        const int fb = Jsys[(size_t)q * ld + i] & (1 << 16);
+        Jsys[(size_t)q * ld + i] = (m - 1) | fb | (1 << 17);
+    }
+}
+
+// truncated final correction (k_final_pass, filtered_tdma_kernels.cuh:196-218);
+// J per partition from Jp, or per (partition, system) from Jsys (low 16
+// bits) where the general path's check kernel ran
 __global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __restrict__ A,
                                                             const double* __restrict__ C,
                                                             double* __restrict__ D, int n_sys,
                                                             int n_row, int P,
                                                             const int* __restrict__ Jp,
-                                                            const double* __restrict__ xbuf) {
+                                                            const double* __restrict__ xbuf,
+                                                            const int* __restrict__ Jsys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     if (i >= n_sys) return;
@@ -2092,7 +2226,7 @@ __global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __rest
     const double* Cp = C + (size_t)s * ld + i;
     Dp[0] = x0;
     Dp[(size_t)(m - 1) * ld] = xN;
-    const int J = Jp[p];
+    const int J = Jsys ? (Jsys[(size_t)p * ld + i] & 0xFFFF) : Jp[p];
     const int jL = J < m - 2 ? J : m - 2;
     const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
     for (int j = 1; j <= jL; ++j) {
@@ -2542,14 +2676,39 @@ hipError_t launch_gen_pascal(double* A, const double* B, double* C, double* D, i
 }
 
 hipError_t launch_gen_filtered_sweep(int version, double* A, const double* B, double* C, double* D,
-                                     int n_sys, int n_row, int P, const int* Jp, hipStream_t st) {
+                                     int n_sys, int n_row, int P, const int* Jp, int learned,
+                                     double eps_value, int eps_direct, int* Jsys, hipStream_t st) {
     const dim3 g2((n_sys + 255) / 256, P);
+    if (learned && version != 1) return hipErrorInvalidValue;
     if (version == 2)
-        hipLaunchKernelGGL(k_gen_filtered_sweep<2>, g2, dim3(256), 0, st, A, B, C, D, n_sys, n_row,
-                           P, Jp);
+        hipLaunchKernelGGL((k_gen_filtered_sweep<2, false>), g2, dim3(256), 0, st, A, B, C, D,
+                           n_sys, n_row, P, Jp, eps_value, eps_direct, Jsys);
+    else if (learned)
+        hipLaunchKernelGGL((k_gen_filtered_sweep<1, true>), g2, dim3(256), 0, st, A, B, C, D,
+                           n_sys, n_row, P, Jp, eps_value, eps_direct, Jsys);
     else
-        hipLaunchKernelGGL(k_gen_filtered_sweep<1>, g2, dim3(256), 0, st, A, B, C, D, n_sys, n_row,
-                           P, Jp);
+        hipLaunchKernelGGL((k_gen_filtered_sweep<1, false>), g2, dim3(256), 0, st, A, B, C, D,
+                           n_sys, n_row, P, Jp, eps_value, eps_direct, Jsys);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_filtered_check(int learned, int guard, const double* A, const double* C,
+                                     const double* D, int n_sys, int n_row, int P, const int* Jp,
+                                     double* xbuf, int* Jsys, double eps_value, int eps_direct,
+                                     double* gbuf, hipStream_t st) {
+    const dim3 g((n_sys + 255) / 256);
+#define LAUNCH(l_, g_)                                                                          \
+    hipLaunchKernelGGL((k_gen_filtered_check<l_, g_>), g, dim3(256), 0, st, A, C, D, n_sys, n_row, \
+                       P, Jp, xbuf, Jsys, eps_value, eps_direct, gbuf)
+    if (learned && guard)
+        LAUNCH(true, true);
+    else if (learned)
+        LAUNCH(true, false);
+    else if (guard)
+        LAUNCH(false, true);
+    else
+        return hipErrorInvalidValue;
+#undef LAUNCH
     return hipGetLastError();
 }
 
@@ -2565,10 +2724,10 @@ hipError_t launch_gen_filtered_iface(const double* A, const double* C, const dou
 
 hipError_t launch_gen_filtered_final(const double* A, const double* C, double* D, int n_sys,
                                      int n_row, int P, const int* Jp, const double* xbuf,
-                                     hipStream_t st) {
+                                     const int* Jsys, hipStream_t st) {
     const dim3 g2((n_sys + 255) / 256, P);
     hipLaunchKernelGGL(k_gen_filtered_final, g2, dim3(256), 0, st, A, C, D, n_sys, n_row, P, Jp,
-                       xbuf);
+                       xbuf, Jsys);
     return hipGetLastError();
 }
 

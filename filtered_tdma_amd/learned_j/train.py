@@ -136,8 +136,8 @@ def samples(rng, count):
     X, Y, S, Ms = [], [], [], []
     while len(Y) < count:
         n_sys = 512
-        m = int(rng.choice([8, 16, 32, 48, 64]))
-        P = int(rng.choice([2, 4, 8]))
+        m = int(rng.choice([8, 16, 32, 48, 64, 128, 256, 512]))
+        P = int(rng.choice([2, 4, 8] if m <= 64 else [2, 4]))
         n = m * P
         kind = rng.integers(5)
         if kind == 3:

@@ -185,8 +185,11 @@ ftd_status ftd_autotune(ftd_handle* h, double* A, double* B, double* C, const do
  * the partition's coupling bound, right-hand-side magnitudes and eps_,
  * instead of one cal_J per partition; the first correction term it drops is
  * checked against eps_ and the full correction (J = m-1) is used where it
- * exceeds it. One pass instead of v1's two (no global J). Mode 0 (default)
- * is the reference's cal_J. Register-resident kernels only. */
+ * exceeds it. One pass instead of v1's two (no global J) on partitions of
+ * at most 64 rows; longer partitions run on the general kernels (the J
+ * records kept in device memory). Mode 0 (default) is the reference's
+ * cal_J. Real solves, periodic or not; complex ones on partitions of at
+ * most 64 rows. */
 ftd_status ftd_set_j_mode(ftd_handle* h, int mode);
 
 /* Extension (no counterpart in the reference): the guard of the filtered
@@ -200,9 +203,10 @@ ftd_status ftd_set_j_mode(ftd_handle* h, int mode);
  * solves a system exactly (its 2P-row reduced system by Thomas, then the
  * full correction) where any of them exceeds eps_/4, so that x stays within
  * about eps_ of the exact solve. Off by default (the reference's bits).
- * Non-periodic solves on the fused and streamed kernels (partitions of at
- * most 64 rows); others return FTD_ERR_UNSUPPORTED. ftd_last_J_systems
- * reports which systems it solved exactly. */
+ * Non-periodic real solves, any partition length (partitions longer than 64
+ * rows run on the general kernels, with global scratch of 4 P n_sys
+ * doubles); periodic and complex ones return FTD_ERR_UNSUPPORTED.
+ * ftd_last_J_systems reports which systems it solved exactly. */
 ftd_status ftd_set_guard(ftd_handle* h, int enable);
 
 /* Extension: the J used per (partition, system) by the last learned-mode or

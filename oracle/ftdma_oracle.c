@@ -1013,6 +1013,18 @@ int ftdo_mlp_J(const float* weights, int hidden, int margin, const float* featur
 
 float ftdo_lg(double x) { return lg_feature(x); }
 
+/* Whether truncating partition (s, m) of system i at depth J drops a term
+ * above eps: the first a-term |a'_{kL} x0| or the first c-term |c'_{kR} xN|
+ * (kL = min(J, m-2) + 1, kR = max(m-1-J, 1) - 1), on the swept A, C. */
+static int dropped_exceeds(const double* A, const double* C, size_t ld, int s, int m, size_t i,
+                           int J, double x0, double xN, double eps) {
+    const int kL = (J < m - 2 ? J : m - 2) + 1;
+    const int kR = ((m - 1) - J > 1 ? (m - 1) - J : 1) - 1;
+    const double ta = kL <= m - 2 ? fabs(ROW(A, s + kL)[i] * x0) : 0.0;
+    const double tc = kR >= 1 ? fabs(ROW(C, s + kR)[i] * xN) : 0.0;
+    return ta > eps || tc > eps;
+}
+
 int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
                           int n_sys, int n_row, int P, int cyclic,
                           double eps_value, int eps_direct,
@@ -1074,12 +1086,9 @@ int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
             int Jl = mlp_J(weights, hidden, margin, fv);
             Jl = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
             const double x0 = D0[i], xN = DN[i];
-            /* first a-term and first c-term the truncation drops */
-            const int kL = (Jl < m - 2 ? Jl : m - 2) + 1;
-            const int kR = ((m - 1) - Jl > 1 ? (m - 1) - Jl : 1) - 1;
-            const double ta = kL <= m - 2 ? fabs(ROW(A, s + kL)[i] * x0) : 0.0;
-            const double tc = kR >= 1 ? fabs(ROW(C, s + kR)[i] * xN) : 0.0;
-            const int fallback = ta > eps || tc > eps;
+            /* the network's J where the first a-term and the first c-term it
+             * drops are at most eps_; else the full correction, m-1 */
+            const int fallback = dropped_exceeds(A, C, ld, s, m, i, Jl, x0, xN, eps);
             const int J = fallback ? m - 1 : Jl;
             if (J_out) J_out[(size_t)p * ld + i] = J | (fallback << 16);
             const int left_end = J < m - 2 ? J : m - 2;

@@ -71,6 +71,10 @@ CASES = [
     (Kind.FILTERED_V2, False, 300, 200, 4, (5.0, 500.0), 1e-4),   # fused, rho to 0.499
     (Kind.FILTERED, True, 257, 128, 8, (0.05, 3.0), 1e-6),        # fused learned, m = 16
     (Kind.FILTERED_V2, False, 512, 256, 16, (0.5, 20.0), 1e-3),   # 16 partitions: fused
+    (Kind.FILTERED, False, 300, 600, 2, (0.5, 20.0), 1e-3),       # m = 300: general kernels
+    (Kind.FILTERED_V2, False, 256, 1024, 2, (0.5, 20.0), 1e-3),   # general, m = 512
+    (Kind.FILTERED, True, 200, 400, 4, (0.05, 3.0), 1e-6),        # general learned, m = 100
+    (Kind.FILTERED_V2, False, 130, 387, 3, (5.0, 500.0), 1e-4),   # general, uneven 129/129/129
 ]
 
 
@@ -104,22 +108,15 @@ def test_guard_complex_parts_guarded_separately():
         np.testing.assert_array_equal(rec[:, par::2], recr, err_msg=path)
 
 
-def test_guard_refuses_periodic_and_long_partitions():
+def test_guard_refuses_periodic_solves():
     A, B, C, D, _ = cases.toeplitz_batch(64, 96, 0.5, 20.0, seed=1)
-    be = TdmaBackendGPU(Kind.FILTERED, 64, 96, 4, 1e-3)
-    try:
-        be.set_guard(True)
-        tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
-        with pytest.raises(FtdError):
-            be.solve_cyclic(tA, tB, tC, tD)
-    finally:
-        be.close()
-    A, B, C, D, _ = cases.toeplitz_batch(64, 300, 0.5, 20.0, seed=1)
-    be = TdmaBackendGPU(Kind.FILTERED_V2, 64, 300, 2, 1e-3)
-    try:
-        be.set_guard(True)
-        tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
-        with pytest.raises(FtdError):
-            be.solve(tA, tB, tC, tD)
-    finally:
-        be.close()
+    for P, n_row in ((4, 96), (2, 300)):  # fused and general kernels
+        A, B, C, D, _ = cases.toeplitz_batch(64, n_row, 0.5, 20.0, seed=1)
+        be = TdmaBackendGPU(Kind.FILTERED, 64, n_row, P, 1e-3)
+        try:
+            be.set_guard(True)
+            tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
+            with pytest.raises(FtdError):
+                be.solve_cyclic(tA, tB, tC, tD)
+        finally:
+            be.close()

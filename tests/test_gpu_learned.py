@@ -59,14 +59,19 @@ def run_learned(A, B, C, D, P, eps_c, cyclic=False, complex_rhs=False):
     (1024, 512, 8, False),   # the streamed pass (m = 64, n_sys % 32 == 0)
     (2048, 200, 4, False),   # streamed, partitions of 50 rows
     (512, 384, 8, True),     # streamed, periodic
+    (200, 300, 2, False),    # partitions of 150 rows: the general kernels
+    (130, 1000, 4, True),    # general, periodic, m = 250
+    (257, 520, 3, False),    # general, uneven partitions 174/173/173
 ])
 @pytest.mark.parametrize("eps_c", [1e-2, 1e-8])
 def test_learned_bitwise(n_sys, n_row, P, cyclic, eps_c):
     note(f"learned n_sys={n_sys} n_row={n_row} P={P} cyclic={cyclic} eps_c={eps_c}")
     A, B, C, D = mixed_scale(n_sys, n_row, seed=n_row + P)
     X, J, fb, path = run_learned(A, B, C, D, P, eps_c, cyclic=cyclic)
-    streamed = n_sys % 32 == 0 and (n_row + P - 1) // P > 32 and (n_row + P - 1) // P <= 64
-    assert path.startswith("filtered_learned_" + ("stream" if streamed else "fused")), path
+    m_max = (n_row + P - 1) // P
+    streamed = n_sys % 32 == 0 and 32 < m_max <= 64
+    shape = "stream" if streamed else ("fused" if m_max <= 64 else "general")
+    assert path.startswith("filtered_learned_" + shape), path
     xr, Jr = O.filtered_learned(A, B, C, D, P, eps_c, cyclic=cyclic)
     np.testing.assert_array_equal(J, Jr & 0xFFFF)
     np.testing.assert_array_equal(fb, Jr >> 16)
