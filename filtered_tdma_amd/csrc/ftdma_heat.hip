@@ -253,6 +253,23 @@ struct RowAt {
     }
 };
 
+// d of row j of one z partition formed from th as k_rhs_z forms it (the
+// z sweep then needs no rhs pass): c0 is the cell of the partition's first
+// row, k0 its z index; rows past the partition repeat its last row
+struct RowRhsZ {
+    const double* th;
+    const double* cz;
+    size_t c0, sy, sz;
+    int cnt, k0;
+    double xa, xc, ya, yc, za, zc, diag, cxy;
+    __device__ __forceinline__ double operator()(int j) const {
+        const int jj = j < cnt ? j : cnt - 1;
+        const size_t c = c0 + (size_t)jj * sz;
+        return (xc * th[c + 1] + xa * th[c - 1]) + (yc * th[c + sy] + ya * th[c - sy]) +
+               (zc * th[c + sz] + za * th[c - sz]) + diag * th[c] + cxy * cz[k0 + jj];
+    }
+};
+
 // One lane per (system, partition) with the partition's rows in registers.
 // rows(j) returns d of local row j (clamped to the partition), Store(j, x)
 // writes x of local row j. The d loads run PF rows ahead of their use. Restates k_thomas_fused (Q == 1) and
@@ -425,11 +442,16 @@ __device__ __forceinline__ void adi_solve_lane(int N, int Q, int q, int r0, int 
 
 // Z (dir 2) and Y (dir 1) sweeps: systems consecutive in x, so a wave's row
 // loads are contiguous; rows strided by a plane (Z) or a line (Y). The
-// solution goes back into rhs.
-template <int M, bool THOMAS>
+// solution goes back into rhs. RHSZ (the z sweep, first of the step): d is
+// formed from th by the explicit stencil (RowRhsZ) instead of read from rhs,
+// so the step has no separate rhs pass; th is not written until the x sweep.
+template <int M, bool THOMAS, bool RHSZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_yz(const double* __restrict__ th, double* rhs, Dims g,
                                                 int nz, int dir, double sa, double sb, double sc,
-                                                int Q, int lgS) {
+                                                int Q, int lgS, Stencil st,
+                                                const double* __restrict__ cx,
+                                                const double* __restrict__ cy,
+                                                const double* __restrict__ cz) {
     extern __shared__ double lds[];
     const int S = 1 << lgS;
     const int t = threadIdx.x;
@@ -450,10 +472,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
         cnt = base + (q < rem ? 1 : 0);
     }
     double* const pbase = rhs + here0 + (size_t)r0 * rs;
-    RowAt rows{pbase, rs, cnt};
-    adi_solve_lane<M, (M == 16 ? 16 : M == 32 ? 8 : 4), THOMAS>(
-        N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1) * rs, rs, live,
-        lds, rows, [&](int j, double x) { pbase[(size_t)j * rs] = x; });
+    if (RHSZ) {
+        RowRhsZ rows{th, cz, here0 + (size_t)r0 * rs, (size_t)g.px, rs, cnt, r0 + 1,
+                     st.xa, st.xc, st.ya, st.yc, st.za, st.zc, 1.0 + st.xb + st.yb + st.zb,
+                     st.c0 * cx[ii] * cy[oo]};
+        adi_solve_lane<M, (M >= 32 ? 2 : 4), THOMAS>(N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0,
+                                     here0 + (size_t)(N - 1) * rs, rs, live, lds, rows,
+                                     [&](int j, double x) { pbase[(size_t)j * rs] = x; });
+    } else {
+        RowAt rows{pbase, rs, cnt};
+        adi_solve_lane<M, (M == 16 ? 16 : M == 32 ? 8 : 4), THOMAS>(
+            N, Q, q, r0, cnt, S, sl, sa, sb, sc, th, here0, here0 + (size_t)(N - 1) * rs, rs,
+            live, lds, rows, [&](int j, double x) { pbase[(size_t)j * rs] = x; });
+    }
 }
 
 // line stride of the x sweep's LDS tile, in doubles: even, so that every
@@ -614,6 +645,7 @@ struct ftd_heat {
         size_t lds = 0;
     } fused[3];
     int use_fused = 1;
+    int fuse_rhs = 0;  // the fused z sweep forms the rhs (no k_rhs_z pass) this step
 };
 
 namespace {
@@ -717,9 +749,12 @@ hipError_t launch_adi_t(ftd_heat* h, int d, hipStream_t st) {
     if (d == 0) {
         hipLaunchKernelGGL((k_adi_x<M, TH>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz,
                            sa[0], sb[0], sc[0], f.Q, f.lgS);
+    } else if (d == 2 && h->fuse_rhs) {
+        hipLaunchKernelGGL((k_adi_yz<M, TH, true>), grid, block, f.lds, st, h->th, h->rhs, g,
+                           h->nz, d, sa[d], sb[d], sc[d], f.Q, f.lgS, h->s, h->cx, h->cy, h->cz);
     } else {
-        hipLaunchKernelGGL((k_adi_yz<M, TH>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz, d,
-                           sa[d], sb[d], sc[d], f.Q, f.lgS);
+        hipLaunchKernelGGL((k_adi_yz<M, TH, false>), grid, block, f.lds, st, h->th, h->rhs, g,
+                           h->nz, d, sa[d], sb[d], sc[d], f.Q, f.lgS, h->s, h->cx, h->cy, h->cz);
     }
     return hipGetLastError();
 }
@@ -739,8 +774,14 @@ hipError_t launch_adi(ftd_heat* h, int d, hipStream_t st) {
 // dynamic LDS above the default limit must be declared for the kernel
 template <int M, bool TH>
 hipError_t allow_lds_t(int d, size_t bytes) {
-    const void* k = d == 0 ? (const void*)k_adi_x<M, TH> : (const void*)k_adi_yz<M, TH>;
-    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (d == 0)
+        return hipFuncSetAttribute((const void*)k_adi_x<M, TH>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const hipError_t e = hipFuncSetAttribute((const void*)k_adi_yz<M, TH, false>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess || d != 2) return e;
+    return hipFuncSetAttribute((const void*)k_adi_yz<M, TH, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t allow_lds(const ftd_heat::Fused& f, int d) {
@@ -773,10 +814,21 @@ ftd_status enqueue_step(ftd_heat* h) {
     hipStream_t st = h->stream;
     const dim3 b2(16, 16);
     launch_dirichlet(h, st);
-    const int kb = h->rhs_kb;
-    hipLaunchKernelGGL(k_rhs_z, dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kb - 1) / kb)),
-                       dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz, kb);
-    HH(hipGetLastError());
+    // the fused z sweep forms the rhs itself (k_adi_yz<..., RHSZ>), except a
+    // one-partition (Thomas) sweep with fewer workgroups than CUs: that one
+    // is a serial chain per lane on part of the chip, and the separate rhs
+    // pass, spread over every CU, costs less than the stencil loads added to
+    // the chain (64^3: 0.057 against 0.0595 ms per step)
+    const ftd_heat::Fused& fz = h->fused[2];
+    const int z_groups = (g.nx * g.ny + (1 << fz.lgS) - 1) >> fz.lgS;
+    h->fuse_rhs = h->use_fused && fz.on && !(fz.thomas && z_groups < device_cus());
+    if (!h->fuse_rhs) {
+        const int kb = h->rhs_kb;
+        hipLaunchKernelGGL(k_rhs_z,
+                           dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kb - 1) / kb)),
+                           dim3(64, 4), 0, st, h->th, h->rhs, g, h->s, h->cx, h->cy, h->cz, kb);
+        HH(hipGetLastError());
+    }
     const size_t n_in = (size_t)g.nx * g.ny * h->nz;
     const dim3 b1(256), g1((unsigned)((n_in + 255) / 256));
     const int order[3] = {2, 1, 0};
