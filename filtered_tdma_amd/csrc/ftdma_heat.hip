@@ -459,7 +459,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     const int q = t >> lgS;
     const int n_sys = g.nx * (dir == 2 ? g.ny : nz);
     const int N = dir == 2 ? nz : g.ny;
-    const int s_raw = blockIdx.x * S + sl;
+    // RHSZ reads th at x +- 1 and y +- 1: each XCD takes a contiguous run of
+    // system blocks (whole x lines, consecutive y), so the halo lines its
+    // workgroups read are the ones their neighbours read, in the same L2
+    const int blk = RHSZ ? ftd::xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int s_raw = blk * S + sl;
     const bool live = s_raw < n_sys;
     const int s = live ? s_raw : n_sys - 1;
     const int ii = s % g.nx + 1, oo = s / g.nx + 1;
