@@ -2,7 +2,7 @@
 its sources into oracle/_ref/heat_cpu by oracle/Makefile.ref) on the 64^3
 heat workload: dt = eps_constant = 1e-3 (rho = 0.512 / 2.024 for 64 cells),
 4 z-slabs (MPI ranks), 100 timed steps after 10 warm-up steps, for each
-backend. heat_cpu's option = strong records per-step phase times (rhs, the
+backend; then the 256^3 step (pascal, 8 z-slabs, 10 timed steps). heat_cpu's option = strong records per-step phase times (rhs, the
 z, y and x solves, the ghost exchange) to the CSV named by TIMING_CSV
 (apps/heat_cpu/solve_theta.cpp:356-377); this prints their medians and the
 mean step time as JSON lines. Test infrastructure: the CPU baseline of the
@@ -59,6 +59,8 @@ def main():
         sys.exit("reference heat build missing: make -C oracle -f Makefile.ref")
     for kind in ("pascal", "filtered", "filtered_v2"):
         print(json.dumps(run(kind)), flush=True)
+    # the 256^3 step (same dt rule), 8 z-slabs, 10 timed steps
+    print(json.dumps(run("pascal", n=256, parts=(1, 1, 8), nt=12, warmup=2)), flush=True)
 
 
 if __name__ == "__main__":
