@@ -294,7 +294,6 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     if (mmin < 3) return fail(FTD_ERR_UNSUPPORTED, "every partition needs at least 3 rows");
     // the guard: systems whose dropped terms exceed eps_/4 are solved exactly
     const int g = h->guard;
-    if (g && cyclic) return fail(FTD_ERR_UNSUPPORTED, "the guard takes non-periodic solves");
     const char* gs = g ? " guard" : "";
 
     // fused shape: smallest compiled M holding the largest partition
@@ -328,7 +327,8 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
     const size_t nj = (size_t)P * n_sys;
     const double ev = g ? h->eps_value : 0.0;
     const int ed = g ? h->eps_direct : 0;
-    if (Ms && ftd::stream_eligible(n_sys, A, B, C, D) && (!learned || Ms == 64)) {
+    // learned J: the streamed kernel for 64-row partitions, not under the guard
+    if (Ms && ftd::stream_eligible(n_sys, A, B, C, D) && (!learned || (Ms == 64 && !g))) {
         const int R = ftd::stream_ring_depth(Ms, P);
         if (learned || g) {
             FTD_TRY(ensure(h, h->jsys, sizeof(int) * nj));
@@ -337,8 +337,8 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         int* const Js = (learned || g) ? (int*)h->jsys.p : nullptr;
         if (learned) {
             // learned J: one streamed pass (the 64-row variants hold the
-            // network's registers without spilling; shorter partitions take
-            // the fused kernel below)
+            // network's registers without spilling; shorter partitions and
+            // guarded solves take the fused kernel below)
             std::snprintf(h->path, sizeof h->path, "filtered_learned_stream M=%d Q=%d R=%d%s", Ms,
                           P, R, gs);
             FTD_LAUNCH(ftd::launch_filtered_stream(Ms, P, 1, 3, A, B, C, D, n_sys, n_row, cyclic, Jp,
@@ -450,8 +450,8 @@ ftd_status solve_filtered(ftd_handle* h, double* A, double* B, double* C, double
         FTD_LAUNCH(ftd::launch_compute_J((double*)h->partial.p, nt, P, n_row, h->rho, h->eps_value,
                                          h->eps_direct, csh, Jp, st));
     if (per_sys)
-        FTD_LAUNCH(ftd::launch_gen_filtered_check(learned, g, A, C, D, n_sys, n_row, P, Jp,
-                                                  (double*)h->xbuf.p, Js, h->eps_value,
+        FTD_LAUNCH(ftd::launch_gen_filtered_check(learned, g, A, C, D, n_sys, n_row, P, cyclic,
+                                                  Jp, (double*)h->xbuf.p, Js, h->eps_value,
                                                   h->eps_direct, (double*)h->R.p, st));
     FTD_LAUNCH(ftd::launch_gen_filtered_final(A, C, D, n_sys, n_row, P, Jp, (double*)h->xbuf.p, Js,
                                               st));

@@ -135,9 +135,9 @@ hipError_t launch_gen_filtered_sweep(int version, double* A, const double* B, do
 // gbuf of 4P x n_sys doubles); records J | fallback << 16 | guard << 17 to
 // Jsys, x0 and xN of guarded systems to xbuf
 hipError_t launch_gen_filtered_check(int learned, int guard, const double* A, const double* C,
-                                     const double* D, int n_sys, int n_row, int P, const int* Jp,
-                                     double* xbuf, int* Jsys, double eps_value, int eps_direct,
-                                     double* gbuf, hipStream_t st);
+                                     const double* D, int n_sys, int n_row, int P, int cyclic,
+                                     const int* Jp, double* xbuf, int* Jsys, double eps_value,
+                                     int eps_direct, double* gbuf, hipStream_t st);
 hipError_t launch_gen_filtered_iface(const double* A, const double* C, const double* D, int n_sys,
                                      int n_row, int P, int cyclic, double* xbuf, double* partial,
                                      int* n_tiles, hipStream_t st);

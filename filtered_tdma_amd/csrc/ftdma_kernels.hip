@@ -1238,13 +1238,63 @@ __device__ inline void guard_reduced_thomas(double* gA, double* gC0, double* gD0
     }
 }
 
-// the guard's test of one (system, partition): the dropped couplings and the
-// first truncated terms for J, against eps_/4
+// the same for periodic systems, whose reduced system is periodic too (row 0
+// of partition 0 couples to row m-1 of partition P-1): Thomas over rows
+// 1..2P-1 with a second right-hand side e (-a at row 1, -c at row 2P-1),
+// row 0 from both, then x_k = y_k + x_0 e_k -- the Sherman-Morrison order of
+// the periodic reduced solve, B == 1. Row k's e takes its a slot once the a
+// has been read (row 0's a stays).
+__device__ inline void guard_reduced_cyclic(double* gA, double* gC0, double* gD0, double* gAN,
+                                            double* gCN, double* gDN, int P, int S, int sl) {
+    const int n = 2 * P;
+    double cp = gCN[sl], dp = gDN[sl], e = -gAN[sl];  // row 1: row m-1 of partition 0
+    gAN[sl] = e;
+    for (int k = 2; k < n; ++k) {
+        const int o = (k >> 1) * S + sl;
+        double* const pa = (k & 1) ? gAN : gA;
+        double* const pc = (k & 1) ? gCN : gC0;
+        double* const pd = (k & 1) ? gDN : gD0;
+        const double ak = pa[o], ck = pc[o];
+        const double e0 = k == n - 1 ? -ck : 0.0;
+        const double r = 1.0 / (1.0 - ak * cp);
+        dp = r * (pd[o] - ak * dp);
+        e = r * (e0 - ak * e);
+        cp = r * ck;
+        pa[o] = e;
+        pc[o] = cp;
+        pd[o] = dp;
+    }
+    const double yl = dp, el = e;  // row n-1
+    for (int k = n - 2; k >= 1; --k) {
+        const int o = (k >> 1) * S + sl;
+        double* const pa = (k & 1) ? gAN : gA;
+        double* const pd = (k & 1) ? gDN : gD0;
+        const double ck = ((k & 1) ? gCN : gC0)[o];
+        dp = pd[o] - ck * dp;
+        e = pa[o] - ck * e;
+        pd[o] = dp;
+        pa[o] = e;
+    }
+    const double x0 = (gD0[sl] - gA[sl] * yl - gC0[sl] * dp) / (1.0 + gA[sl] * el + gC0[sl] * e);
+    gD0[sl] = x0;
+    for (int k = 1; k < n; ++k) {
+        const int o = (k >> 1) * S + sl;
+        double* const pd = (k & 1) ? gDN : gD0;
+        pd[o] = pd[o] + x0 * ((k & 1) ? gAN : gA)[o];
+    }
+}
+
+// the guard's test of one (system, partition): the dropped couplings against
+// eps_/4 min(1, |det|) of the interface they reach (c0' xN the left one,
+// a'_{m-1} x0 the right one: the interface 2x2 solve divides them by its
+// determinant 1 - cN_left a0'), the first truncated terms for J against eps_/4
 template <int M>
 __device__ __forceinline__ bool guard_flag(const double (&a)[M], const double (&c)[M], int m,
                                            int J, double c0p, double aN, double x0, double xN,
-                                           double eps) {
+                                           double eps, double detL, double detR) {
     const double thr = eps * 0.25;
+    const double thrL = thr * fmin(1.0, fabs(detL));
+    const double thrR = thr * fmin(1.0, fabs(detR));
     const int kL = (J < m - 2 ? J : m - 2) + 1;
     const int kR = ((m - 1) - J > 1 ? (m - 1) - J : 1) - 1;
     double akL = 0.0, ckR = 0.0;
@@ -1255,7 +1305,7 @@ __device__ __forceinline__ bool guard_flag(const double (&a)[M], const double (&
     }
     const double ta = kL <= m - 2 ? fabs(akL * x0) : 0.0;
     const double tc = kR >= 1 ? fabs(ckR * xN) : 0.0;
-    return fabs(c0p * xN) > thr || fabs(aN * x0) > thr || ta > thr || tc > thr;
+    return fabs(c0p * xN) > thrL || fabs(aN * x0) > thrR || ta > thr || tc > thr;
 }
 
 // The learned J of one (system, partition), checked (PASS 3): the network's
@@ -1403,34 +1453,46 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     const double a0p = rr * a[0];
 
     double x0, xN;
-    if (PASS == 2) {
-        x0 = xbuf[(size_t)(2 * p) * ld + i];
-        xN = xbuf[(size_t)(2 * p + 1) * ld + i];
-    } else {
-        // neighbour exchange through LDS: [C|D] of row m-1 to the right,
-        // [A|D] of row 0 to the left (filtered_tdma_cuda.cu:93-101)
-        double* sC = lds;
-        double* sDr = sC + P * S;
-        double* sA = sDr + P * S;
-        double* sDl = sA + P * S;
+    double detL = 1.0, detR = 1.0;  // the interfaces' 2x2 determinants (the guard's)
+    // neighbour exchange through LDS: [C|D] of row m-1 to the right, [A|D]
+    // of row 0 to the left (filtered_tdma_cuda.cu:93-101)
+    double* sC = lds;
+    double* sDr = sC + P * S;
+    double* sA = sDr + P * S;
+    double* sDl = sA + P * S;
+    const bool has_left = cyclic || p > 0;
+    const bool has_right = cyclic || p < P - 1;
+    const int pl = (p + P - 1) % P, pr = (p + 1) % P;
+    if (PASS != 2 || GRD) {
         sC[p * S + sl] = cN;
         sDr[p * S + sl] = dN;
         sA[p * S + sl] = a0p;
         sDl[p * S + sl] = d0p;
         __syncthreads();
-        const bool has_left = cyclic || p > 0;
-        const bool has_right = cyclic || p < P - 1;
-        const int pl = (p + P - 1) % P, pr = (p + 1) % P;
-        x0 = d0p;
-        xN = dN;
-        if (has_left) {  // k_solve_both (filtered_tdma_kernels.cuh:284-303)
-            const double C_left = sC[pl * S + sl], D_left = sDr[pl * S + sl];
-            x0 = (d0p - a0p * D_left) / (1.0 - C_left * a0p);
+    }
+    // both interface solves on every lane, then selected (no lane-divergent
+    // branch while the rows are held in registers; as k_filtered_stream)
+    if (PASS == 2) {
+        x0 = xbuf[(size_t)(2 * p) * ld + i];
+        xN = xbuf[(size_t)(2 * p + 1) * ld + i];
+        if (GRD) {
+            const double dl = 1.0 - sC[pl * S + sl] * a0p;
+            const double dr = 1.0 - sA[pr * S + sl] * cN;
+            detL = has_left ? dl : 1.0;
+            detR = has_right ? dr : 1.0;
         }
-        if (has_right) {
-            const double A_right = sA[pr * S + sl], D0_right = sDl[pr * S + sl];
-            xN = (dN - cN * D0_right) / (1.0 - A_right * cN);
-        }
+    } else {
+        // k_solve_both (filtered_tdma_kernels.cuh:284-303)
+        const double C_left = sC[pl * S + sl], D_left = sDr[pl * S + sl];
+        const double dl = 1.0 - C_left * a0p;
+        const double xl = (d0p - a0p * D_left) / dl;
+        const double A_right = sA[pr * S + sl], D0_right = sDl[pr * S + sl];
+        const double dr = 1.0 - A_right * cN;
+        const double xr = (dN - cN * D0_right) / dr;
+        x0 = has_left ? xl : d0p;
+        xN = has_right ? xr : dN;
+        detL = has_left ? dl : 1.0;
+        detR = has_right ? dr : 1.0;
     }
 
     if (PASS == 1) {
@@ -1468,7 +1530,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         const double Nl = (double)(m * P);
         const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
         const double c0p = -rr * c[0] * c[1];
-        const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps);
+        const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps, detL, detR);
         const int PS = P * S, o = p * S + sl;
         double* gCN = lds;  // the exchange arrays hold cN, dN, a0', d0'
         double* gDN = gCN + PS;
@@ -1477,19 +1539,18 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
         double* gC0 = gD0 + PS;
         double* gAN = gC0 + PS;
         int* gF = (int*)(gAN + PS);
-        if (PASS == 2) {
-            gCN[o] = cN;
-            gDN[o] = dN;
-            gA[o] = a0p;
-            gD0[o] = d0p;
-        }
         gC0[o] = c0p;
         gAN[o] = aN;
         gF[o] = flag ? 1 : 0;
         __syncthreads();
         int sys = 0;
         for (int q = 0; q < P; ++q) sys |= gF[q * S + sl];
-        if (sys && p == 0) guard_reduced_thomas(gA, gC0, gD0, gAN, gCN, gDN, P, S, sl);
+        if (sys && p == 0) {
+            if (cyclic)
+                guard_reduced_cyclic(gA, gC0, gD0, gAN, gCN, gDN, P, S, sl);
+            else
+                guard_reduced_thomas(gA, gC0, gD0, gAN, gCN, gDN, P, S, sl);
+        }
         __syncthreads();
         if (sys) {
             x0 = gD0[o];
@@ -1542,6 +1603,10 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr int S = 32, NW = Q / 2;
     static_assert(Q % 2 == 0 && M % R == 0 && R >= 2, "stream shape");
     static_assert(!GRD || VERSION == 1, "the guard runs the v1 sweeps");
+    // learned J under the guard spills the rows held in registers, and the
+    // compiler has placed some of those copies under lane-divergent branch
+    // masks (stale rows in the lanes that skipped the branch): not built
+    static_assert(!(GRD && PASS == 3), "learned J under the guard: k_filtered_fused");
     __shared__ __attribute__((aligned(16))) double ring[NW][R][2][128];
     // C, D of row m-1; A, D of row 0 (packed); under the guard also c0' and
     // a'_{m-1}, and the guard flags
@@ -1650,6 +1715,10 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
         const double d0p = rr * (d[0] - c[0] * d[1]);
         const double a0p = rr * a[0];
 
+        const bool has_left = cyclic || p > 0;
+        const bool has_right = cyclic || p < Q - 1;
+        const int pl = (p + Q - 1) % Q, pr = (p + 1) % Q;
+        double detL = 1.0, detR = 1.0;  // the interfaces' 2x2 determinants (the guard's)
         if (PASS == 2) {
             // the 2M row DMAs of this tile were issued after the xbuf DMA
             wait_vmcnt<(2 * M < 63 ? 2 * M : 63)>();
@@ -1660,6 +1729,11 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 ex[1][p][sl] = dN;
                 ex[2][p][sl] = a0p;
                 ex[3][p][sl] = d0p;
+                lds_barrier();
+                const double dl = 1.0 - ex[0][pl][sl] * a0p;
+                const double dr = 1.0 - ex[2][pr][sl] * cN;
+                detL = has_left ? dl : 1.0;
+                detR = has_right ? dr : 1.0;
             }
         } else {
             lds_barrier();  // the previous tile's readers of ex[] are done
@@ -1668,13 +1742,18 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
             ex[2][p][sl] = a0p;
             ex[3][p][sl] = d0p;
             lds_barrier();
-            const bool has_left = cyclic || p > 0;
-            const bool has_right = cyclic || p < Q - 1;
-            const int pl = (p + Q - 1) % Q, pr = (p + 1) % Q;
-            x0 = d0p;
-            xN = dN;
-            if (has_left) x0 = (d0p - a0p * ex[1][pl][sl]) / (1.0 - ex[0][pl][sl] * a0p);
-            if (has_right) xN = (dN - cN * ex[3][pr][sl]) / (1.0 - ex[2][pr][sl] * cN);
+            // both interface solves on every lane, then selected: under a
+            // lane-divergent branch here the compiler has placed copies of the
+            // register-resident rows before the exec restore of the join,
+            // leaving stale values in the lanes that skipped the branch
+            const double dl = 1.0 - ex[0][pl][sl] * a0p;
+            const double xl = (d0p - a0p * ex[1][pl][sl]) / dl;
+            const double dr = 1.0 - ex[2][pr][sl] * cN;
+            const double xr = (dN - cN * ex[3][pr][sl]) / dr;
+            x0 = has_left ? xl : d0p;
+            xN = has_right ? xr : dN;
+            detL = has_left ? dl : 1.0;
+            detR = has_right ? dr : 1.0;
         }
         // the next tile's first R rows have landed before this tile's stores
         wait_vmcnt<0>();
@@ -1706,7 +1785,7 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const double Nl = (double)(m * Q);
                 const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
                 const double c0p = -rr * c[0] * c[1];
-                const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps);
+                const bool flag = guard_flag<M>(a, c, m, Jf, c0p, aN, x0, xN, eps, detL, detR);
                 ex[4][p][sl] = c0p;
                 ex[5][p][sl] = aN;
                 gflag[p][sl] = flag ? 1 : 0;
@@ -1714,9 +1793,14 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 int sys = 0;
 #pragma unroll
                 for (int q = 0; q < Q; ++q) sys |= gflag[q][sl];
-                if (sys && p == 0)
-                    guard_reduced_thomas(&ex[2][0][0], &ex[4][0][0], &ex[3][0][0], &ex[5][0][0],
-                                         &ex[0][0][0], &ex[1][0][0], Q, S, sl);
+                if (sys && p == 0) {
+                    if (cyclic)
+                        guard_reduced_cyclic(&ex[2][0][0], &ex[4][0][0], &ex[3][0][0],
+                                             &ex[5][0][0], &ex[0][0][0], &ex[1][0][0], Q, S, sl);
+                    else
+                        guard_reduced_thomas(&ex[2][0][0], &ex[4][0][0], &ex[3][0][0],
+                                             &ex[5][0][0], &ex[0][0][0], &ex[1][0][0], Q, S, sl);
+                }
                 lds_barrier();
                 if (sys) {
                     x0 = ex[3][p][sl];
@@ -2112,12 +2196,13 @@ __global__ __launch_bounds__(256) void k_gen_filtered_iface(const double* __rest
 //     after the backward sweep, a0', d0' of row 0, the forward aN, cN, dN of
 //     row m-1, c0' = -r c0 c1 recomputed); a flagged system is solved
 //     exactly, guard_reduced_thomas over its 2P reduced rows (c', d' kept in
-//     gbuf [4P x n_sys]), its x0, xN replaced in xbuf and J = m - 1.
+//     gbuf [4P x n_sys]; periodic systems: guard_reduced_cyclic, e in xbuf),
+//     its x0, xN replaced in xbuf and J = m - 1.
 // Jsys[p][i] = J | fallback << 16 | guard << 17 for the final pass.
 template <bool LEARNED, bool GRD>
 __global__ __launch_bounds__(256) void k_gen_filtered_check(
     const double* __restrict__ A, const double* __restrict__ C, const double* __restrict__ D,
-    int n_sys, int n_row, int P, const int* __restrict__ Jp, double* __restrict__ xbuf,
+    int n_sys, int n_row, int P, int cyclic, const int* __restrict__ Jp, double* __restrict__ xbuf,
     int* __restrict__ Jsys, double eps_value, int eps_direct, double* __restrict__ gbuf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_sys) return;
@@ -2148,7 +2233,22 @@ __global__ __launch_bounds__(256) void k_gen_filtered_check(
             }
         }
         if (GRD) {
+            // guard_flag's bounds; the interface determinants as
+            // k_gen_filtered_iface forms them
+            double detL = 1.0, detR = 1.0;
+            if (cyclic || p > 0) {
+                int sl, ml;
+                part_of(n_row, P, (p + P - 1) % P, sl, ml);
+                detL = 1.0 - at(C, sl + ml - 1) * at(A, s);
+            }
+            if (cyclic || p < P - 1) {
+                int sr, mr;
+                part_of(n_row, P, (p + 1) % P, sr, mr);
+                detR = 1.0 - at(A, sr) * at(C, s + m - 1);
+            }
             const double thr = eps * 0.25;
+            const double thrL = thr * fmin(1.0, fabs(detL));
+            const double thrR = thr * fmin(1.0, fabs(detR));
             const int kL = (Jf < m - 2 ? Jf : m - 2) + 1;
             const int kR = ((m - 1) - Jf > 1 ? (m - 1) - Jf : 1) - 1;
             const double ta = kL <= m - 2 ? fabs(at(A, s + kL) * x0) : 0.0;
@@ -2157,7 +2257,7 @@ __global__ __launch_bounds__(256) void k_gen_filtered_check(
             const double rr = 1.0 / (1.0 - at(A, s + 1) * c0);
             const double c0p = -rr * c0 * c1;
             const double aN = at(A, s + m - 1);
-            sys = sys || fabs(c0p * xN) > thr || fabs(aN * x0) > thr || ta > thr || tc > thr;
+            sys = sys || fabs(c0p * xN) > thrL || fabs(aN * x0) > thrR || ta > thr || tc > thr;
         }
         Jsys[(size_t)p * ld + i] = Jf | fb;
     }
@@ -2165,6 +2265,63 @@ __global__ __launch_bounds__(256) void k_gen_filtered_check(
     // the exact reduced solve: rows 2q (a0', 1, c0' | d0') and 2q + 1
     // (aN, 1, cN | dN), in guard_reduced_thomas's order
     auto g = [&](int q, int k) -> double& { return gbuf[(size_t)(4 * q + k) * ld + i]; };
+    if (cyclic) {
+        // periodic, in guard_reduced_cyclic's order: c', y of row k in gbuf,
+        // its e in xbuf row k (whose x0 / xN the solve replaces)
+        auto xb = [&](int k) -> double& { return xbuf[(size_t)k * ld + i]; };
+        auto rrow = [&](int k, double& a, double& c, double& d) {
+            int s, m;
+            part_of(n_row, P, k >> 1, s, m);
+            if (k & 1) {
+                a = at(A, s + m - 1);
+                c = at(C, s + m - 1);
+                d = at(D, s + m - 1);
+            } else {
+                const double c0 = at(C, s), c1 = at(C, s + 1);
+                const double rr = 1.0 / (1.0 - at(A, s + 1) * c0);
+                a = at(A, s);
+                c = -rr * c0 * c1;
+                d = at(D, s);
+            }
+        };
+        const int n = 2 * P;
+        double a, c, d;
+        rrow(1, a, c, d);
+        double cp = c, dp = d, e = -a;
+        g(0, 2) = cp;
+        g(0, 3) = dp;
+        xb(1) = e;
+        for (int k = 2; k < n; ++k) {
+            rrow(k, a, c, d);
+            const double e0 = k == n - 1 ? -c : 0.0;
+            const double r = 1.0 / (1.0 - a * cp);
+            dp = r * (d - a * dp);
+            e = r * (e0 - a * e);
+            cp = r * c;
+            g(k >> 1, 2 * (k & 1)) = cp;
+            g(k >> 1, 2 * (k & 1) + 1) = dp;
+            xb(k) = e;
+        }
+        const double yl = dp, el = e;
+        for (int k = n - 2; k >= 1; --k) {
+            const double ck = g(k >> 1, 2 * (k & 1));
+            dp = g(k >> 1, 2 * (k & 1) + 1) - ck * dp;
+            e = xb(k) - ck * e;
+            g(k >> 1, 2 * (k & 1) + 1) = dp;
+            xb(k) = e;
+        }
+        rrow(0, a, c, d);
+        const double x0 = (d - a * yl - c * dp) / (1.0 + a * el + c * e);
+        xb(0) = x0;
+        for (int k = 1; k < n; ++k) xb(k) = g(k >> 1, 2 * (k & 1) + 1) + x0 * xb(k);
+        for (int q = 0; q < P; ++q) {
+            int s, m;
+            part_of(n_row, P, q, s, m);
+            const int fb = Jsys[(size_t)q * ld + i] & (1 << 16);
+            Jsys[(size_t)q * ld + i] = (m - 1) | fb | (1 << 17);
+        }
+        return;
+    }
     double cp = 0.0, dp = 0.0;
     for (int q = 0; q < P; ++q) {
         int s, m;
@@ -2433,7 +2590,7 @@ hipError_t launch_filtered_fused(int M, int maxt, int version, int pass, const d
     const int lgS = log2i(S);
     const dim3 grid((n_sys + S - 1) / S), block(S * P);
     // the guard runs after the first pass of v1 only
-    if (guard && (cyclic || pass == 1)) return hipErrorInvalidValue;
+    if (guard && pass == 1) return hipErrorInvalidValue;
     const size_t lds = guard ? filtered_fused_lds(P, S, 1)
                              : (pass == 2) ? 0 : filtered_fused_lds(P, S, 0);
 #define LAUNCH(m_, t_, v_, p_, pf_, w_, g_)                                                   \
@@ -2611,7 +2768,7 @@ hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const dou
                                   int guard, hipStream_t st) {
     if (!stream_eligible(n_sys, A, B, C, D)) return hipErrorInvalidValue;
     if ((pass == 3 || guard) && !Jsys) return hipErrorInvalidValue;
-    if (guard && (cyclic || pass == 1)) return hipErrorInvalidValue;
+    if (guard && pass == 1) return hipErrorInvalidValue;
     const int n_tiles = n_sys / 32;
 #define LAUNCH(m_, q_, r_, w_, v_, p_, g_)                                                    \
     do {                                                                                     \
@@ -2622,16 +2779,16 @@ hipError_t launch_filtered_stream(int M, int Q, int version, int pass, const dou
                            eps_value, eps_direct, Jsys);                                     \
     } while (0)
     // the guard's variants: one wave per SIMD (its LDS flags and exact reduced
-    // solve make them larger), and none for 16 partitions (their ring fills the LDS)
+    // solve make them larger), none for 16 partitions (their ring fills the
+    // LDS) and none for learned J (the network, the guard and 64 rows in
+    // registers outgrow the register file; the fused kernel takes that case)
 #define X(m_, q_, r_, w_)                         \
     if (M == m_ && Q == q_) {                     \
         if (guard) {                              \
-            if (q_ > 8)                           \
+            if (q_ > 8 || pass == 3)              \
                 return hipErrorInvalidValue;      \
             else if (version == 2)                \
                 LAUNCH(m_, q_, r_, 1, 1, 0, (q_ <= 8));  \
-            else if (pass == 3)                   \
-                LAUNCH(m_, q_, r_, 1, 1, 3, (q_ <= 8));  \
             else                                  \
                 LAUNCH(m_, q_, r_, 1, 1, 2, (q_ <= 8));  \
         } else if (version == 2)                  \
@@ -2693,13 +2850,13 @@ hipError_t launch_gen_filtered_sweep(int version, double* A, const double* B, do
 }
 
 hipError_t launch_gen_filtered_check(int learned, int guard, const double* A, const double* C,
-                                     const double* D, int n_sys, int n_row, int P, const int* Jp,
-                                     double* xbuf, int* Jsys, double eps_value, int eps_direct,
-                                     double* gbuf, hipStream_t st) {
+                                     const double* D, int n_sys, int n_row, int P, int cyclic,
+                                     const int* Jp, double* xbuf, int* Jsys, double eps_value,
+                                     int eps_direct, double* gbuf, hipStream_t st) {
     const dim3 g((n_sys + 255) / 256);
 #define LAUNCH(l_, g_)                                                                          \
     hipLaunchKernelGGL((k_gen_filtered_check<l_, g_>), g, dim3(256), 0, st, A, C, D, n_sys, n_row, \
-                       P, Jp, xbuf, Jsys, eps_value, eps_direct, gbuf)
+                       P, cyclic, Jp, xbuf, Jsys, eps_value, eps_direct, gbuf)
     if (learned && guard)
         LAUNCH(true, true);
     else if (learned)

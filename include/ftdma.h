@@ -201,12 +201,17 @@ ftd_status ftd_set_j_mode(ftd_handle* h, int mode);
  * the v1 (full) sweeps, evaluates per (partition, system) those dropped
  * terms and the first term the truncation by J drops on each side, and
  * solves a system exactly (its 2P-row reduced system by Thomas, then the
- * full correction) where any of them exceeds eps_/4, so that x stays within
+ * full correction) where a dropped coupling exceeds eps_/4 times
+ * min(1, |1 - cN_left a0'|) (the determinant of the interface 2x2 solve that
+ * passes it on) or a truncated term exceeds eps_/4, so that x stays within
  * about eps_ of the exact solve. Off by default (the reference's bits).
- * Non-periodic real solves, any partition length (partitions longer than 64
- * rows run on the general kernels, with global scratch of 4 P n_sys
- * doubles); periodic and complex ones return FTD_ERR_UNSUPPORTED.
- * ftd_last_J_systems reports which systems it solved exactly. */
+ * Periodic systems (ftd_solve_cyclic) wrap the interfaces and solve the
+ * periodic reduced system (Sherman-Morrison order, as the periodic PaScaL
+ * reduced solve). Real solves of any partition length (partitions longer
+ * than 64 rows run on the general kernels, with global scratch of 4 P n_sys
+ * doubles); complex ones on partitions of at most 64 rows, others return
+ * FTD_ERR_UNSUPPORTED. ftd_last_J_systems reports which systems it solved
+ * exactly. */
 ftd_status ftd_set_guard(ftd_handle* h, int enable);
 
 /* Extension: the J used per (partition, system) by the last learned-mode or

@@ -1113,16 +1113,28 @@ int ftdo_filtered_learned(double* A, const double* B, double* C, double* D,
  * m-1's to its own row 0 (solve_interfaces, :197-218). The guard evaluates,
  * per (partition, system) and with the interface values x0, xN, the terms
  * those drops leave out -- |c0' xN|, |a'_{m-1} x0| -- and the first term the
- * truncation leaves out on each side -- |a'_{kL} x0|, |c'_{kR} xN|. A system
- * where any of them exceeds eps_/4 (in any partition) is solved exactly: its
+ * truncation leaves out on each side -- |a'_{kL} x0|, |c'_{kR} xN|. The
+ * interface 2x2 solve passes a dropped coupling u, v on as (u - cN v)/det,
+ * (v - a0' u)/det, det = 1 - cN_left a0'; the interior rows are convex
+ * combinations of x0, xN (|a'_j| + |c'_j| <= 1 for diagonally dominant rows)
+ * and drop at most the first truncated term each. So the couplings are held
+ * to eps_/4 min(1, |det|) (the partition's left interface for c0' xN, its
+ * right one for a'_{m-1} x0), the truncated terms to eps_/4, which keeps
+ * x within about eps_ of the exact solve. A system
+ * where any of them exceeds its bound (in any partition) is solved exactly: its
  * 2P-row reduced system, c0' kept, by Thomas (B == 1, row 0 "divided by 1"
  * left as it is), then the full correction. Every kind runs the v1 (full)
  * sweeps under the guard.
  * J[p * n_sys + i]: the J of (partition p, system i) before the guard (v1,
  * v2: the partition's J in every column; learned: the J after its own
- * check). rec_out[p * n_sys + i] = the J used | guard << 17. Non-periodic. */
+ * check). rec_out[p * n_sys + i] = the J used | guard << 17.
+ * Periodic systems (cyclic): the interfaces wrap (partition 0's left
+ * neighbour is partition P-1), and the exact reduced system is periodic too
+ * -- row 0 of partition 0 couples to row m-1 of partition P-1 -- solved as
+ * cyclic_cols solves one (Thomas over rows 1..2P-1 with a second right-hand
+ * side e, then row 0, then x_k = y_k + x_0 e_k), B == 1. */
 int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
-                        int n_sys, int n_row, int P, const int* J,
+                        int n_sys, int n_row, int P, int cyclic, const int* J,
                         double eps_value, int eps_direct, int* rec_out, int nthreads) {
     int rc = check_parts(n_row, P);
     if (rc) return rc;
@@ -1168,29 +1180,35 @@ int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
             const double a0p = Rp[0 * ld + i], c0p = Rp[1 * ld + i], d0p = Rp[2 * ld + i];
             const double aN = Rp[3 * ld + i], cN = Rp[4 * ld + i], dN = Rp[5 * ld + i];
             /* interface 2x2 solves (filtered_tdma_kernels.cuh:284-303) */
-            double v0 = d0p, vN = dN;
-            if (p > 0) {
-                const double* Rl = R + (size_t)(6 * (p - 1)) * ld;
-                v0 = (d0p - a0p * Rl[5 * ld + i]) / (1.0 - Rl[4 * ld + i] * a0p);
+            double v0 = d0p, vN = dN, detL = 1.0, detR = 1.0;
+            if (cyclic || p > 0) {
+                const double* Rl = R + (size_t)(6 * ((p + P - 1) % P)) * ld;
+                detL = 1.0 - Rl[4 * ld + i] * a0p;
+                v0 = (d0p - a0p * Rl[5 * ld + i]) / detL;
             }
-            if (p < P - 1) {
-                const double* Rr = R + (size_t)(6 * (p + 1)) * ld;
-                vN = (dN - cN * Rr[2 * ld + i]) / (1.0 - Rr[0 * ld + i] * cN);
+            if (cyclic || p < P - 1) {
+                const double* Rr = R + (size_t)(6 * ((p + 1) % P)) * ld;
+                detR = 1.0 - Rr[0 * ld + i] * cN;
+                vN = (dN - cN * Rr[2 * ld + i]) / detR;
             }
             x0[p] = v0;
             xN[p] = vN;
             const double Nl = (double)(m * P);
             const double eps = eps_direct ? eps_value : eps_value / (Nl * Nl);
             const double thr = eps * 0.25;
+            /* a dropped coupling reaches the interface values divided by the
+             * interface's 2x2 determinant (1 - cN_left a0', < 1 for M-matrices) */
+            const double thrL = thr * fmin(1.0, fabs(detL));
+            const double thrR = thr * fmin(1.0, fabs(detR));
             const int Jp = J[(size_t)p * ld + i];
             const int kL = (Jp < m - 2 ? Jp : m - 2) + 1;
             const int kR = ((m - 1) - Jp > 1 ? (m - 1) - Jp : 1) - 1;
             const double ta = kL <= m - 2 ? fabs(ROW(A, s + kL)[i] * v0) : 0.0;
             const double tc = kR >= 1 ? fabs(ROW(C, s + kR)[i] * vN) : 0.0;
             const double t0 = fabs(c0p * vN), tN = fabs(aN * v0);
-            if (t0 > thr || tN > thr || ta > thr || tc > thr) guard = 1;
+            if (t0 > thrL || tN > thrR || ta > thr || tc > thr) guard = 1;
         }
-        if (guard) {
+        if (guard && !cyclic) {
             /* the exact reduced solve: rows [row 0 of p, row m-1 of p] */
             double cp[1024], dp[1024];
             cp[0] = R[1 * ld + i];
@@ -1205,6 +1223,37 @@ int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
                 cp[k] = inv * ck;
             }
             for (int k = 2 * P - 2; k >= 0; --k) dp[k] = dp[k] - cp[k] * dp[k + 1];
+            for (int p = 0; p < P; ++p) {
+                x0[p] = dp[2 * p];
+                xN[p] = dp[2 * p + 1];
+            }
+        } else if (guard) {
+            /* the periodic reduced system: y and e over rows 1..n-1 (e's
+             * right-hand side -a_1 at row 1, -c_{n-1} at row n-1), then
+             * x_0 from row 0 and x_k = y_k + x_0 e_k (cyclic_cols, B == 1) */
+            const int n = 2 * P;
+            double cp[1024], dp[1024], e[1024];
+            const double a0 = R[0 * ld + i], c0 = R[1 * ld + i], d0 = R[2 * ld + i];
+            cp[1] = R[4 * ld + i];
+            dp[1] = R[5 * ld + i];
+            e[1] = -R[3 * ld + i];
+            for (int k = 2; k < n; ++k) {
+                const double* Rp = R + (size_t)(6 * (k >> 1)) * ld;
+                const double ak = Rp[((k & 1) ? 3 : 0) * ld + i];
+                const double ck = Rp[((k & 1) ? 4 : 1) * ld + i];
+                const double dk = Rp[((k & 1) ? 5 : 2) * ld + i];
+                const double e0 = k == n - 1 ? -ck : 0.0;
+                const double r = 1.0 / (1.0 - ak * cp[k - 1]);
+                dp[k] = r * (dk - ak * dp[k - 1]);
+                e[k] = r * (e0 - ak * e[k - 1]);
+                cp[k] = r * ck;
+            }
+            for (int k = n - 2; k >= 1; --k) {
+                dp[k] = dp[k] - cp[k] * dp[k + 1];
+                e[k] = e[k] - cp[k] * e[k + 1];
+            }
+            dp[0] = (d0 - a0 * dp[n - 1] - c0 * dp[1]) / (1.0 + a0 * e[n - 1] + c0 * e[1]);
+            for (int k = 1; k < n; ++k) dp[k] = dp[k] + dp[0] * e[k];
             for (int p = 0; p < P; ++p) {
                 x0[p] = dp[2 * p];
                 xN[p] = dp[2 * p + 1];

@@ -106,10 +106,10 @@ float ftdo_lg(double x);
  * solve leaves out (|c0' xN|, |a'_{m-1} x0| and the first truncated term of
  * each side for J[p * n_sys + i]) against eps_/4; a system where any
  * exceeds it is solved exactly (2P-row reduced Thomas, full correction).
- * rec_out[p * n_sys + i] = J used | guard << 17. Non-periodic, 2..512
- * partitions. */
+ * rec_out[p * n_sys + i] = J used | guard << 17. 2..512 partitions; with
+ * cyclic the interfaces wrap and the exact reduced system is periodic. */
 int ftdo_filtered_guard(double* A, const double* B, double* C, double* D,
-                        int n_sys, int n_row, int P, const int* J,
+                        int n_sys, int n_row, int P, int cyclic, const int* J,
                         double eps_value, int eps_direct, int* rec_out, int nthreads);
 
 /* Heat ADI harness (apps/heat_cpu on a parts[0] x parts[1] x parts[2] process

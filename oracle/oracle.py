@@ -49,8 +49,8 @@ def lib() -> ctypes.CDLL:
         L.ftdo_pascal.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int, c_int]
         L.ftdo_pascal_nested.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int,
                                          c_int]
-        L.ftdo_filtered_guard.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, ip, c_double,
-                                          c_int, ip, c_int]
+        L.ftdo_filtered_guard.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, ip,
+                                          c_double, c_int, ip, c_int]
         L.ftdo_set_rho.argtypes = [dp, dp, dp, c_int, c_int, dp, dp]
         L.ftdo_set_rho.restype = None
         L.ftdo_cal_J.argtypes = [c_double, c_double, c_double, c_int, c_int]
@@ -131,14 +131,16 @@ def pascal_ir(A, B, C, D, Q, sweeps, nthreads=0):
     return X
 
 
-def filtered_guard(A, B, C, D, P, J, eps, eps_direct=False, nthreads=0, inplace=False):
+def filtered_guard(A, B, C, D, P, J, eps, eps_direct=False, cyclic=False, nthreads=0,
+                   inplace=False):
     """The guard mode: J [P, n_sys] int (the J of each (partition, system)
-    before the guard); returns (x, records [P, n_sys] = J used | guard << 17)."""
+    before the guard); returns (x, records [P, n_sys] = J used | guard << 17).
+    cyclic: periodic systems (wrapped interfaces, periodic reduced system)."""
     (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
     Jm = np.ascontiguousarray(np.broadcast_to(np.asarray(J, dtype=np.int32), (P, n_sys)))
     rec = np.zeros((P, n_sys), dtype=np.int32)
     ip = ctypes.POINTER(ctypes.c_int)
-    _check(lib().ftdo_filtered_guard(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P,
+    _check(lib().ftdo_filtered_guard(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, int(bool(cyclic)),
                                      Jm.ctypes.data_as(ip), float(eps), int(bool(eps_direct)),
                                      rec.ctypes.data_as(ip), nthreads))
     return D, rec

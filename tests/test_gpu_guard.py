@@ -1,16 +1,16 @@
 """The guard of the filtered kinds (ftd_set_guard) against the oracle's
 restatement (ftdo_filtered_guard): x and the per (partition, system)
-records bit for bit, for v1, v2 and learned J on the streamed and the fused
-kernels, real and complex right-hand sides; and the property the guard is
-for: every system within eps_ of the exact (PaScaL) solve, where the
-reference's filtered solve is not."""
+records bit for bit, for v1, v2 and learned J on the streamed, the fused and
+the general kernels, real and complex right-hand sides, periodic systems; and
+the property the guard is for: every system within eps_ of the exact (PaScaL)
+solve, where the reference's filtered solve is not."""
 import sys
 
 import numpy as np
 import pytest
 
 import oracle as O
-from filtered_tdma_amd import FtdError, Kind, TdmaBackendGPU, cases
+from filtered_tdma_amd import Kind, TdmaBackendGPU, cases
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +26,7 @@ def dev(x):
     return torch.from_numpy(np.ascontiguousarray(x)).cuda()
 
 
-def run_guard(kind, A, B, C, D, P, eps_c, learned=False, complex_rhs=False):
+def run_guard(kind, A, B, C, D, P, eps_c, learned=False, complex_rhs=False, cyclic=False):
     n_row, n_sys = A.shape
     be = TdmaBackendGPU(kind, n_sys, n_row, P, eps_c)
     try:
@@ -37,6 +37,8 @@ def run_guard(kind, A, B, C, D, P, eps_c, learned=False, complex_rhs=False):
         be.set_rho_device(tA, tB, tC)
         if complex_rhs:
             be.solve_complex(tA, tB, tC, tD)
+        elif cyclic:
+            be.solve_cyclic(tA, tB, tC, tD)
         else:
             be.solve(tA, tB, tC, tD)
         torch.cuda.synchronize()
@@ -45,20 +47,20 @@ def run_guard(kind, A, B, C, D, P, eps_c, learned=False, complex_rhs=False):
         be.close()
 
 
-def oracle_guard(kind, A, B, C, D, P, eps_c, learned=False):
+def oracle_guard(kind, A, B, C, D, P, eps_c, learned=False, cyclic=False):
     """x and the records the guarded solve should give: the kind's J per
     (partition, system) by the oracle, then ftdo_filtered_guard."""
     n_sys = D.shape[1]
     fb = np.zeros((P, n_sys), dtype=np.int32)
     if learned:
-        _, Jr = O.filtered_learned(A, B, C, D, P, eps_c)
+        _, Jr = O.filtered_learned(A, B, C, D, P, eps_c, cyclic=cyclic)
         J, fb = Jr & 0xFFFF, (Jr >> 16) & 1
     else:
         version = 2 if kind == Kind.FILTERED_V2 else 1
         _, Jp = O.filtered(A, B, C, D, P, version, eps_c, sem=O.SEM_GPU, rho="auto",
-                           return_J=True)
+                           cyclic=cyclic, return_J=True)
         J = np.repeat(np.asarray(Jp, dtype=np.int32)[:, None], n_sys, axis=1)
-    x, rec = O.filtered_guard(A, B, C, D, P, J, eps_c)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, eps_c, cyclic=cyclic)
     return x, rec | (fb << 16)
 
 
@@ -66,7 +68,7 @@ CASES = [
     # (kind, learned, n_sys, n_row, P, r range, eps_constant)
     (Kind.FILTERED, False, 1024, 512, 8, (0.5, 20.0), 1e-3),      # streamed, two passes
     (Kind.FILTERED_V2, False, 1024, 512, 8, (0.5, 20.0), 1e-3),   # streamed, one pass
-    (Kind.FILTERED, True, 1024, 512, 8, (0.5, 20.0), 1e-3),       # streamed learned
+    (Kind.FILTERED, True, 1024, 512, 8, (0.5, 20.0), 1e-3),       # learned, m = 64: fused
     (Kind.FILTERED, False, 300, 256, 8, (0.05, 3.0), 1e-2),       # fused (n_sys % 32 != 0)
     (Kind.FILTERED_V2, False, 300, 200, 4, (5.0, 500.0), 1e-4),   # fused, rho to 0.499
     (Kind.FILTERED, True, 257, 128, 8, (0.05, 3.0), 1e-6),        # fused learned, m = 16
@@ -84,6 +86,7 @@ def test_guard_bitwise_and_within_eps(kind, learned, n_sys, n_row, P, rr, eps_c)
     A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, rr[0], rr[1], seed=31 + P)
     x, rec, path = run_guard(kind, A, B, C, D, P, eps_c, learned)
     assert path.endswith(" guard"), path
+    assert not (learned and "stream" in path), path
     xr, recr = oracle_guard(kind, A, B, C, D, P, eps_c, learned)
     np.testing.assert_array_equal(x, xr, err_msg=path)
     np.testing.assert_array_equal(rec, recr, err_msg=path)
@@ -108,15 +111,35 @@ def test_guard_complex_parts_guarded_separately():
         np.testing.assert_array_equal(rec[:, par::2], recr, err_msg=path)
 
 
-def test_guard_refuses_periodic_solves():
-    A, B, C, D, _ = cases.toeplitz_batch(64, 96, 0.5, 20.0, seed=1)
-    for P, n_row in ((4, 96), (2, 300)):  # fused and general kernels
-        A, B, C, D, _ = cases.toeplitz_batch(64, n_row, 0.5, 20.0, seed=1)
-        be = TdmaBackendGPU(Kind.FILTERED, 64, n_row, P, 1e-3)
-        try:
-            be.set_guard(True)
-            tA, tB, tC, tD = dev(A), dev(B), dev(C), dev(D)
-            with pytest.raises(FtdError):
-                be.solve_cyclic(tA, tB, tC, tD)
-        finally:
-            be.close()
+PERIODIC = [
+    # (kind, learned, n_sys, n_row, P, r range, eps_constant)
+    (Kind.FILTERED, False, 1024, 512, 8, (0.5, 20.0), 1e-3),      # streamed, two passes
+    (Kind.FILTERED_V2, False, 1024, 384, 8, (0.5, 20.0), 1e-3),   # streamed, m = 48
+    (Kind.FILTERED, True, 1024, 512, 8, (0.5, 20.0), 1e-3),       # learned, m = 64: fused
+    (Kind.FILTERED, False, 300, 256, 8, (0.05, 3.0), 1e-2),       # fused
+    (Kind.FILTERED_V2, False, 200, 96, 2, (5.0, 500.0), 1e-4),    # fused, two partitions
+    (Kind.FILTERED, True, 257, 131, 4, (0.05, 3.0), 1e-6),        # fused learned, uneven
+    (Kind.FILTERED, False, 130, 600, 2, (0.5, 20.0), 1e-3),       # general, m = 300
+    (Kind.FILTERED_V2, False, 257, 1000, 4, (5.0, 500.0), 1e-4),  # general, m = 250
+    (Kind.FILTERED, True, 200, 520, 3, (0.05, 3.0), 1e-6),        # general learned, uneven
+]
+
+
+@pytest.mark.parametrize("kind,learned,n_sys,n_row,P,rr,eps_c", PERIODIC)
+def test_guard_periodic_bitwise_and_within_eps(kind, learned, n_sys, n_row, P, rr, eps_c):
+    """Periodic systems: wrapped interfaces, the periodic reduced system
+    solved exactly for a guarded system."""
+    note(f"guard periodic {kind.name} learned={learned} n_sys={n_sys} n_row={n_row} P={P}")
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, rr[0], rr[1], seed=41 + P)
+    x, rec, path = run_guard(kind, A, B, C, D, P, eps_c, learned, cyclic=True)
+    assert path.endswith(" guard"), path
+    assert not (learned and "stream" in path), path
+    xr, recr = oracle_guard(kind, A, B, C, D, P, eps_c, learned, cyclic=True)
+    np.testing.assert_array_equal(x, xr, err_msg=path)
+    np.testing.assert_array_equal(rec, recr, err_msg=path)
+    exact = O.pascal(A, B, C, D, P, cyclic=True)
+    m_max = -(-n_row // P)
+    eps = eps_c / float(m_max * P) ** 2
+    assert np.max(np.abs(x - exact)) <= eps, (np.max(np.abs(x - exact)), eps)
+    if rr[1] > 10:
+        assert (rec >> 17).any(), "no system guarded: the reduced solve went untested"

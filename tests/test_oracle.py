@@ -317,8 +317,9 @@ def test_filtered_error_bounded_by_decoupling(version):
 
 # ------------------------------------------------------------- guard mode
 
-def _v1_J(A, B, C, D, P, eps, eps_direct):
-    _, J = O.filtered(A, B, C, D, P, 1, eps, eps_direct=eps_direct, rho="auto", return_J=True)
+def _v1_J(A, B, C, D, P, eps, eps_direct, cyclic=False):
+    _, J = O.filtered(A, B, C, D, P, 1, eps, eps_direct=eps_direct, rho="auto", cyclic=cyclic,
+                      return_J=True)
     return np.repeat(J[:, None], D.shape[1], axis=1)
 
 
@@ -368,6 +369,64 @@ def test_guard_keeps_every_system_within_eps_of_exact(r_lo, r_hi, P, eps):
     if r_hi > 10:
         assert np.max(np.abs(ref - exact)) > eps  # what the guard fixes
         assert guarded.mean() > 0.1
+
+
+def test_guard_periodic_never_triggered_is_periodic_v1():
+    """Periodic systems, unreachable threshold: the periodic v1 solve (wrapped
+    interfaces) with the same J, bit for bit."""
+    A, B, C, D, _ = cases.toeplitz_batch(30, 256, 0.5, 20.0, seed=24)
+    J = _v1_J(A, B, C, D, 4, 1e-6, False, cyclic=True)
+    x, rec = O.filtered_guard(A, B, C, D, 4, J, 1e300, eps_direct=True, cyclic=True)
+    np.testing.assert_array_equal(x, O.filtered(A, B, C, D, 4, 1, 1e-6, rho="auto", cyclic=True,
+                                                J_force=J[:, 0]))
+    assert not np.any(rec >> 17)
+
+
+@pytest.mark.parametrize("P", [2, 3, 8])
+def test_guard_periodic_always_triggered_is_an_exact_periodic_solve(P):
+    """eps_ = 0 on periodic systems: the periodic reduced system (row 0 of
+    partition 0 coupled to row m-1 of partition P-1) solved exactly, then the
+    full correction -- a dense solve of the periodic matrix to 1e-13. A
+    reduced solve that dropped either corner coupling fails this."""
+    A, B, C, D = cases.general_batch(12, 24 * P + 1, seed=25 + P)
+    J = np.zeros((P, 12), dtype=np.int32)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, 0.0, eps_direct=True, cyclic=True)
+    assert np.all(rec >> 17 == 1)
+    ref = np.stack([np.linalg.solve(tridiag_dense(A, B, C, i, cyclic=True), D[:, i])
+                    for i in range(D.shape[1])], axis=1)
+    assert relerr(x, ref) <= 1e-13
+    # the non-periodic guard on the same input solves another system
+    xn, _ = O.filtered_guard(A, B, C, D, P, J, 0.0, eps_direct=True)
+    assert relerr(xn, ref) > 1e-6
+
+
+def test_guard_periodic_keeps_every_system_within_eps_of_exact():
+    n_sys, n_row, P, eps = 200, 512, 8, 1e-10
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 0.5, 20.0, seed=26)
+    J = _v1_J(A, B, C, D, P, eps, True, cyclic=True)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, eps, eps_direct=True, cyclic=True)
+    exact = O.pascal(A, B, C, D, P, cyclic=True)
+    assert np.max(np.abs(x - exact)) <= eps
+    ref = O.filtered(A, B, C, D, P, 1, eps, eps_direct=True, rho="auto", cyclic=True)
+    assert np.max(np.abs(ref - exact)) > eps
+    assert (rec >> 17).max(axis=0).mean() > 0.1
+
+
+@pytest.mark.parametrize("cyclic", [False, True])
+@pytest.mark.parametrize("eps", [1e-8, 1e-10, 1e-12])
+def test_guard_bound_holds_through_small_interface_determinants(cyclic, eps):
+    """Long weakly dominant partitions (250 rows, rho to 0.499): the
+    interface 2x2 solve divides a dropped coupling by 1 - cN_left a0', far
+    below 1 here, so the couplings are held to eps_/4 times that determinant.
+    With eps_/4 alone these systems end up to 3 eps_ off the exact solve."""
+    n_sys, n_row, P = 128, 1000, 4
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 5.0, 500.0, seed=7)
+    J = _v1_J(A, B, C, D, P, eps, True, cyclic=cyclic)
+    x, rec = O.filtered_guard(A, B, C, D, P, J, eps, eps_direct=True, cyclic=cyclic)
+    exact = O.pascal(A, B, C, D, P, cyclic=cyclic)
+    assert np.max(np.abs(x - exact)) <= eps
+    guarded = (rec >> 17).max(axis=0) == 1
+    assert 0 < guarded.mean() < 1  # some systems keep the filtered solve
 
 
 def test_filtered_gpu_vs_cpu_semantics_same_when_J_full():
