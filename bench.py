@@ -645,12 +645,13 @@ def main():
     del x_pascal
 
     # periodic systems (solve_cyclic) on the same batch: the wrap-around
-    # interface of the last partition, every kind
+    # interface of the last partition, every kind, and the guard's periodic
+    # reduced solve
     periodic = None
     if not args.no_periodic:
         periodic = {}
         x_ref = None
-        for kind in ("pascal", "filtered_v2", "filtered"):
+        for kind in ("pascal", "filtered_v2", "filtered", "filtered_v2_guard", "filtered_guard"):
             wall, ev_ms, path, J = time_kind(kind, tA, tB, tC, D0, copies, args.steps,
                                              args.warmup, torch, cyclic=True)
             ev_avg = statistics.mean(ev_ms)
