@@ -2360,9 +2360,24 @@ __global__ __launch_bounds__(256) void k_gen_filtered_check(
     }
 }
 
+__device__ inline int wave_max_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ inline int wave_min_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // truncated final correction (k_final_pass, filtered_tdma_kernels.cuh:196-218);
 // J per partition from Jp, or per (partition, system) from Jsys (low 16
-// bits) where the general path's check kernel ran
+// bits) where the general path's check kernel ran.
+// One J per partition: the left rows 1..jL, then the right rows jR..m-2.
+// A J per system: one walk over the rows, each read and written once, the
+// two corrections predicated per lane ((d - a x0) - c xN where both apply,
+// as the two walks give), and the rows no lane of the wave corrects skipped
+// at once. A lane that falls back to J = m-1 then costs its wave one walk
+// over the partition instead of two.
 __global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __restrict__ A,
                                                             const double* __restrict__ C,
                                                             double* __restrict__ D, int n_sys,
@@ -2370,9 +2385,11 @@ __global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __rest
                                                             const int* __restrict__ Jp,
                                                             const double* __restrict__ xbuf,
                                                             const int* __restrict__ Jsys) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i_raw = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
-    if (i >= n_sys) return;
+    const bool live = i_raw < n_sys;
+    if (!Jsys && !live) return;
+    const int i = live ? i_raw : n_sys - 1;
     int s, m;
     part_of(n_row, P, p, s, m);
     const size_t ld = (size_t)n_sys;
@@ -2381,18 +2398,41 @@ __global__ __launch_bounds__(256) void k_gen_filtered_final(const double* __rest
     double* Dp = D + (size_t)s * ld + i;
     const double* Ap = A + (size_t)s * ld + i;
     const double* Cp = C + (size_t)s * ld + i;
-    Dp[0] = x0;
-    Dp[(size_t)(m - 1) * ld] = xN;
     const int J = Jsys ? (Jsys[(size_t)p * ld + i] & 0xFFFF) : Jp[p];
-    const int jL = J < m - 2 ? J : m - 2;
-    const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
-    for (int j = 1; j <= jL; ++j) {
-        const size_t o = (size_t)j * ld;
-        Dp[o] = Dp[o] - Ap[o] * x0;
+    int jL = J < m - 2 ? J : m - 2;
+    int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+    if (!Jsys) {
+        Dp[0] = x0;
+        Dp[(size_t)(m - 1) * ld] = xN;
+        for (int j = 1; j <= jL; ++j) {
+            const size_t o = (size_t)j * ld;
+            Dp[o] = Dp[o] - Ap[o] * x0;
+        }
+        for (int j = jR; j < m - 1; ++j) {
+            const size_t o = (size_t)j * ld;
+            Dp[o] = Dp[o] - Cp[o] * xN;
+        }
+        return;
     }
-    for (int j = jR; j < m - 1; ++j) {
-        const size_t o = (size_t)j * ld;
-        Dp[o] = Dp[o] - Cp[o] * xN;
+    if (live) {
+        Dp[0] = x0;
+        Dp[(size_t)(m - 1) * ld] = xN;
+    } else {
+        jL = 0;  // no row
+        jR = m - 1;
+    }
+    const int wjL = wave_max_i(jL), wjR = wave_min_i(jR);
+    for (int j = 1; j <= m - 2; ++j) {
+        if (j > wjL && j < wjR) j = wjR;  // wave-uniform: no lane corrects rows wjL+1..wjR-1
+        if (j > m - 2) break;
+        const bool l = j <= jL, r = j >= jR;
+        if (l || r) {
+            const size_t o = (size_t)j * ld;
+            double x = Dp[o];
+            if (l) x = x - Ap[o] * x0;
+            if (r) x = x - Cp[o] * xN;
+            Dp[o] = x;
+        }
     }
 }
 
