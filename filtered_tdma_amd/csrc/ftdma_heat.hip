@@ -256,17 +256,34 @@ struct RowAt {
 // d of row j of one z partition formed from th as k_rhs_z forms it (the
 // z sweep then needs no rhs pass): c0 is the cell of the partition's first
 // row, k0 its z index; rows past the partition repeat its last row
+// Rows are formed in order (j = 0, 1, ...), so th at z - 1, z, z + 1 rolls
+// through zm, z0, zp: one z load per row instead of three. Rows past the
+// partition (j >= cnt) repeat its last row, as the clamped index does; the
+// roll is by selects, not a branch (the lanes of a wave may hold partitions
+// of different lengths).
 struct RowRhsZ {
     const double* th;
     const double* cz;
     size_t c0, sy, sz;
     int cnt, k0;
     double xa, xc, ya, yc, za, zc, diag, cxy;
-    __device__ __forceinline__ double operator()(int j) const {
+    double zm = 0.0, z0 = 0.0, zp = 0.0;
+    __device__ __forceinline__ double operator()(int j) {
         const int jj = j < cnt ? j : cnt - 1;
         const size_t c = c0 + (size_t)jj * sz;
+        if (j == 0) {
+            zm = th[c - sz];
+            z0 = th[c];
+            zp = th[c + sz];
+        } else {
+            const bool adv = j < cnt;
+            const double nzp = th[c + sz];
+            zm = adv ? z0 : zm;
+            z0 = adv ? zp : z0;
+            zp = adv ? nzp : zp;
+        }
         return (xc * th[c + 1] + xa * th[c - 1]) + (yc * th[c + sy] + ya * th[c - sy]) +
-               (zc * th[c + sz] + za * th[c - sz]) + diag * th[c] + cxy * cz[k0 + jj];
+               (zc * zp + za * zm) + diag * z0 + cxy * cz[k0 + jj];
     }
 };
 
