@@ -28,7 +28,7 @@
  * Scratch memory: ftd_create allocates all the device scratch that
  * ftd_solve, ftd_solve_cyclic, ftd_solve_complex, ftd_solve_f32 and
  * ftd_solve_f32c of the handle's dimensions can need, for any array
- * alignment; ftd_set_tuning, ftd_set_j_mode and ftd_autotune size it again
+ * alignment; ftd_set_tuning, ftd_set_j_mode, ftd_set_guard and ftd_autotune size it again
  * for their new choice (and may wait for the device to replace a buffer).
  * Those solves therefore allocate nothing, and a handle's first solve may be
  * captured into a hipGraph. Only ftd_solve_nr with dimensions other than the

@@ -36,7 +36,7 @@ public:
 
     TdmaBackendGPU(Kind kind, int n_sys, int n_row, int partitions, double eps_constant,
                    void* stream = nullptr)
-        : kind_(kind), partitions_(partitions) {
+        : kind_(kind), partitions_(partitions), n_sys_(n_sys) {
         check(ftd_create(&h_, static_cast<int>(kind), n_sys, n_row, partitions, eps_constant,
                          stream));
         const ftd_status s = ftd_set_profiling(h_, 1);
@@ -78,6 +78,28 @@ public:
         return J;
     }
 
+    // ---- extensions (no counterpart in the reference; ftdma.h) ----
+
+    /// ftd_set_guard: the filtered kinds solve exactly the systems whose
+    /// dropped terms could put x more than eps_ from the exact solve.
+    void set_guard(bool enable) { check(ftd_set_guard(h_, enable ? 1 : 0)); }
+    /// ftd_set_j_mode: 0 = the reference's cal_J, 1 = learned J per
+    /// (partition, system), checked against eps_ (FILTERED kind).
+    void set_j_mode(int mode) { check(ftd_set_j_mode(h_, mode)); }
+    /// ftd_solve_complex: complex128 right-hand sides (interleaved re, im),
+    /// one coefficient set for both parts.
+    void solve_complex(double* d_A, double* d_B, double* d_C, double* d_Z) {
+        check(ftd_solve_complex(h_, d_A, d_B, d_C, d_Z));
+    }
+    /// ftd_last_J_systems: J | fallback << 16 | guard << 17 per (partition,
+    /// system) of the last learned or guarded solve, index p * n + i (n =
+    /// n_sys, doubled for a complex solve). Blocking.
+    std::vector<int> last_J_systems(bool complex_rhs = false) const {
+        std::vector<int> out((size_t)partitions_ * n_sys_ * (complex_rhs ? 2 : 1));
+        check(ftd_last_J_systems(h_, out.data(), (int64_t)out.size()));
+        return out;
+    }
+
     ftd_handle* handle() { return h_; }
 
 private:
@@ -86,5 +108,6 @@ private:
     }
     Kind kind_;
     int partitions_;
+    int n_sys_;
     ftd_handle* h_ = nullptr;
 };

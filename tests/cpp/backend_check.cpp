@@ -7,13 +7,15 @@
 //   backend_check <cases.bin> <results.bin>
 //
 // cases.bin: int32 n_cases, then per case
-//   int32 kind, n_sys, n_row, partitions, cyclic, pad; float64 eps_constant;
+//   int32 kind, n_sys, n_row, partitions, cyclic, options (bit 0: the guard,
+//   bit 1: learned J); float64 eps_constant;
 //   float64 A, B, C, D [n_row * n_sys] each (row-major, systems fastest).
 // results.bin: per case
 //   int32 J[partitions]; float64 gpu_ms (profiled solve), gpu_ms_off (after
 //   set_profiling(false)), comm_ms; int32 parse_ok, graph_equal (3 replays
 //   of the captured first solve of a fresh handle, each bit-identical to the
-//   class's solve); float64 x [n_row * n_sys] (the class's solve).
+//   class's solve); float64 x [n_row * n_sys] (the class's solve); with
+//   options, int32 records [partitions * n_sys] (last_J_systems).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -80,14 +82,15 @@ int main(int argc, char** argv) {
             if (std::fread(hdr, 4, 6, in) != 6 || std::fread(&eps_constant, 8, 1, in) != 1)
                 throw std::runtime_error("short case header");
             const int kind = hdr[0], n_sys = hdr[1], n_row = hdr[2], P = hdr[3], cyclic = hdr[4];
+            const int guard = hdr[5] & 1, learned = (hdr[5] >> 1) & 1;
             const size_t n = (size_t)n_sys * n_row;
             std::vector<double> h[4];
             for (auto& v : h) {
                 v.resize(n);
                 if (std::fread(v.data(), 8, n, in) != n) throw std::runtime_error("short arrays");
             }
-            std::printf("[backend_check] case %d: kind %d n_sys %d n_row %d P %d cyclic %d\n", c,
-                        kind, n_sys, n_row, P, cyclic);
+            std::printf("[backend_check] case %d: kind %d n_sys %d n_row %d P %d cyclic %d "
+                        "guard %d learned %d\n", c, kind, n_sys, n_row, P, cyclic, guard, learned);
             std::fflush(stdout);
             Dev A(n), B(n), C(n), D(n);
             auto load = [&]() {
@@ -100,12 +103,14 @@ int main(int argc, char** argv) {
             // the reference's calling protocol on the C++ class
             // (apps/heat_gpu/solve_theta.cu:522-552): profiled by default
             std::vector<double> x(n), x_off(n);
-            std::vector<int> J;
+            std::vector<int> J, rec;
             double gpu_ms = 0.0, gpu_ms_off = -1.0, comm_ms = -1.0;
             {
                 TdmaBackendGPU be(static_cast<TdmaBackendGPU::Kind>(kind), n_sys, n_row, P,
                                   eps_constant);
                 load();
+                if (guard) be.set_guard(true);
+                if (learned) be.set_j_mode(1);
                 be.set_eps_constant(eps_constant);
                 be.set_rho_device(A.p, B.p, C.p);
                 if (cyclic)
@@ -115,6 +120,7 @@ int main(int argc, char** argv) {
                 gpu_ms = be.last_gpu_ms();
                 comm_ms = be.last_comm_ms();
                 J = be.last_J();
+                if (guard || learned) rec = be.last_J_systems();
                 D.get(x);
                 be.set_profiling(false);
                 load();
@@ -138,6 +144,8 @@ int main(int argc, char** argv) {
                 hip_check(hipStreamCreate(&st), "hipStreamCreate");
                 ftd_handle* hd = nullptr;
                 ftd_check(ftd_create(&hd, kind, n_sys, n_row, P, eps_constant, st), "ftd_create");
+                if (guard) ftd_check(ftd_set_guard(hd, 1), "ftd_set_guard");
+                if (learned) ftd_check(ftd_set_j_mode(hd, 1), "ftd_set_j_mode");
                 load();
                 ftd_check(ftd_set_rho(hd, A.p, B.p, C.p), "ftd_set_rho");
                 hip_check(hipStreamSynchronize(st), "sync");
@@ -172,6 +180,10 @@ int main(int argc, char** argv) {
             std::fwrite(ms, 8, 3, out);
             std::fwrite(flags, 4, 2, out);
             std::fwrite(x.data(), 8, n, out);
+            if (guard || learned) {
+                if (rec.size() != (size_t)P * n_sys) throw std::runtime_error("records size");
+                std::fwrite(rec.data(), 4, rec.size(), out);
+            }
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "backend_check: %s\n", e.what());
