@@ -77,6 +77,10 @@ CASES = [
     (Kind.FILTERED_V2, False, 256, 1024, 2, (0.5, 20.0), 1e-3),   # general, m = 512
     (Kind.FILTERED, True, 200, 400, 4, (0.05, 3.0), 1e-6),        # general learned, m = 100
     (Kind.FILTERED_V2, False, 130, 387, 3, (5.0, 500.0), 1e-4),   # general, uneven 129/129/129
+    # partitions of 64 and 63 rows: the waves that hold one of each diverge
+    (Kind.FILTERED, False, 1024, 509, 8, (0.5, 20.0), 1e-3),      # streamed, two passes
+    (Kind.FILTERED_V2, False, 1024, 509, 8, (0.5, 20.0), 1e-3),   # streamed, one pass
+    (Kind.FILTERED, True, 300, 511, 8, (0.5, 20.0), 1e-3),        # learned, fused
 ]
 
 
@@ -122,6 +126,8 @@ PERIODIC = [
     (Kind.FILTERED, False, 130, 600, 2, (0.5, 20.0), 1e-3),       # general, m = 300
     (Kind.FILTERED_V2, False, 257, 1000, 4, (5.0, 500.0), 1e-4),  # general, m = 250
     (Kind.FILTERED, True, 200, 520, 3, (0.05, 3.0), 1e-6),        # general learned, uneven
+    (Kind.FILTERED, True, 1024, 509, 8, (0.5, 20.0), 1e-3),       # learned, fused, 64/63 rows
+    (Kind.FILTERED_V2, False, 1024, 381, 8, (0.5, 20.0), 1e-3),   # streamed, 48/47 rows
 ]
 
 

@@ -59,6 +59,7 @@ def run_learned(A, B, C, D, P, eps_c, cyclic=False, complex_rhs=False):
     (1024, 512, 8, False),   # the streamed pass (m = 64, n_sys % 32 == 0)
     (2048, 200, 4, False),   # streamed, partitions of 50 rows
     (512, 384, 8, True),     # streamed, periodic
+    (1024, 509, 8, False),   # streamed, partitions of 64 and 63 rows in one wave
     (200, 300, 2, False),    # partitions of 150 rows: the general kernels
     (130, 1000, 4, True),    # general, periodic, m = 250
     (257, 520, 3, False),    # general, uneven partitions 174/173/173
