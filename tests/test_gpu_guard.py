@@ -100,6 +100,22 @@ def test_guard_bitwise_and_within_eps(kind, learned, n_sys, n_row, P, rr, eps_c)
     assert np.max(np.abs(x - exact)) <= eps, (np.max(np.abs(x - exact)), eps)
 
 
+def test_learned_guard_meets_1e_12_relative_at_eps_1e_12():
+    """The workload's accuracy bar for learned J, 1e-12 relative to FP64
+    PaScaL: met with the guard at eps_ = 1e-12 (eps_constant = 1e-12 (m P)^2)."""
+    n_sys, n_row, P = 2048, 512, 8
+    eps_c = 1e-12 * float(n_row) ** 2
+    A, B, C, D, _ = cases.toeplitz_batch(n_sys, n_row, 0.5, 20.0, seed=0)
+    note(f"learned + guard at eps_ 1e-12, n_sys={n_sys} n_row={n_row} P={P}")
+    x, rec, path = run_guard(Kind.FILTERED, A, B, C, D, P, eps_c, learned=True)
+    xr, recr = oracle_guard(Kind.FILTERED, A, B, C, D, P, eps_c, learned=True)
+    np.testing.assert_array_equal(x, xr, err_msg=path)
+    np.testing.assert_array_equal(rec, recr, err_msg=path)
+    exact = O.pascal(A, B, C, D, P)
+    rel = np.max(np.abs(x - exact), axis=0) / np.max(np.abs(exact), axis=0)
+    assert rel.max() <= 1e-12, rel.max()
+
+
 def test_guard_complex_parts_guarded_separately():
     """Complex right-hand sides: each part is its own system for the guard
     (the reference solves the parts as two real systems)."""
