@@ -473,13 +473,19 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
             }
             Dr[0] = (Dr[0] - Ar[0] * Dr[(NR - 1) * S] - Cr[0] * Dr[1 * S]) /
                     (1.0 + Ar[0] * Er[(NR - 1) * S] + Cr[0] * Er[1 * S]);
-            for (int k = 1; k < NR; ++k) Dr[k * S] = Dr[k * S] + Dr[0] * Er[k * S];
+            // x_k = y_k + x_0 e_k: each partition forms its own two rows below
         }
     }
     __syncthreads();
 
-    const double x0 = rD[(2 * q) * S + sl];
-    const double xN = rD[(2 * q + 1) * S + sl];
+    double x0 = rD[(2 * q) * S + sl];
+    double xN = rD[(2 * q + 1) * S + sl];
+    if constexpr (CYCLIC) {
+        const double x0r = rD[sl];
+        const double y0 = x0 + x0r * rE[(2 * q) * S + sl];
+        xN = xN + x0r * rE[(2 * q + 1) * S + sl];
+        x0 = q == 0 ? x0 : y0;  // row 0 of the reduced system is x_0 itself
+    }
     if (live) {
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -911,16 +917,22 @@ __global__ __launch_bounds__(Q * 32) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const double x0r = (Dr[0] - Ar[0] * dp - Cr[0] * Dr[1 * S]) /
                                    (1.0 + Ar[0] * ep + Cr[0] * Er[1 * S]);
                 Dr[0] = x0r;
-#pragma unroll 2
-                for (int k = 1; k < NR; ++k) Dr[k * S] = Dr[k * S] + x0r * Er[k * S];
+                Er[0] = 0.0;
+                // x_k = y_k + x_0 e_k: each partition forms its own two rows below
             }
         }
         lds_barrier();
         // the next tile's first R rows have landed before this tile's stores
         // (requested during the last R row-steps, so this costs no wait)
         wait_vmcnt<0>();
-        const double x0 = red[2][2 * q][sl];
-        const double xN = red[2][2 * q + 1][sl];
+        double x0 = red[2][2 * q][sl];
+        double xN = red[2][2 * q + 1][sl];
+        if constexpr (CYCLIC) {
+            const double x0r = red[2][0][sl];
+            const double y0 = x0 + x0r * red[3][2 * q][sl];
+            xN = xN + x0r * red[3][2 * q + 1][sl];
+            x0 = q == 0 ? x0 : y0;  // row 0 of the reduced system is x_0 itself
+        }
         // rows past m_p store row m_p - 1 (x_N) again
         double* const xlast = D + base + (size_t)(cnt - 1) * ld;
 #pragma unroll
