@@ -654,6 +654,7 @@ struct ftd_heat {
     double *cx = nullptr, *cy = nullptr, *cz = nullptr;
     std::vector<double> hx, hy, hz, sx_, sy_, sz_, cx_, cy_, cz_;
     ftd_handle* solver[3] = {nullptr, nullptr, nullptr};  // x, y, z sweeps
+    int rho_set[3] = {0, 0, 0};  // the solver's column-0 bounds are set (materialized sweeps)
     long steps_done = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -868,7 +869,14 @@ ftd_status enqueue_step(ftd_heat* h) {
         HH(hipGetLastError());
         ftd_handle* sv = h->solver[d];
         HT(ftd_set_eps_constant(sv, h->dt));
-        HT(ftd_set_rho(sv, h->A, h->B, h->C));
+        // the coefficients are the stencil's constants in every step, and so
+        // are their column-0 bounds (solve_theta.cu calls set_rho_device per
+        // step on the same values): set once per solver, in the first step,
+        // which always runs eagerly, so no captured step holds the launch
+        if (!h->rho_set[d]) {
+            HT(ftd_set_rho(sv, h->A, h->B, h->C));
+            h->rho_set[d] = 1;
+        }
         if (h->periodic[d])
             HT(ftd_solve_cyclic(sv, h->A, h->B, h->C, h->D));
         else
