@@ -23,7 +23,7 @@ EXPORTS = (
     "ftd_solve_static", "ftd_set_guard",
     "ftd_heat_create", "ftd_heat_destroy", "ftd_heat_init", "ftd_heat_step", "ftd_heat_field",
     "ftd_heat_l2", "ftd_heat_last_step_ms", "ftd_heat_last_error_string", "ftd_heat_set_fused",
-    "ftd_heat_set_plan",
+    "ftd_heat_set_plan", "ftd_heat_last_plan",
     "ftd_poisson_create", "ftd_poisson_destroy", "ftd_poisson_solver", "ftd_poisson_solve",
     "ftd_poisson_solve_rhs", "ftd_poisson_last_ms", "ftd_poisson_last_error_string",
     "ftd_poisson_set_profiling",
@@ -100,6 +100,8 @@ def load() -> ctypes.CDLL:
     L.ftd_heat_last_step_ms.argtypes = [vp]
     L.ftd_heat_last_step_ms.restype = c_double
     L.ftd_heat_last_error_string.restype = ctypes.c_char_p
+    L.ftd_heat_last_plan.argtypes = [vp]
+    L.ftd_heat_last_plan.restype = ctypes.c_char_p
     L.ftd_poisson_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_int, c_int, c_double, vp]
     L.ftd_poisson_destroy.argtypes = [vp]

@@ -1257,3 +1257,18 @@ const char* ftd_last_path(const ftd_handle* h) { return h ? h->path : "none"; }
 const char* ftd_last_error_string(void) { return g_err.c_str(); }
 
 }  // extern "C"
+
+namespace ftd {
+int handle_filtered_state(ftd_handle* h, int** J, const double** rho, double* eps_value,
+                          int* eps_direct) {
+    if (!h || h->kind == FTD_KIND_PASCAL || h->P < 2 || !h->J || !h->rho)
+        return fail(FTD_ERR_INVALID, "not a filtered handle over partitions");
+    if (h->j_mode != 0 || h->guard)
+        return fail(FTD_ERR_UNSUPPORTED, "learned J and the guard run on the handle's own solve");
+    *J = h->J;
+    *rho = h->rho;
+    *eps_value = h->eps_value;
+    *eps_direct = h->eps_direct;
+    return FTD_OK;
+}
+}  // namespace ftd

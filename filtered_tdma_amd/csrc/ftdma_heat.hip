@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/ftdma.h"
+#include "ftdma_kernels.h"
 #include "ftdma_xcd.h"
 
 namespace {
@@ -508,6 +509,221 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused z sweep of the filtered kinds over the caller's z partitions
+// (Dirichlet walls). The rows are generated as k_build forms them (a = -s.za,
+// b = 1 - s.zb, c = -s.zc, d = the explicit rhs of th as k_rhs_z forms it,
+// the wall fold-in on the system's first and last rows), and the solve is
+// k_filtered_fused's (ftdma_kernels.hip), statement for statement: the bits
+// of the materialized build + solve + scatter, x written into rhs.
+//   VERSION 2, PASS 0: one pass with J[q] (k_zbound_maxabs + k_compute_J).
+//   VERSION 1, PASS 1: x0, xN of every (partition, system) to xbuf, the
+//     tile's largest |x0|, |xN| per partition to partial[tile * Q + q];
+//   VERSION 1, PASS 2: the same sweeps, x0, xN from xbuf, J[q] applied.
+// One lane per (system, partition q); S systems per workgroup.
+// ---------------------------------------------------------------------------
+template <int M, int VERSION, int PASS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_filt_z(
+    const double* __restrict__ th, double* rhs, Dims g, int nz, double sa, double sb, double sc,
+    int Q, int lgS, Stencil st, const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ cz, const int* __restrict__ Jp, double* __restrict__ xbuf,
+    double* __restrict__ partial) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int n_sys = g.nx * g.ny;
+    const int blk = ftd::xcd_tile(blockIdx.x, gridDim.x);
+    const int s_raw = blk * S + sl;
+    const bool live = s_raw < n_sys;
+    const int s = live ? s_raw : n_sys - 1;  // dead lanes shadow the last system
+    const int ii = s % g.nx + 1, oo = s / g.nx + 1;
+    const size_t rs = (size_t)g.px * g.ny1;
+    const size_t here0 = cell(g, ii, oo, 1);
+    int r0, m;
+    {
+        const int base = nz / Q, rem = nz % Q;
+        r0 = q * base + (q < rem ? q : rem);
+        m = base + (q < rem ? 1 : 0);
+    }
+    const size_t lo = here0, hi = here0 + (size_t)(nz - 1) * rs;
+    double* const pbase = rhs + here0 + (size_t)r0 * rs;
+    RowRhsZ rows{th, cz, here0 + (size_t)r0 * rs, (size_t)g.px, rs, m, r0 + 1,
+                 st.xa, st.xc, st.ya, st.yc, st.za, st.zc, 1.0 + st.xb + st.yb + st.zb,
+                 st.c0 * cx[ii] * cy[oo]};
+    const int J = (VERSION == 2 || PASS == 2) ? Jp[q] : m;
+    constexpr int PF = M >= 32 ? 2 : 4;
+    double rd[M];
+#pragma unroll
+    for (int j = 0; j < PF && j < M; ++j) rd[j] = rows(j);
+
+    double a[M], c[M], d[M];
+    double aN = 0.0, cN = 0.0, dN = 0.0;
+    double b1 = 1.0;
+    // forward sweep (k_fwd_pass_v1 / k_fwd_pass_v2)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j + PF < M) rd[j + PF] = rows(j + PF);
+        if (j < m) {
+            double aj, bj, cj, dj;
+            adi_row(j == 0 && q == 0, j == m - 1 && q == Q - 1, sa, sb, sc, rd[j], th, lo, hi, rs,
+                    aj, bj, cj, dj);
+            if (j < 2) {
+                const double r = 1.0 / bj;
+                a[j] = aj * r;
+                c[j] = cj * r;
+                d[j] = dj * r;
+                if (j == 1) b1 = bj;
+            } else if (VERSION == 1 || j <= J) {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = -inv * aj * a[j - 1];
+            } else {
+                const double inv = 1.0 / (bj - aj * c[j - 1]);
+                d[j] = inv * (dj - aj * d[j - 1]);
+                c[j] = inv * cj;
+                a[j] = aj;  // left raw, as the skip-A loop leaves A[j]
+            }
+            if (VERSION == 2 && j == 1 && J == 0) {
+                // k_fwd_pass_v2 with J == 0 eliminates the normalised row 1
+                // against itself
+                const double inv = 1.0 / (b1 - a[1] * c[1]);
+                d[1] = inv * (d[1] - a[1] * d[1]);
+                c[1] = inv * c[1];
+            }
+            if (j == m - 1) {
+                aN = a[j];
+                cN = c[j];
+                dN = d[j];
+            }
+        }
+    }
+    // backward sweep (k_bwd_pass_v1 / k_bwd_pass_v2)
+    const int lo_c = (m - 2) - J;
+#pragma unroll
+    for (int j = M - 3; j >= 1; --j) {
+        if (j <= m - 3) {
+            const double cj = c[j];
+            d[j] = d[j] - cj * d[j + 1];
+            if (VERSION == 1) {
+                a[j] = a[j] - cj * a[j + 1];
+                c[j] = -cj * c[j + 1];
+            } else {
+                if (J >= 2 && j < J) {
+                    const double an = (J == m - 1 && j == m - 3) ? aN : a[j + 1];
+                    a[j] = a[j] - cj * an;
+                }
+                if (j > lo_c) c[j] = -cj * c[j + 1];
+            }
+        }
+    }
+    // pack_reduced: row 0 keeps d0, a0 (its c0 coupling dropped)
+    const double rr = 1.0 / (1.0 - a[1] * c[0]);
+    const double d0p = rr * (d[0] - c[0] * d[1]);
+    const double a0p = rr * a[0];
+
+    double x0, xN;
+    if (PASS == 2) {
+        x0 = xbuf[(size_t)(2 * q) * n_sys + s];
+        xN = xbuf[(size_t)(2 * q + 1) * n_sys + s];
+    } else {
+        // neighbour exchange through LDS and the interface 2x2 solves, both
+        // formed on every lane and selected (no lane-divergent branch while
+        // the rows are held in registers)
+        double* sC = lds;
+        double* sDr = sC + Q * S;
+        double* sA = sDr + Q * S;
+        double* sDl = sA + Q * S;
+        sC[q * S + sl] = cN;
+        sDr[q * S + sl] = dN;
+        sA[q * S + sl] = a0p;
+        sDl[q * S + sl] = d0p;
+        __syncthreads();
+        const int ql = q > 0 ? q - 1 : 0, qr = q < Q - 1 ? q + 1 : Q - 1;
+        const double xl = (d0p - a0p * sDr[ql * S + sl]) / (1.0 - sC[ql * S + sl] * a0p);
+        const double xr = (dN - cN * sDl[qr * S + sl]) / (1.0 - sA[qr * S + sl] * cN);
+        x0 = q > 0 ? xl : d0p;
+        xN = q < Q - 1 ? xr : dN;
+    }
+    if (PASS == 1) {
+        if (live) {
+            xbuf[(size_t)(2 * q) * n_sys + s] = x0;
+            xbuf[(size_t)(2 * q + 1) * n_sys + s] = xN;
+        }
+        // dead lanes shadow a live system of this workgroup: the max holds
+        double v = fmax(fabs(x0), fabs(xN));
+        for (int off = S >> 1; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+        if (sl == 0) partial[(size_t)blk * Q + q] = v;
+        return;
+    }
+    // k_final_pass: truncated correction
+    const int jL = J < m - 2 ? J : m - 2;
+    const int jR = (m - 1) - J > 1 ? (m - 1) - J : 1;
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < m) {
+                double x;
+                if (j == 0) {
+                    x = x0;
+                } else if (j == m - 1) {
+                    x = xN;
+                } else {
+                    x = d[j];
+                    if (j <= jL) x = x - a[j] * x0;
+                    if (j >= jR) x = x - c[j] * xN;
+                }
+                pbase[(size_t)j * rs] = x;
+            }
+        }
+    }
+}
+
+// |d| of rows 0 and m-1 of every z partition, d as k_build forms it (the
+// explicit rhs of th, the wall fold-in on the system's first and last rows),
+// the largest per partition over each block of 256 systems: the tile maxima
+// k_boundary_maxabs takes from the built D, for the fused filtered_v2 sweep
+__global__ __launch_bounds__(256) void k_zbound_maxabs(const double* __restrict__ th, Dims g,
+                                                       int nz, int Q, double sa, double sc,
+                                                       Stencil st, const double* __restrict__ cx,
+                                                       const double* __restrict__ cy,
+                                                       const double* __restrict__ cz,
+                                                       double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int q = blockIdx.y;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int n_sys = g.nx * g.ny;
+    double v = 0.0;
+    if (s < n_sys) {
+        const int ii = s % g.nx + 1, oo = s / g.nx + 1;
+        const size_t sy = (size_t)g.px, sz = (size_t)g.px * g.ny1;
+        const int base = nz / Q, rem = nz % Q;
+        const int r0 = q * base + (q < rem ? q : rem);
+        const int m = base + (q < rem ? 1 : 0);
+        const double diag = 1.0 + st.xb + st.yb + st.zb;
+        const double cxy = st.c0 * cx[ii] * cy[oo];
+        const size_t lo = cell(g, ii, oo, 1), hi = lo + (size_t)(nz - 1) * sz;
+        auto d_of = [&](int row) {
+            const size_t c = lo + (size_t)row * sz;
+            double d = (st.xc * th[c + 1] + st.xa * th[c - 1]) +
+                       (st.yc * th[c + sy] + st.ya * th[c - sy]) +
+                       (st.zc * th[c + sz] + st.za * th[c - sz]) + diag * th[c] +
+                       cxy * cz[row + 1];
+            if (row == 0) d += sa * (th[lo - sz] + th[lo]);
+            if (row == nz - 1) d += sc * (th[hi + sz] + th[hi]);
+            return d;
+        };
+        v = fmax(fabs(d_of(r0)), fabs(d_of(r0 + m - 1)));
+    }
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[(size_t)blockIdx.x * Q + q] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 // line stride of the x sweep's LDS tile, in doubles: even, so that every
 // line starts on a 16-byte boundary for the 16-byte DMAs
 __host__ __device__ inline int x_tile_stride(int N) { return N + 2 - (N & 1); }
@@ -666,8 +882,16 @@ struct ftd_heat {
         int on = 0, M = 0, thomas = 0, Q = 0, lgS = 0;
         size_t lds = 0;
     } fused[3];
+    // fused z sweep of the filtered kinds over the caller's z partitions
+    // (k_adi_filt_z): enabled, rows per lane, partitions, log2 systems per
+    // workgroup, dynamic LDS bytes of the exchange
+    struct FiltZ {
+        int on = 0, M = 0, Q = 0, lgS = 0;
+        size_t lds = 0;
+    } filt;
     int use_fused = 1;
     int fuse_rhs = 0;  // the fused z sweep forms the rhs (no k_rhs_z pass) this step
+    char plan[256] = "";  // how the last enqueued step ran each sweep (ftd_heat_last_plan)
 };
 
 namespace {
@@ -754,6 +978,93 @@ void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
     f.lds = lds_of(S);
     f.on = 1;
     h->fused[d] = f;
+}
+
+// the filtered kinds' fused z sweep: a filtered kind split in z into 2..16
+// Dirichlet partitions of 3..64 rows. The tile is halved (down to 16
+// systems) until the grid covers the CUs; the bits do not depend on it.
+void plan_filt_z(ftd_heat* h) {
+    ftd_heat::FiltZ f;
+    const int Q = h->parts[2], N = h->nz, n_sys = h->g.nx * h->g.ny;
+    h->filt = f;
+    if (h->kind == FTD_KIND_PASCAL || Q < 2 || Q > 16 || h->periodic[2] || n_sys < 1) return;
+    const int m_max = (N + Q - 1) / Q, m_min = N / Q;
+    if (m_max > 64 || m_min < 3) return;
+    f.M = m_max <= 16 ? 16 : m_max <= 32 ? 32 : 64;
+    int S = 64;
+    while (S * Q > 256) S >>= 1;
+    const int cus = device_cus();
+    while (S > 16 && (n_sys + S - 1) / S < cus) S >>= 1;
+    int lg = 0;
+    while ((1 << lg) < S) ++lg;
+    f.Q = Q;
+    f.lgS = lg;
+    f.lds = sizeof(double) * 4 * (size_t)Q * S;
+    f.on = 1;
+    h->filt = f;
+}
+
+// whether this step's z sweep can run fused: planned, fused sweeps on, the
+// solver's column-0 bounds set (by the first, materialized step) and the
+// solver in its plain mode (no learned J, no guard)
+bool filt_z_ready(ftd_heat* h) {
+    if (!h->use_fused || !h->filt.on || !h->rho_set[2]) return false;
+    int* J;
+    const double* rho;
+    double ev;
+    int ed;
+    return ftd::handle_filtered_state(h->solver[2], &J, &rho, &ev, &ed) == FTD_OK;
+}
+
+template <int M>
+hipError_t launch_filt_z_t(ftd_heat* h, hipStream_t st) {
+    const ftd_heat::FiltZ& f = h->filt;
+    int* J;
+    const double* rho;
+    double ev;
+    int ed;
+    if (ftd::handle_filtered_state(h->solver[2], &J, &rho, &ev, &ed) != FTD_OK)
+        return hipErrorInvalidValue;
+    const Dims g = h->g;
+    const int S = 1 << f.lgS, n_sys = g.nx * g.ny;
+    const dim3 grid((n_sys + S - 1) / S), block(S * f.Q);
+    // scratch: the materialized sweeps' A (x0, xN: 2Q x n_sys) and B (tile
+    // maxima); the z sweep runs first in the step
+    double* const xbuf = h->A;
+    double* const partial = h->B;
+    const Stencil& s = h->s;
+    hipError_t e;
+    if (h->kind == FTD_KIND_FILTERED_V2) {
+        const dim3 gb((n_sys + 255) / 256, f.Q);
+        hipLaunchKernelGGL(k_zbound_maxabs, gb, dim3(256), 0, st, h->th, g, h->nz, f.Q, s.za,
+                           s.zc, s, h->cx, h->cy, h->cz, partial);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = ftd::launch_compute_J(partial, (int)gb.x, f.Q, h->nz, rho, ev, ed, 0, J, st)) !=
+            hipSuccess)
+            return e;
+        hipLaunchKernelGGL((k_adi_filt_z<M, 2, 0>), grid, block, f.lds, st, h->th, h->rhs, g,
+                           h->nz, s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf,
+                           partial);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_adi_filt_z<M, 1, 1>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz,
+                       s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = ftd::launch_compute_J(partial, (int)grid.x, f.Q, h->nz, rho, ev, ed, 0, J, st)) !=
+        hipSuccess)
+        return e;
+    hipLaunchKernelGGL((k_adi_filt_z<M, 1, 2>), grid, block, 0, st, h->th, h->rhs, g, h->nz,
+                       s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_filt_z(ftd_heat* h, hipStream_t st) {
+    switch (h->filt.M) {
+        case 16: return launch_filt_z_t<16>(h, st);
+        case 32: return launch_filt_z_t<32>(h, st);
+        case 64: return launch_filt_z_t<64>(h, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int M, bool TH>
@@ -844,7 +1155,9 @@ ftd_status enqueue_step(ftd_heat* h) {
     const ftd_heat::Fused& fz = h->fused[2];
     const int z_groups = (g.nx * g.ny + (1 << fz.lgS) - 1) >> fz.lgS;
     h->fuse_rhs = h->use_fused && fz.on && !(fz.thomas && z_groups < device_cus());
-    if (!h->fuse_rhs) {
+    // the filtered kinds' fused z sweep forms the rhs itself as well
+    const bool z_filt = filt_z_ready(h);
+    if (!h->fuse_rhs && !z_filt) {
         const int kb = h->rhs_kb;
         hipLaunchKernelGGL(k_rhs_z,
                            dim3(((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((h->nz + kb - 1) / kb)),
@@ -857,13 +1170,33 @@ ftd_status enqueue_step(ftd_heat* h) {
     const int n_sys[3] = {g.ny * h->nz, g.nx * h->nz, g.nx * g.ny};
     const double sa[3] = {h->s.xa, h->s.ya, h->s.za}, sb[3] = {h->s.xb, h->s.yb, h->s.zb},
                  sc[3] = {h->s.xc, h->s.yc, h->s.zc};
+    size_t np = 0;
+    auto note_plan = [&](int d, const char* how, int M, int Q, int lgS) {
+        const char* sep = np ? "; " : "";
+        const char dn = "xyz"[d];
+        int w = M ? std::snprintf(h->plan + np, sizeof h->plan - np, "%s%c:%s M=%d Q=%d S=%d", sep,
+                                  dn, how, M, Q, 1 << lgS)
+                  : std::snprintf(h->plan + np, sizeof h->plan - np, "%s%c:%s", sep, dn, how);
+        if (w > 0) np = std::min(sizeof h->plan - 1, np + (size_t)w);
+    };
+    h->plan[0] = '\0';
     for (int q = 0; q < 3; ++q) {
         const int d = order[q];
         if (h->use_fused && h->fused[d].on) {
             // coefficients generated, rhs read and x written in place
             HH(launch_adi(h, d, st));
+            note_plan(d, "fused", h->fused[d].M, h->fused[d].Q, h->fused[d].lgS);
             continue;
         }
+        if (d == 2 && z_filt) {
+            // the filtered kinds over the caller's z partitions: rows
+            // generated, rhs formed from th, x written into rhs
+            HT(ftd_set_eps_constant(h->solver[2], h->dt));
+            HH(launch_filt_z(h, st));
+            note_plan(d, "filtered_fused", h->filt.M, h->filt.Q, h->filt.lgS);
+            continue;
+        }
+        note_plan(d, "materialized", 0, 0, 0);
         hipLaunchKernelGGL(k_build, g1, b1, 0, st, h->th, h->rhs, g, d, sa[d], sb[d], sc[d],
                            h->periodic[d] ? 0 : 1, h->A, h->B, h->C, h->D);
         HH(hipGetLastError());
@@ -982,6 +1315,7 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     h->rhs_kb = rhs_planes(nx, ny, nz);
     // x, y, z sweeps: n_sys = ny*nz, nx*nz, nx*ny
     const int ns[3] = {ny * nz, nx * nz, nx * ny}, nr[3] = {nx, ny, nz};
+    plan_filt_z(h);
     for (int d = 0; d < 3; ++d) {
         plan_fused(h, d, nr[d], ns[d]);
         if (h->fused[d].on && h->fused[d].lds > 64 * 1024) {
@@ -1156,6 +1490,8 @@ ftd_status ftd_heat_l2(ftd_heat* h, double* l2) {
 }
 
 double ftd_heat_last_step_ms(const ftd_heat* h) { return h ? h->last_ms : 0.0; }
+
+const char* ftd_heat_last_plan(const ftd_heat* h) { return h ? h->plan : ""; }
 
 const char* ftd_heat_last_error_string(void) { return g_heat_err.c_str(); }
 

@@ -147,3 +147,15 @@ hipError_t launch_gen_filtered_final(const double* A, const double* C, double* D
                                      const int* Jsys, hipStream_t st);
 
 }  // namespace ftd
+
+// For a caller inside the library that runs a filtered kind's sweeps itself
+// (the heat step's fused filtered z sweep): the handle's J per partition
+// (device), its column-0 bounds as set by ftd_set_rho (device, A_rho then
+// C_rho, n_row + 1 each) and its eps_ setting. An error for a handle that is
+// not a filtered kind over more than one partition, or that has learned J or
+// the guard switched on (those run on the handle's own solve).
+struct ftd_handle;
+namespace ftd {
+int handle_filtered_state(ftd_handle* h, int** J, const double** rho, double* eps_value,
+                          int* eps_direct);
+}  // namespace ftd

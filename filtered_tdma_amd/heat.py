@@ -72,6 +72,11 @@ class SolveTheta:
     def last_step_ms(self) -> float:
         return _lib.load().ftd_heat_last_step_ms(self._h)
 
+    def last_plan(self) -> str:
+        """How the last enqueued step ran each sweep (fused, filtered_fused or
+        materialized, with the fused shapes)."""
+        return _lib.load().ftd_heat_last_plan(self._h).decode()
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().ftd_heat_destroy(self._h)

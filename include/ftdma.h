@@ -335,10 +335,13 @@ ftd_status ftd_heat_init(ftd_heat* h);
 ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph);
 
 /* Extension: fused sweeps on (default) or off. Where a sweep's coefficients
- * are the stencil's (PaScaL kind, Dirichlet walls), the fused sweep generates
- * them in the kernel and reads and writes the field in place (16 B per row
- * instead of build + solve + scatter); the field is bit-identical either
- * way. A captured step is dropped and re-captured by the next graph step. */
+ * are the stencil's (Dirichlet walls: the PaScaL kind, the filtered kinds in
+ * directions of one caller partition, and the filtered kinds' z sweep over
+ * 2..16 caller partitions of at most 64 rows from the second step on), the
+ * fused sweep generates them in the kernel and reads and writes the field in
+ * place (16 B per row instead of build + solve + scatter); the field is
+ * bit-identical either way. A captured step is dropped and re-captured by
+ * the next graph step. */
 ftd_status ftd_heat_set_fused(ftd_heat* h, int enable);
 
 /* Extension (tuning hook): the fused sweep of direction dir (0 = x, 1 = y,
@@ -361,6 +364,13 @@ double ftd_heat_last_step_ms(const ftd_heat* h);
 
 /* text of the last error reported by an ftd_heat_* call in this thread */
 const char* ftd_heat_last_error_string(void);
+
+/* Extension (diagnostics): how the last enqueued step ran each sweep, e.g.
+ * "z:fused M=16 Q=4 S=64; y:fused M=64 Q=1 S=64; x:fused M=64 Q=1 S=16" --
+ * fused (coefficients generated in the kernel), filtered_fused (the filtered
+ * kinds over the caller's z partitions) or materialized (build, solve,
+ * scatter). Owned by h; valid until its next step. */
+const char* ftd_heat_last_plan(const ftd_heat* h);
 
 /* ------------------------------------------------------------------------
  * Channel pressure Poisson solve: the reference's PressureSolver::solve

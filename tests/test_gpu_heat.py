@@ -141,16 +141,24 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
     ("pascal", 64, (2, 4, 8), (0, 0, 0)),     # PaScaL with the caller's partitions
     ("pascal", 40, (1, 2, 1), (1, 0, 0)),     # periodic x: materialized there, fused in y, z
     ("pascal", 37, (1, 1, 3), (0, 0, 0)),     # odd sizes, uneven partitions
-    # filtered kinds: x and y (one partition) fused, z materialized
-    ("filtered", 100, (1, 1, 4), (0, 0, 0)),
-    ("filtered_v2", 48, (1, 1, 4), (0, 0, 0)),
-    ("filtered", 40, (1, 2, 1), (0, 0, 0)),   # y materialized, x and z fused
+    # filtered kinds: x and y (one partition) fused as the exact solve; z over
+    # the caller's partitions by the fused filtered sweep (k_adi_filt_z)
+    ("filtered", 100, (1, 1, 4), (0, 0, 0)),      # 25 rows per partition (M = 32)
+    ("filtered_v2", 48, (1, 1, 4), (0, 0, 0)),    # 12 rows (M = 16)
+    ("filtered_v2", 200, (1, 1, 4), (0, 0, 0)),   # 50 rows (M = 64)
+    ("filtered", 37, (1, 1, 3), (0, 0, 0)),       # uneven 13/12/12
+    ("filtered_v2", 64, (2, 2, 8), (0, 0, 0)),    # 8 z partitions; x, y materialized
+    ("filtered", 150, (1, 1, 16), (0, 0, 0)),     # 16 z partitions of 9-10 rows
+    ("filtered", 40, (1, 2, 1), (0, 0, 0)),       # y materialized, x and z fused
+    ("filtered_v2", 48, (1, 1, 4), (0, 0, 1)),    # periodic z: materialized
 ])
 def test_heat_fused_sweeps_equal_materialized(kind, n, parts, periodic):
     """Fused sweeps (coefficients generated in the kernel, rhs and x in the
     field) give the materialized build + solve + scatter field bit for bit,
     eagerly and from a graph. The filtered kinds fuse the directions of one
-    caller partition, where their solve is the exact one."""
+    caller partition, where their solve is the exact one, and their z sweep
+    over the caller's z partitions (after the first step, which sets the
+    solver's column-0 bounds)."""
     dt = dt_policy("order", 0.3, n)[0]
     note(f"heat fused vs materialized {kind} n={n} parts={parts} periodic={periodic}")
     sims = [SolveTheta(n, n, n, parts=parts, backend=kind, dt=dt, periodic=periodic)
@@ -161,6 +169,10 @@ def test_heat_fused_sweeps_equal_materialized(kind, n, parts, periodic):
             for sim in sims:
                 sim.run(3, use_graph=use_graph)
             np.testing.assert_array_equal(sims[0].field(), sims[1].field())
+        plan = sims[0].last_plan()
+        z_filtered = kind != "pascal" and parts[2] > 1 and not periodic[2]
+        assert ("z:filtered_fused" in plan) == z_filtered, plan
+        assert "fused" not in sims[1].last_plan(), sims[1].last_plan()
     finally:
         for sim in sims:
             sim.close()
