@@ -510,48 +510,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
 }
 
 // ---------------------------------------------------------------------------
-// Fused z sweep of the filtered kinds over the caller's z partitions
-// (Dirichlet walls). The rows are generated as k_build forms them (a = -s.za,
-// b = 1 - s.zb, c = -s.zc, d = the explicit rhs of th as k_rhs_z forms it,
-// the wall fold-in on the system's first and last rows), and the solve is
+// Fused y or z sweep of the filtered kinds over the caller's partitions in
+// that direction (Dirichlet walls). The rows are generated as k_build forms
+// them (a = -s.a, b = 1 - s.b, c = -s.c of the direction; d = for z the
+// explicit rhs of th as k_rhs_z forms it, for y the rhs array; the wall
+// fold-in on the system's first and last rows), and the solve is
 // k_filtered_fused's (ftdma_kernels.hip), statement for statement: the bits
 // of the materialized build + solve + scatter, x written into rhs.
-//   VERSION 2, PASS 0: one pass with J[q] (k_zbound_maxabs + k_compute_J).
+//   VERSION 2, PASS 0: one pass with J[q] (k_bound_maxabs + k_compute_J).
 //   VERSION 1, PASS 1: x0, xN of every (partition, system) to xbuf, the
 //     tile's largest |x0|, |xN| per partition to partial[tile * Q + q];
 //   VERSION 1, PASS 2: the same sweeps, x0, xN from xbuf, J[q] applied.
 // One lane per (system, partition q); S systems per workgroup.
 // ---------------------------------------------------------------------------
-template <int M, int VERSION, int PASS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_filt_z(
-    const double* __restrict__ th, double* rhs, Dims g, int nz, double sa, double sb, double sc,
-    int Q, int lgS, Stencil st, const double* __restrict__ cx, const double* __restrict__ cy,
-    const double* __restrict__ cz, const int* __restrict__ Jp, double* __restrict__ xbuf,
-    double* __restrict__ partial) {
-    extern __shared__ double lds[];
-    const int S = 1 << lgS;
-    const int t = threadIdx.x;
-    const int sl = t & (S - 1);
-    const int q = t >> lgS;
-    const int n_sys = g.nx * g.ny;
-    const int blk = ftd::xcd_tile(blockIdx.x, gridDim.x);
-    const int s_raw = blk * S + sl;
-    const bool live = s_raw < n_sys;
-    const int s = live ? s_raw : n_sys - 1;  // dead lanes shadow the last system
-    const int ii = s % g.nx + 1, oo = s / g.nx + 1;
-    const size_t rs = (size_t)g.px * g.ny1;
-    const size_t here0 = cell(g, ii, oo, 1);
-    int r0, m;
-    {
-        const int base = nz / Q, rem = nz % Q;
-        r0 = q * base + (q < rem ? q : rem);
-        m = base + (q < rem ? 1 : 0);
-    }
-    const size_t lo = here0, hi = here0 + (size_t)(nz - 1) * rs;
-    double* const pbase = rhs + here0 + (size_t)r0 * rs;
-    RowRhsZ rows{th, cz, here0 + (size_t)r0 * rs, (size_t)g.px, rs, m, r0 + 1,
-                 st.xa, st.xc, st.ya, st.yc, st.za, st.zc, 1.0 + st.xb + st.yb + st.zb,
-                 st.c0 * cx[ii] * cy[oo]};
+// rows is taken by value: a reference kept the row source's rolling state
+// in scratch memory (1.5 KB per lane in the 64-row filtered_v2 z variant)
+template <int M, int VERSION, int PASS, typename Rows>
+__device__ __forceinline__ void filt_lane(Rows rows, const double* __restrict__ th, int Q, int q,
+                                          int S, int sl, int m, int n_sys, int s, bool live,
+                                          double sa, double sb, double sc, size_t lo, size_t hi,
+                                          size_t rs, double* pbase, const int* __restrict__ Jp,
+                                          double* __restrict__ xbuf, double* __restrict__ partial,
+                                          int tile, double* lds) {
     const int J = (VERSION == 2 || PASS == 2) ? Jp[q] : m;
     constexpr int PF = M >= 32 ? 2 : 4;
     double rd[M];
@@ -655,7 +635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
         // dead lanes shadow a live system of this workgroup: the max holds
         double v = fmax(fabs(x0), fabs(xN));
         for (int off = S >> 1; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-        if (sl == 0) partial[(size_t)blk * Q + q] = v;
+        if (sl == 0) partial[(size_t)tile * Q + q] = v;
         return;
     }
     // k_final_pass: truncated correction
@@ -681,38 +661,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1
     }
 }
 
-// |d| of rows 0 and m-1 of every z partition, d as k_build forms it (the
-// explicit rhs of th, the wall fold-in on the system's first and last rows),
-// the largest per partition over each block of 256 systems: the tile maxima
+// DIR 2 (z: systems (x, y), rows strided by a plane, d formed from th) or
+// 1 (y: systems (x, z), rows strided by a line, d read from rhs)
+template <int M, int VERSION, int PASS, int DIR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M >= 64 ? 1 : 2))) void k_adi_filt(
+    const double* __restrict__ th, double* rhs, Dims g, int nz, double sa, double sb, double sc,
+    int Q, int lgS, Stencil st, const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ cz, const int* __restrict__ Jp, double* __restrict__ xbuf,
+    double* __restrict__ partial) {
+    extern __shared__ double lds[];
+    const int S = 1 << lgS;
+    const int t = threadIdx.x;
+    const int sl = t & (S - 1);
+    const int q = t >> lgS;
+    const int n_sys = g.nx * (DIR == 2 ? g.ny : nz);
+    const int N = DIR == 2 ? nz : g.ny;
+    // z: each XCD takes a contiguous run of system blocks, so the halo lines
+    // of th its workgroups read are the ones their neighbours read
+    const int blk = DIR == 2 ? ftd::xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int s_raw = blk * S + sl;
+    const bool live = s_raw < n_sys;
+    const int s = live ? s_raw : n_sys - 1;  // dead lanes shadow the last system
+    const int ii = s % g.nx + 1, oo = s / g.nx + 1;
+    const size_t rs = DIR == 2 ? (size_t)g.px * g.ny1 : (size_t)g.px;
+    const size_t here0 = DIR == 2 ? cell(g, ii, oo, 1) : cell(g, ii, 1, oo);
+    int r0, m;
+    {
+        const int base = N / Q, rem = N % Q;
+        r0 = q * base + (q < rem ? q : rem);
+        m = base + (q < rem ? 1 : 0);
+    }
+    const size_t lo = here0, hi = here0 + (size_t)(N - 1) * rs;
+    double* const pbase = rhs + here0 + (size_t)r0 * rs;
+    if constexpr (DIR == 2) {
+        RowRhsZ rows{th, cz, here0 + (size_t)r0 * rs, (size_t)g.px, rs, m, r0 + 1,
+                     st.xa, st.xc, st.ya, st.yc, st.za, st.zc, 1.0 + st.xb + st.yb + st.zb,
+                     st.c0 * cx[ii] * cy[oo]};
+        filt_lane<M, VERSION, PASS>(rows, th, Q, q, S, sl, m, n_sys, s, live, sa, sb, sc, lo, hi,
+                                    rs, pbase, Jp, xbuf, partial, blk, lds);
+    } else {
+        RowAt rows{pbase, rs, m};
+        filt_lane<M, VERSION, PASS>(rows, th, Q, q, S, sl, m, n_sys, s, live, sa, sb, sc, lo, hi,
+                                    rs, pbase, Jp, xbuf, partial, blk, lds);
+    }
+}
+
+// |d| of rows 0 and m-1 of every partition of direction DIR (2: z, d from th
+// as k_rhs_z forms it; 1: y, d from rhs), d as k_build forms it with the
+// wall fold-in on the system's first and last rows, the largest per
+// partition over each block of 256 systems: the tile maxima that
 // k_boundary_maxabs takes from the built D, for the fused filtered_v2 sweep
-__global__ __launch_bounds__(256) void k_zbound_maxabs(const double* __restrict__ th, Dims g,
-                                                       int nz, int Q, double sa, double sc,
-                                                       Stencil st, const double* __restrict__ cx,
-                                                       const double* __restrict__ cy,
-                                                       const double* __restrict__ cz,
-                                                       double* __restrict__ partial) {
+template <int DIR>
+__global__ __launch_bounds__(256) void k_bound_maxabs(const double* __restrict__ th,
+                                                      const double* __restrict__ rhs, Dims g,
+                                                      int nz, int Q, double sa, double sc,
+                                                      Stencil st, const double* __restrict__ cx,
+                                                      const double* __restrict__ cy,
+                                                      const double* __restrict__ cz,
+                                                      double* __restrict__ partial) {
     __shared__ double red[4];
     const int q = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
-    const int n_sys = g.nx * g.ny;
+    const int n_sys = g.nx * (DIR == 2 ? g.ny : nz);
+    const int N = DIR == 2 ? nz : g.ny;
     double v = 0.0;
     if (s < n_sys) {
         const int ii = s % g.nx + 1, oo = s / g.nx + 1;
         const size_t sy = (size_t)g.px, sz = (size_t)g.px * g.ny1;
-        const int base = nz / Q, rem = nz % Q;
+        const size_t rs = DIR == 2 ? sz : sy;
+        const int base = N / Q, rem = N % Q;
         const int r0 = q * base + (q < rem ? q : rem);
         const int m = base + (q < rem ? 1 : 0);
-        const double diag = 1.0 + st.xb + st.yb + st.zb;
-        const double cxy = st.c0 * cx[ii] * cy[oo];
-        const size_t lo = cell(g, ii, oo, 1), hi = lo + (size_t)(nz - 1) * sz;
+        const size_t lo = DIR == 2 ? cell(g, ii, oo, 1) : cell(g, ii, 1, oo);
+        const size_t hi = lo + (size_t)(N - 1) * rs;
         auto d_of = [&](int row) {
-            const size_t c = lo + (size_t)row * sz;
-            double d = (st.xc * th[c + 1] + st.xa * th[c - 1]) +
-                       (st.yc * th[c + sy] + st.ya * th[c - sy]) +
-                       (st.zc * th[c + sz] + st.za * th[c - sz]) + diag * th[c] +
-                       cxy * cz[row + 1];
-            if (row == 0) d += sa * (th[lo - sz] + th[lo]);
-            if (row == nz - 1) d += sc * (th[hi + sz] + th[hi]);
+            const size_t c = lo + (size_t)row * rs;
+            double d;
+            if constexpr (DIR == 2) {
+                const double diag = 1.0 + st.xb + st.yb + st.zb;
+                const double cxy = st.c0 * cx[ii] * cy[oo];
+                d = (st.xc * th[c + 1] + st.xa * th[c - 1]) +
+                    (st.yc * th[c + sy] + st.ya * th[c - sy]) +
+                    (st.zc * th[c + sz] + st.za * th[c - sz]) + diag * th[c] + cxy * cz[row + 1];
+            } else {
+                d = rhs[c];
+            }
+            if (row == 0) d += sa * (th[lo - rs] + th[lo]);
+            if (row == N - 1) d += sc * (th[hi + rs] + th[hi]);
             return d;
         };
         v = fmax(fabs(d_of(r0)), fabs(d_of(r0 + m - 1)));
@@ -882,13 +917,13 @@ struct ftd_heat {
         int on = 0, M = 0, thomas = 0, Q = 0, lgS = 0;
         size_t lds = 0;
     } fused[3];
-    // fused z sweep of the filtered kinds over the caller's z partitions
-    // (k_adi_filt_z): enabled, rows per lane, partitions, log2 systems per
-    // workgroup, dynamic LDS bytes of the exchange
-    struct FiltZ {
+    // fused y, z sweeps of the filtered kinds over the caller's partitions
+    // (k_adi_filt): enabled, rows per lane, partitions, log2 systems per
+    // workgroup, dynamic LDS bytes of the exchange (x stays materialized)
+    struct Filt {
         int on = 0, M = 0, Q = 0, lgS = 0;
         size_t lds = 0;
-    } filt;
+    } filt[3];
     int use_fused = 1;
     int fuse_rhs = 0;  // the fused z sweep forms the rhs (no k_rhs_z pass) this step
     char plan[256] = "";  // how the last enqueued step ran each sweep (ftd_heat_last_plan)
@@ -980,14 +1015,17 @@ void plan_fused(ftd_heat* h, int d, int N, int n_sys, int rows_per_lane = 0,
     h->fused[d] = f;
 }
 
-// the filtered kinds' fused z sweep: a filtered kind split in z into 2..16
-// Dirichlet partitions of 3..64 rows. The tile is halved (down to 16
-// systems) until the grid covers the CUs; the bits do not depend on it.
-void plan_filt_z(ftd_heat* h) {
-    ftd_heat::FiltZ f;
-    const int Q = h->parts[2], N = h->nz, n_sys = h->g.nx * h->g.ny;
-    h->filt = f;
-    if (h->kind == FTD_KIND_PASCAL || Q < 2 || Q > 16 || h->periodic[2] || n_sys < 1) return;
+// the filtered kinds' fused sweep of direction d (1 = y, 2 = z): a filtered
+// kind split there into 2..16 Dirichlet partitions of 3..64 rows. The tile
+// is halved (down to 16 systems) until the grid covers the CUs; the bits do
+// not depend on it.
+void plan_filt(ftd_heat* h, int d) {
+    ftd_heat::Filt f;
+    const int Q = h->parts[d], N = d == 2 ? h->nz : h->g.ny;
+    const int n_sys = h->g.nx * (d == 2 ? h->g.ny : h->nz);
+    h->filt[d] = f;
+    if (d == 0 || h->kind == FTD_KIND_PASCAL || Q < 2 || Q > 16 || h->periodic[d] || n_sys < 1)
+        return;
     const int m_max = (N + Q - 1) / Q, m_min = N / Q;
     if (m_max > 64 || m_min < 3) return;
     f.M = m_max <= 16 ? 16 : m_max <= 32 ? 32 : 64;
@@ -1001,70 +1039,74 @@ void plan_filt_z(ftd_heat* h) {
     f.lgS = lg;
     f.lds = sizeof(double) * 4 * (size_t)Q * S;
     f.on = 1;
-    h->filt = f;
+    h->filt[d] = f;
 }
 
-// whether this step's z sweep can run fused: planned, fused sweeps on, the
-// solver's column-0 bounds set (by the first, materialized step) and the
-// solver in its plain mode (no learned J, no guard)
-bool filt_z_ready(ftd_heat* h) {
-    if (!h->use_fused || !h->filt.on || !h->rho_set[2]) return false;
+// whether this step's sweep of direction d can run fused: planned, fused
+// sweeps on, the solver's column-0 bounds set (by the first, materialized
+// step) and the solver in its plain mode (no learned J, no guard)
+bool filt_ready(ftd_heat* h, int d) {
+    if (!h->use_fused || !h->filt[d].on || !h->rho_set[d]) return false;
     int* J;
     const double* rho;
     double ev;
     int ed;
-    return ftd::handle_filtered_state(h->solver[2], &J, &rho, &ev, &ed) == FTD_OK;
+    return ftd::handle_filtered_state(h->solver[d], &J, &rho, &ev, &ed) == FTD_OK;
 }
 
-template <int M>
-hipError_t launch_filt_z_t(ftd_heat* h, hipStream_t st) {
-    const ftd_heat::FiltZ& f = h->filt;
+template <int M, int DIR>
+hipError_t launch_filt_t(ftd_heat* h, hipStream_t st) {
+    const ftd_heat::Filt& f = h->filt[DIR];
     int* J;
     const double* rho;
     double ev;
     int ed;
-    if (ftd::handle_filtered_state(h->solver[2], &J, &rho, &ev, &ed) != FTD_OK)
+    if (ftd::handle_filtered_state(h->solver[DIR], &J, &rho, &ev, &ed) != FTD_OK)
         return hipErrorInvalidValue;
     const Dims g = h->g;
-    const int S = 1 << f.lgS, n_sys = g.nx * g.ny;
+    const int S = 1 << f.lgS, n_sys = g.nx * (DIR == 2 ? g.ny : h->nz);
+    const int N = DIR == 2 ? h->nz : g.ny;
     const dim3 grid((n_sys + S - 1) / S), block(S * f.Q);
     // scratch: the materialized sweeps' A (x0, xN: 2Q x n_sys) and B (tile
-    // maxima); the z sweep runs first in the step
+    // maxima); sweeps run one after the other on the stream
     double* const xbuf = h->A;
     double* const partial = h->B;
     const Stencil& s = h->s;
+    const double sa = DIR == 2 ? s.za : s.ya, sb = DIR == 2 ? s.zb : s.yb,
+                 sc = DIR == 2 ? s.zc : s.yc;
     hipError_t e;
     if (h->kind == FTD_KIND_FILTERED_V2) {
         const dim3 gb((n_sys + 255) / 256, f.Q);
-        hipLaunchKernelGGL(k_zbound_maxabs, gb, dim3(256), 0, st, h->th, g, h->nz, f.Q, s.za,
-                           s.zc, s, h->cx, h->cy, h->cz, partial);
+        hipLaunchKernelGGL((k_bound_maxabs<DIR>), gb, dim3(256), 0, st, h->th, h->rhs, g, h->nz,
+                           f.Q, sa, sc, s, h->cx, h->cy, h->cz, partial);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = ftd::launch_compute_J(partial, (int)gb.x, f.Q, h->nz, rho, ev, ed, 0, J, st)) !=
+        if ((e = ftd::launch_compute_J(partial, (int)gb.x, f.Q, N, rho, ev, ed, 0, J, st)) !=
             hipSuccess)
             return e;
-        hipLaunchKernelGGL((k_adi_filt_z<M, 2, 0>), grid, block, f.lds, st, h->th, h->rhs, g,
-                           h->nz, s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf,
+        hipLaunchKernelGGL((k_adi_filt<M, 2, 0, DIR>), grid, block, f.lds, st, h->th, h->rhs, g,
+                           h->nz, sa, sb, sc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf,
                            partial);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_adi_filt_z<M, 1, 1>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz,
-                       s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
+    hipLaunchKernelGGL((k_adi_filt<M, 1, 1, DIR>), grid, block, f.lds, st, h->th, h->rhs, g, h->nz,
+                       sa, sb, sc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = ftd::launch_compute_J(partial, (int)grid.x, f.Q, h->nz, rho, ev, ed, 0, J, st)) !=
+    if ((e = ftd::launch_compute_J(partial, (int)grid.x, f.Q, N, rho, ev, ed, 0, J, st)) !=
         hipSuccess)
         return e;
-    hipLaunchKernelGGL((k_adi_filt_z<M, 1, 2>), grid, block, 0, st, h->th, h->rhs, g, h->nz,
-                       s.za, s.zb, s.zc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
+    hipLaunchKernelGGL((k_adi_filt<M, 1, 2, DIR>), grid, block, 0, st, h->th, h->rhs, g, h->nz,
+                       sa, sb, sc, f.Q, f.lgS, s, h->cx, h->cy, h->cz, J, xbuf, partial);
     return hipGetLastError();
 }
 
-hipError_t launch_filt_z(ftd_heat* h, hipStream_t st) {
-    switch (h->filt.M) {
-        case 16: return launch_filt_z_t<16>(h, st);
-        case 32: return launch_filt_z_t<32>(h, st);
-        case 64: return launch_filt_z_t<64>(h, st);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t launch_filt(ftd_heat* h, int d, hipStream_t st) {
+#define FTD_FILT_CASE(m_)                                                              \
+    if (h->filt[d].M == m_) return d == 2 ? launch_filt_t<m_, 2>(h, st) : launch_filt_t<m_, 1>(h, st);
+    FTD_FILT_CASE(16)
+    FTD_FILT_CASE(32)
+    FTD_FILT_CASE(64)
+#undef FTD_FILT_CASE
+    return hipErrorInvalidValue;
 }
 
 template <int M, bool TH>
@@ -1156,7 +1198,7 @@ ftd_status enqueue_step(ftd_heat* h) {
     const int z_groups = (g.nx * g.ny + (1 << fz.lgS) - 1) >> fz.lgS;
     h->fuse_rhs = h->use_fused && fz.on && !(fz.thomas && z_groups < device_cus());
     // the filtered kinds' fused z sweep forms the rhs itself as well
-    const bool z_filt = filt_z_ready(h);
+    const bool z_filt = filt_ready(h, 2);
     if (!h->fuse_rhs && !z_filt) {
         const int kb = h->rhs_kb;
         hipLaunchKernelGGL(k_rhs_z,
@@ -1188,12 +1230,12 @@ ftd_status enqueue_step(ftd_heat* h) {
             note_plan(d, "fused", h->fused[d].M, h->fused[d].Q, h->fused[d].lgS);
             continue;
         }
-        if (d == 2 && z_filt) {
-            // the filtered kinds over the caller's z partitions: rows
-            // generated, rhs formed from th, x written into rhs
-            HT(ftd_set_eps_constant(h->solver[2], h->dt));
-            HH(launch_filt_z(h, st));
-            note_plan(d, "filtered_fused", h->filt.M, h->filt.Q, h->filt.lgS);
+        if ((d == 2 && z_filt) || (d == 1 && filt_ready(h, 1))) {
+            // the filtered kinds over the caller's partitions: rows
+            // generated (z: the rhs formed from th), x written into rhs
+            HT(ftd_set_eps_constant(h->solver[d], h->dt));
+            HH(launch_filt(h, d, st));
+            note_plan(d, "filtered_fused", h->filt[d].M, h->filt[d].Q, h->filt[d].lgS);
             continue;
         }
         note_plan(d, "materialized", 0, 0, 0);
@@ -1315,7 +1357,8 @@ ftd_status ftd_heat_create(ftd_heat** out, int nx, int ny, int nz, const int* pa
     h->rhs_kb = rhs_planes(nx, ny, nz);
     // x, y, z sweeps: n_sys = ny*nz, nx*nz, nx*ny
     const int ns[3] = {ny * nz, nx * nz, nx * ny}, nr[3] = {nx, ny, nz};
-    plan_filt_z(h);
+    plan_filt(h, 1);
+    plan_filt(h, 2);
     for (int d = 0; d < 3; ++d) {
         plan_fused(h, d, nr[d], ns[d]);
         if (h->fused[d].on && h->fused[d].lds > 64 * 1024) {

@@ -336,8 +336,8 @@ ftd_status ftd_heat_step(ftd_heat* h, int nsteps, int use_graph);
 
 /* Extension: fused sweeps on (default) or off. Where a sweep's coefficients
  * are the stencil's (Dirichlet walls: the PaScaL kind, the filtered kinds in
- * directions of one caller partition, and the filtered kinds' z sweep over
- * 2..16 caller partitions of at most 64 rows from the second step on), the
+ * directions of one caller partition, and the filtered kinds' y and z sweeps
+ * over 2..16 caller partitions of at most 64 rows from the second step on), the
  * fused sweep generates them in the kernel and reads and writes the field in
  * place (16 B per row instead of build + solve + scatter); the field is
  * bit-identical either way. A captured step is dropped and re-captured by

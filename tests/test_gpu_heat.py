@@ -149,8 +149,12 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
     ("filtered", 37, (1, 1, 3), (0, 0, 0)),       # uneven 13/12/12
     ("filtered_v2", 64, (2, 2, 8), (0, 0, 0)),    # 8 z partitions; x, y materialized
     ("filtered", 150, (1, 1, 16), (0, 0, 0)),     # 16 z partitions of 9-10 rows
-    ("filtered", 40, (1, 2, 1), (0, 0, 0)),       # y materialized, x and z fused
+    ("filtered", 40, (1, 2, 1), (0, 0, 0)),       # y over 2 partitions, x and z exact
+    ("filtered_v2", 48, (1, 4, 1), (0, 0, 0)),    # y over 4 partitions of 12 rows
+    ("filtered", 70, (1, 3, 2), (0, 0, 0)),       # y 24/23/23 rows and z 35 rows (M = 64)
+    ("filtered_v2", 48, (3, 2, 2), (0, 0, 0)),    # x split: materialized there
     ("filtered_v2", 48, (1, 1, 4), (0, 0, 1)),    # periodic z: materialized
+    ("filtered", 48, (1, 4, 1), (0, 1, 0)),       # periodic y: materialized
 ])
 def test_heat_fused_sweeps_equal_materialized(kind, n, parts, periodic):
     """Fused sweeps (coefficients generated in the kernel, rhs and x in the
@@ -170,8 +174,11 @@ def test_heat_fused_sweeps_equal_materialized(kind, n, parts, periodic):
                 sim.run(3, use_graph=use_graph)
             np.testing.assert_array_equal(sims[0].field(), sims[1].field())
         plan = sims[0].last_plan()
-        z_filtered = kind != "pascal" and parts[2] > 1 and not periodic[2]
-        assert ("z:filtered_fused" in plan) == z_filtered, plan
+        for d, name in ((1, "y"), (2, "z")):
+            fused_filtered = kind != "pascal" and parts[d] > 1 and not periodic[d]
+            assert (f"{name}:filtered_fused" in plan) == fused_filtered, plan
+        if kind != "pascal":  # the filtered kinds' split x stays materialized
+            assert ("x:materialized" in plan) == (parts[0] > 1 or bool(periodic[0])), plan
         assert "fused" not in sims[1].last_plan(), sims[1].last_plan()
     finally:
         for sim in sims:
