@@ -142,7 +142,7 @@ def test_heat_config1_filtered_vs_pascal_vs_thomas():
     ("pascal", 40, (1, 2, 1), (1, 0, 0)),     # periodic x: materialized there, fused in y, z
     ("pascal", 37, (1, 1, 3), (0, 0, 0)),     # odd sizes, uneven partitions
     # filtered kinds: x and y (one partition) fused as the exact solve; z over
-    # the caller's partitions by the fused filtered sweep (k_adi_filt_z)
+    # the caller's partitions by the fused filtered sweep (k_adi_filt)
     ("filtered", 100, (1, 1, 4), (0, 0, 0)),      # 25 rows per partition (M = 32)
     ("filtered_v2", 48, (1, 1, 4), (0, 0, 0)),    # 12 rows (M = 16)
     ("filtered_v2", 200, (1, 1, 4), (0, 0, 0)),   # 50 rows (M = 64)
