@@ -2096,19 +2096,30 @@ __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__
         Dp[o] *= r0;
     }
     double am = Ap[ld], cm = Cp[ld], dm = Dp[ld];
+    // Row j + 1's loads are issued before row j's stores: the compiler cannot
+    // move them there itself (the same pointers one row apart, and it cannot
+    // rule out ld == 0), and a lane then waits for every row's loads in turn.
+    // The next row is clamped to the partition's last row (a reload, unused).
     int j = 2;
+    const int j2 = 2 < m ? 2 : m - 1;
+    double na = Ap[(size_t)j2 * ld], nb = Bp[(size_t)j2 * ld], nc = Cp[(size_t)j2 * ld],
+           nd = Dp[(size_t)j2 * ld];
     for (; j <= J && j < m; ++j) {
         const size_t o = (size_t)j * ld;
-        const double aj = Ap[o];
+        const double aj = na, bj = nb, cj = nc, dj = nd;
+        const size_t on = (size_t)(j + 1 < m ? j + 1 : m - 1) * ld;
+        na = Ap[on];
+        nb = Bp[on];
+        nc = Cp[on];
+        nd = Dp[on];
         if (FEAT) {
-            const double cj = Cp[o], dj = Dp[o];
-            f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(Bp[o]));
+            f_rho = fmax(f_rho, fmax(fabs(aj), fabs(cj)) / fabs(bj));
             if (j == m - 1) f_dN = dj;
             f_dmax = fmax(f_dmax, fabs(dj));
         }
-        const double inv = 1.0 / (Bp[o] - aj * cm);
-        dm = inv * (Dp[o] - aj * dm);
-        cm = inv * Cp[o];
+        const double inv = 1.0 / (bj - aj * cm);
+        dm = inv * (dj - aj * dm);
+        cm = inv * cj;
         am = -inv * aj * am;
         Ap[o] = am;
         Cp[o] = cm;
@@ -2122,28 +2133,55 @@ __global__ __launch_bounds__(256) void k_gen_filtered_sweep(double* __restrict__
         Jsys[(size_t)p * ld + i] = Jl < 0 ? 0 : (Jl > m - 1 ? m - 1 : Jl);
     }
     if (VERSION == 2) {
-        for (j = (J + 1 > 1 ? J + 1 : 1); j < m; ++j) {
+        // the skip-A rows, loads one row ahead as above
+        j = J + 1 > 1 ? J + 1 : 1;
+        if (j < m) {
+            na = Ap[(size_t)j * ld];
+            nb = Bp[(size_t)j * ld];
+            nc = Cp[(size_t)j * ld];
+            nd = Dp[(size_t)j * ld];
+        }
+        for (; j < m; ++j) {
             const size_t o = (size_t)j * ld;
-            const double aj = Ap[o];
-            const double inv = 1.0 / (Bp[o] - aj * cm);
-            dm = inv * (Dp[o] - aj * dm);
-            cm = inv * Cp[o];
+            const double aj = na, bj = nb, cj = nc, dj = nd;
+            const size_t on = (size_t)(j + 1 < m ? j + 1 : m - 1) * ld;
+            na = Ap[on];
+            nb = Bp[on];
+            nc = Cp[on];
+            nd = Dp[on];
+            const double inv = 1.0 / (bj - aj * cm);
+            dm = inv * (dj - aj * dm);
+            cm = inv * cj;
             Cp[o] = cm;
             Dp[o] = dm;
         }
     }
-    // backward
+    // backward, row j - 1's loads issued before row j's stores
     const size_t onm2 = (size_t)(m - 2) * ld;
     double dn = Dp[onm2], cn = Cp[onm2];
     double an = (VERSION == 1) ? Ap[onm2] : (J >= 2 ? Ap[(size_t)J * ld] : 0.0);
     const int lo = (m - 2) - J;
-    for (j = m - 3; j >= 1; --j) {
+    // (v2 reads a row's A only where it updates a, rows below J: J is the
+    // partition's, so the test is uniform across the wave)
+    j = m - 3;
+    double pa = 0.0, pc = 0.0, pd = 0.0;
+    if (j >= 1) {
+        if (VERSION == 1 || (J >= 2 && j < J)) pa = Ap[(size_t)j * ld];
+        pc = Cp[(size_t)j * ld];
+        pd = Dp[(size_t)j * ld];
+    }
+    for (; j >= 1; --j) {
         const size_t o = (size_t)j * ld;
-        const double cj = Cp[o];
-        dn = Dp[o] - cj * dn;
+        const double aj = pa, cj = pc, dj = pd;
+        const int jp = j > 1 ? j - 1 : 1;
+        const size_t op = (size_t)jp * ld;
+        if (VERSION == 1 || (J >= 2 && jp < J)) pa = Ap[op];
+        pc = Cp[op];
+        pd = Dp[op];
+        dn = dj - cj * dn;
         Dp[o] = dn;
         if (VERSION == 1 || (J >= 2 && j < J)) {
-            an = Ap[o] - cj * an;
+            an = aj - cj * an;
             Ap[o] = an;
         }
         if (VERSION == 1 || j > lo) {
