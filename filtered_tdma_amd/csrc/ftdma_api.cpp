@@ -193,10 +193,13 @@ bool pick_pascal(const ftd_handle* h, int n_row, int cyclic, FusedShape& out, in
         }
     }
     (void)cyclic;
-    if (csh) {
+    if (csh || n_row > 8 * 64) {
         // complex right-hand sides, one caller partition: 32-row partitions
         // in 512-thread workgroups measured fastest (16384 x 129 and
-        // 33024 x 256 channel modes: 9-17 % under 64-row ones)
+        // 33024 x 256 channel modes: 9-17 % under 64-row ones). So do real
+        // systems longer than eight 64-row partitions, where either shape
+        // holds 16 systems per workgroup: 262144 systems of 600 / 768 / 1024
+        // rows in 1.83 / 2.23 / 2.96 against 1.99 / 2.47 / 3.19 ms
         const int Q = std::max(2, (n_row + 31) / 32);
         FusedShape fs;
         if (shape_for(32, 512, Q, n_row, 0, fs) && fs.S >= 16) {

@@ -77,6 +77,8 @@ PASCAL_SHAPES = [
     (300, 100, None),
     (5, 37, None),
     (1, 17, None),
+    (300, 1000, None),  # above 512 rows: 32-row partitions in 512-thread groups
+    (64, 600, None),
     (513, 96, (16, 512, 16, 0)),
     (200, 240, (32, 512, 16, 0)),
     (130, 500, (64, 256, 32, 0)),
