@@ -5,14 +5,16 @@
 // B, C, D (rows of one system contiguous, systems anywhere in the arrays).
 // D is overwritten with x; A and B are read only.
 //
-// Work split (planned on the host once per batch layout):
-//   * a system of N_s <= 48 rows is one lane: plain Thomas with row 0
+// Work split (planned on the host once per batch layout), with chunks of
+// at most K = 48 rows (the default) or 32:
+//   * a system of N_s <= K rows is one lane: plain Thomas with row 0
 //     divided by b (the reference's single-rank path, tdma_local_cuda.cu:32-64);
-//   * a longer system is cut by para_range into Q_s = ceil(N_s / 48) chunks,
+//   * a longer system is cut by para_range into Q_s = ceil(N_s / K) chunks,
 //     one lane each, on consecutive lanes of ONE work item: modified Thomas
 //     per chunk, the 2Q_s-row reduced system (B == 1) solved in the item's
-//     LDS by the lane of chunk 0, then the interior update -- PaScaL with
-//     P = Q_s (tdma_local_cuda.cu:134-224) for that system.
+//     LDS (Thomas in the lane of chunk 0 below 16 rows, else PaScaL over
+//     groups of its rows, one lane per group), then the interior update --
+//     PaScaL with P = Q_s (tdma_local_cuda.cu:134-224) for that system.
 // A work item is one workgroup of 128 lanes (two waves) of one kind only
 // (chunked systems, or single-chunk systems). The chunked items come first,
 // then the single-chunk ones, each range ordered by estimated cost, longest
@@ -23,7 +25,7 @@
 // rows are therefore staged through LDS in coalesced blocks (below), and the
 // elimination reads them from there.
 //
-// Systems longer than 128 * 48 rows take a lane-per-system Thomas in global
+// Systems longer than 128 * K rows take a lane-per-system Thomas in global
 // memory (C is used as scratch for c', which the reference contract allows).
 #include <hip/hip_runtime.h>
 
@@ -57,10 +59,11 @@ namespace {
 
 // Rows per lane, a plan parameter with two compiled shapes. 48 is the longest
 // chunk whose elimination state (a, c, d of every row) fits the register
-// file without scratch (64 does not): fewer chunks per system, so a shorter
-// serial reduced system -- the 20000-system batch ran in 0.264 ms against
-// 0.290 with 32. 32-row chunks leave more lanes to batches of many short
-// systems; ftd_autotune_batched times both on the caller's layout.
+// file without scratch (64 does not), and the default: systems of up to
+// 128 * 48 rows stay on the chunked kernels. 32-row chunks give more items
+// and shorter serial parts per item; with the reduced systems solved over
+// groups of rows they are the faster shape on the 20000-system batch (0.232
+// against 0.245 ms). ftd_autotune_batched times both on the caller's layout.
 constexpr int kChunk = 48;  // the default
 constexpr int kLanes = 128;  // lanes per work item (two waves)
 
