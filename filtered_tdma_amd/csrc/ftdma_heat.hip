@@ -10,6 +10,11 @@
 // Per step (solve_theta.cpp:97-334):
 //   Dirichlet ghosts -> explicit RHS -> Z: build, solve, copy -> Y: build,
 //   solve, copy -> X: build, solve, copy into theta -> periodic ghosts.
+// Where a sweep can, it runs fused: one kernel generates a, b, c from the
+// stencil, reads d from the field (or, in the z sweep, forms it from theta
+// in place of the separate RHS pass), solves and writes x into the field. The
+// materialized build + solve + scatter stays as the other path and gives
+// the same bits.
 // Transcendental factors (cos/sin of the cell centres) are tabulated on the
 // host with the C library, so the device arithmetic is exactly the CPU
 // restatement's; the whole step can be captured once into a hipGraph and

@@ -5,13 +5,22 @@
 // addresses of one row: every row load of a wave is one or more contiguous
 // 128..512-byte segments.
 //
-// Two families of kernels:
+// Three families of kernels:
+//
+//  * streamed kernels (k_pascal_stream, k_filtered_stream): a resident grid
+//    walks tiles of 32 systems x Q partitions; the rows of a tile come into
+//    an LDS ring by direct global->LDS DMA, R rows ahead of their use, and
+//    each lane keeps its partition's elimination state in registers. Used
+//    where a compiled (M, Q) variant holds the partitions and n_sys % 32 ==
+//    0 with 16-byte aligned arrays.
 //
 //  * fused kernels (k_pascal_fused, k_filtered_fused): one lane per
 //    (system, partition). A lane keeps the rows of its partition in
 //    registers (M rows x 3 doubles), the partitions of S systems meet in LDS
-//    for the reduced solve, and D is written once. HBM traffic is the
-//    algorithmic minimum: read A,B,C,D once, write x once (40 B/row).
+//    for the reduced solve, and D is written once. Any n_sys and alignment.
+//
+//   Both move the algorithmic minimum: read A,B,C,D once, write x once
+//   (40 B/row).
 //
 //  * general kernels (k_gen_*): one lane per (system, partition) walking its
 //    rows in global memory, in place in A, C, D as the reference contract
