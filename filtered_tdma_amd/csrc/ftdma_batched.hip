@@ -880,6 +880,31 @@ void split(int n, int P, int p, int& s, int& m) {
     m = base + (p < rem ? 1 : 0);
 }
 
+// Chunk starts (local rows, Q + 1 of them) of a system of N > chunk rows at
+// element offset off, cut so that every chunk after the first starts on a
+// 16-element boundary of the batch arrays: a 128-byte line of FP64 data, so
+// a chunk's staged blocks of 8 rows are whole halves of lines instead of
+// segments straddling two (the 20000-system batch laid out line-aligned ran
+// in 0.194 ms against 0.232 at arbitrary offsets). The first chunk runs to
+// chunk - 16 rows past the first boundary (to chunk rows where the system
+// starts on one), the others are chunk rows, and a last chunk under 16 rows
+// takes 16 from the one before. Returns Q, or 0 where the system keeps the
+// para_range split: a remainder under 16 rows after the first chunk, or
+// more than kLanes chunks.
+int aligned_starts(int64_t off, int N, int chunk, int* starts) {
+    const int a = (int)((16 - off % 16) % 16);
+    const int f = a ? a + chunk - 16 : chunk;
+    const int rem = N - f;
+    if (rem < 16) return 0;
+    const int Q = 1 + (rem + chunk - 1) / chunk;
+    if (Q > kLanes) return 0;
+    starts[0] = 0;
+    for (int q = 1; q < Q; ++q) starts[q] = f + (q - 1) * chunk;
+    starts[Q] = N;
+    if (Q >= 3 && N - starts[Q - 1] < 16) starts[Q - 1] -= 16;
+    return Q;
+}
+
 }  // namespace
 
 int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int chunk,
@@ -895,10 +920,13 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int
         if (n_rows[s] < 1) return -1;
         last = std::max(last, row_offsets[s] + n_rows[s] - 1);
     }
-    std::vector<int> multi, single;
+    std::vector<int> multi, single, nq(n_systems);
+    std::vector<int> tmp(kLanes + 1);
     for (int s = 0; s < n_systems; ++s) {
         const int N = n_rows[s];
         const int Q = (N + chunk - 1) / chunk;
+        const int Qa = Q >= 2 ? aligned_starts(row_offsets[s], N, chunk, tmp.data()) : 0;
+        nq[s] = Qa ? Qa : Q;
         // the staging reads two elements at a time, so a batch spanning one
         // element (a single 1-row system) takes the lane-per-system kernel
         if (Q > kLanes || last == 0) {
@@ -910,7 +938,7 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int
             single.push_back(s);
         }
     }
-    auto chunks = [&](int s) { return (n_rows[s] + chunk - 1) / chunk; };
+    auto chunks = [&](int s) { return nq[s]; };
     // longest first; ties keep the caller's order
     std::stable_sort(multi.begin(), multi.end(), [&](int x, int y) { return chunks(x) > chunks(y); });
     std::stable_sort(single.begin(), single.end(),
@@ -935,9 +963,15 @@ int batch_plan(int n_systems, const int64_t* row_offsets, const int* n_rows, int
         int used = 0, maxQ = 0, maxm = 0;
         auto put = [&](int s) {
             const int Q = chunks(s), N = n_rows[s];
+            const bool al = aligned_starts(row_offsets[s], N, chunk, tmp.data()) == Q;
             for (int q = 0; q < Q; ++q) {
                 int st, m;
-                split(N, Q, q, st, m);
+                if (al) {
+                    st = tmp[q];
+                    m = tmp[q + 1] - tmp[q];
+                } else {
+                    split(N, Q, q, st, m);
+                }
                 BatchLane b{};
                 b.off = row_offsets[s] + st;
                 b.m = m;

@@ -81,7 +81,9 @@ struct alignas(16) BatchLane {
 };
 static_assert(sizeof(BatchLane) == 16, "BatchLane is one dwordx4");
 
-// Plans the batch in chunks of `chunk` rows (32 or 48): lanes (128 per item;
+// Plans the batch in chunks of at most `chunk` rows (32 or 48; a system's
+// chunks after the first start on 16-element boundaries where its length
+// allows, else the para_range split): lanes (128 per item;
 // the n_chunked items of chunked systems first, then single-chunk items, each
 // range longest first), the systems longer than 128 * chunk rows
 // (lane-per-system kernel), and the last element any system holds (the

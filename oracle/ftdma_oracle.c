@@ -316,6 +316,17 @@ static int check_parts(int n_row, int P) {
     return 0;
 }
 
+/* partition p: rows starts[p] .. starts[p + 1] - 1 where the caller gives
+ * the P + 1 starts, else the para_range split */
+static void part_of_rows(int n_row, int P, const int* starts, int p, int* s, int* m) {
+    if (starts) {
+        *s = starts[p];
+        *m = starts[p + 1] - starts[p];
+    } else {
+        part_rows(n_row, P, p, s, m);
+    }
+}
+
 /* The 2P-row reduced system solved itself by PaScaL over G groups of rows
  * (para_range split, B == 1): modified Thomas per group, the 2G-row system
  * of the groups' end rows by Thomas, then the groups' interior update --
@@ -353,11 +364,11 @@ static void reduced_nested_cols(double* Ar, double* Br, double* Cr, double* Dr, 
 }
 
 static int pascal_impl(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
-                       int P, int cyclic, int sem, int nthreads, int G);
+                       int P, int cyclic, int sem, int nthreads, int G, const int* starts);
 
 int ftdo_pascal(double* A, const double* B, double* C, double* D,
                 int n_sys, int n_row, int P, int cyclic, int sem, int nthreads) {
-    return pascal_impl(A, B, C, D, n_sys, n_row, P, cyclic, sem, nthreads, 1);
+    return pascal_impl(A, B, C, D, n_sys, n_row, P, cyclic, sem, nthreads, 1, NULL);
 }
 
 int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
@@ -368,12 +379,29 @@ int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
         part_rows(2 * P, G, G - 1, &s, &m);
         if (m < 3) return fail(-2, "pascal_nested: every group needs at least 3 reduced rows");
     }
-    return pascal_impl(A, B, C, D, n_sys, n_row, P, 0, sem, nthreads, G);
+    return pascal_impl(A, B, C, D, n_sys, n_row, P, 0, sem, nthreads, G, NULL);
+}
+
+int ftdo_pascal_starts(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
+                       int P, const int* starts, int G, int sem, int nthreads) {
+    if (P < 2) return fail(-1, "pascal_starts: need P >= 2");
+    if (starts[0] != 0 || starts[P] != n_row)
+        return fail(-1, "pascal_starts: the starts must run from 0 to n_row");
+    for (int p = 0; p < P; ++p)
+        if (starts[p + 1] - starts[p] < 3)
+            return fail(-2, "pascal_starts: every partition needs at least 3 rows");
+    if (G < 1) return fail(-1, "pascal_starts: need G >= 1");
+    if (G > 1) {
+        int s, m;
+        part_rows(2 * P, G, G - 1, &s, &m);
+        if (m < 3) return fail(-2, "pascal_starts: every group needs at least 3 reduced rows");
+    }
+    return pascal_impl(A, B, C, D, n_sys, n_row, P, 0, sem, nthreads, G, starts);
 }
 
 static int pascal_impl(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
-                       int P, int cyclic, int sem, int nthreads, int G) {
-    int rc = check_parts(n_row, P);
+                       int P, int cyclic, int sem, int nthreads, int G, const int* starts) {
+    int rc = starts ? 0 : check_parts(n_row, P);
     if (rc) return rc;
     if (P == 1) {
         /* pascal_tdma.cpp:116-119, pascal_tdma_cuda.cu:266-270 */
@@ -401,7 +429,7 @@ static int pascal_impl(double* A, const double* B, double* C, double* D, int n_s
         int c0 = k * CHUNK, c1 = c0 + CHUNK < n_sys ? c0 + CHUNK : n_sys;
         for (int p = 0; p < P; ++p) {
             int s, m;
-            part_rows(n_row, P, p, &s, &m);
+            part_of_rows(n_row, P, starts, p, &s, &m);
             double* R = Rp + (size_t)6 * p * ld;
             modthomas_cols(ROW(A, s), ROW(B, s), ROW(C, s), ROW(D, s), ld, c0, c1, m, sem, R);
             for (int i = c0; i < c1; ++i) {
@@ -423,7 +451,7 @@ static int pascal_impl(double* A, const double* B, double* C, double* D, int n_s
             reduced_cols(Ar, Br, Cr, Dr, Rp, ld, c0, c1, n_rd, cyclic, sem);
         for (int p = 0; p < P; ++p) {
             int s, m;
-            part_rows(n_row, P, p, &s, &m);
+            part_of_rows(n_row, P, starts, p, &s, &m);
             update_cols(ROW(A, s), ROW(C, s), ROW(D, s), ld, c0, c1, m,
                         Dr + (size_t)(2 * p) * ld, Dr + (size_t)(2 * p + 1) * ld);
         }

@@ -52,6 +52,13 @@ int ftdo_pascal(double* A, const double* B, double* C, double* D,
 int ftdo_pascal_nested(double* A, const double* B, double* C, double* D,
                        int n_sys, int n_row, int P, int G, int sem, int nthreads);
 
+/* ftdo_pascal_nested over the P >= 2 partitions starts[p] .. starts[p+1] - 1
+ * (starts[0] = 0, starts[P] = n_row, each of at least 3 rows) in place of
+ * the para_range split: the uneven-batch kernel's chunks, whose boundaries
+ * fall on 128-byte lines of the batch arrays. */
+int ftdo_pascal_starts(double* A, const double* B, double* C, double* D, int n_sys, int n_row,
+                       int P, const int* starts, int G, int sem, int nthreads);
+
 /* FP32 elimination with FP64 refinement (no counterpart in the reference):
  * float A, B, C, D [n_row x n_sys] (read only), x (double) to X. A solve is
  * PaScaL over Q partitions with the modified Thomas in float arithmetic (GPU

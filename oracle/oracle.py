@@ -49,6 +49,8 @@ def lib() -> ctypes.CDLL:
         L.ftdo_pascal.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int, c_int]
         L.ftdo_pascal_nested.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, c_int,
                                          c_int]
+        L.ftdo_pascal_starts.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, ip, c_int, c_int,
+                                         c_int]
         L.ftdo_filtered_guard.argtypes = [dp, dp, dp, dp, c_int, c_int, c_int, c_int, ip,
                                           c_double, c_int, ip, c_int]
         L.ftdo_set_rho.argtypes = [dp, dp, dp, c_int, c_int, dp, dp]
@@ -67,7 +69,7 @@ def lib() -> ctypes.CDLL:
         L.ftdo_lg.argtypes = [c_double]
         L.ftdo_lg.restype = ctypes.c_float
         for f in (L.ftdo_thomas, L.ftdo_thomas_cyclic, L.ftdo_pascal, L.ftdo_pascal_nested,
-                  L.ftdo_filtered_guard, L.ftdo_pascal_ir,
+                  L.ftdo_pascal_starts, L.ftdo_filtered_guard, L.ftdo_pascal_ir,
                   L.ftdo_cal_J,
                   L.ftdo_filtered, L.ftdo_filtered_learned, L.ftdo_mlp_J):
             f.restype = c_int
@@ -114,6 +116,17 @@ def pascal_nested(A, B, C, D, P, G, sem=SEM_GPU, nthreads=0, inplace=False):
     PaScaL over G groups of rows (G = 1: Thomas, the same as pascal)."""
     (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
     _check(lib().ftdo_pascal_nested(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, P, G, sem,
+                                    nthreads))
+    return D
+
+
+def pascal_starts(A, B, C, D, starts, G, sem=SEM_GPU, nthreads=0, inplace=False):
+    """pascal_nested over the partitions starts[p] .. starts[p + 1] - 1
+    (starts from 0 to n_row) instead of the para_range split."""
+    (A, B, C, D), n_sys, n_row = _prep(A, B, C, D, inplace)
+    st = np.ascontiguousarray(starts, dtype=np.int32)
+    _check(lib().ftdo_pascal_starts(_p(A), _p(B), _p(C), _p(D), n_sys, n_row, st.size - 1,
+                                    st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), G, sem,
                                     nthreads))
     return D
 

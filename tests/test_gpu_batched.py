@@ -58,18 +58,61 @@ def red_groups(NR):
     return G
 
 
+def aligned_starts(off, n, chunk):
+    """The engine's chunk starts for a system of n > chunk rows at element
+    offset off: every chunk after the first starts on a 16-element boundary
+    of the batch arrays; the first runs to chunk - 16 rows past the first
+    boundary (chunk rows where the system starts on one), the others are
+    chunk rows, and a last chunk under 16 rows takes 16 from the one before.
+    None where the system keeps the para_range split (a remainder under 16
+    rows after the first chunk, or more than 128 chunks)."""
+    a = (16 - off % 16) % 16
+    f = a + chunk - 16 if a else chunk
+    rem = n - f
+    if rem < 16:
+        return None
+    Q = 1 + (rem + chunk - 1) // chunk
+    if Q > 128:
+        return None
+    st = [0] + [f + (q - 1) * chunk for q in range(1, Q)] + [n]
+    if Q >= 3 and n - st[Q - 1] < 16:
+        st[Q - 1] -= 16
+    return st
+
+
+def test_aligned_starts_cut_chunks_on_lines():
+    for off in range(0, 40):
+        for n in (33, 47, 48, 49, 64, 65, 100, 257, 1000, 4096):
+            for chunk in (32, 48):
+                if n <= chunk:
+                    continue
+                st = aligned_starts(off, n, chunk)
+                if st is None:
+                    assert n < chunk + 16 or n > 127 * chunk, (off, n, chunk)
+                    continue
+                lens = np.diff(st)
+                assert st[0] == 0 and st[-1] == n
+                assert lens.min() >= 16 and lens.max() <= chunk, (off, n, chunk, st)
+                assert all((off + x) % 16 == 0 for x in st[1:-1]), (off, n, chunk, st)
+
+
 def oracle_solve(off, n_rows, A, B, C, D, chunk=CHUNK):
     """x of every system by the oracle, with the engine's partitioning:
-    Q = ceil(N / chunk) chunks, the 2Q-row reduced system in red_groups(2Q)
-    groups."""
+    chunks cut on 16-element boundaries (aligned_starts) or, where that
+    split does not apply, Q = ceil(N / chunk) para_range chunks; the 2Q-row
+    reduced system in red_groups(2Q) groups."""
     X = D.copy()
     for s in range(n_rows.size):
         o, n = int(off[s]), int(n_rows[s])
         sl = slice(o, o + n)
         a, b, c, d = (v[sl].reshape(n, 1).copy() for v in (A, B, C, D))
         Q = (n + chunk - 1) // chunk
+        st = aligned_starts(o, n, chunk) if 2 <= Q <= 128 else None
         if Q == 1 or Q > 128:
             x = O.thomas(a, b, c, d, sem=O.SEM_GPU)
+        elif st is not None:
+            Qa = len(st) - 1
+            x = O.pascal_starts(a, b, c, d, st, red_groups(2 * Qa), sem=O.SEM_GPU)
         else:
             x = O.pascal_nested(a, b, c, d, Q, red_groups(2 * Q), sem=O.SEM_GPU)
         X[sl] = x[:, 0]

@@ -492,3 +492,27 @@ def test_heat_second_order_convergence():
         dt, nt = O.heat_dt("order", 0.25, n)
         e.append(O.heat_run(n, n, n, 1, "pascal", dt, nt, periodic=(1, 1, 1)))
     assert 3.9 < e[0] / e[1] < 4.1
+
+
+def test_pascal_starts_pinned_to_para_range_and_thomas():
+    """PaScaL over explicit partition starts: with the para_range starts it is
+    pascal_nested bit for bit (G = 1 and 2); with uneven starts (chunks cut at
+    line boundaries of a batch) it is the exact solve, within rounding of
+    Thomas; partitions under 3 rows are refused."""
+    rng = np.random.default_rng(7)
+    n, ns, P = 200, 6, 7
+    A = rng.uniform(-1, 1, (n, ns))
+    C = rng.uniform(-1, 1, (n, ns))
+    B = (np.abs(A) + np.abs(C)) * (1.5 + rng.uniform(0, 1, (n, ns)))
+    D = rng.uniform(-1, 1, (n, ns))
+    starts = [0] + [p * (n // P) + min(p, n % P) for p in range(1, P)] + [n]
+    for G in (1, 2):
+        np.testing.assert_array_equal(O.pascal_starts(A, B, C, D, starts, G),
+                                      O.pascal_nested(A, B, C, D, P, G))
+    t = O.thomas(A, B, C, D)
+    for st in ([0, 17, 49, 81, 113, 145, 177, 200], [0, 3, 35, 67, 99, 131, 163, 197, 200]):
+        x = O.pascal_starts(A, B, C, D, st, 1)
+        assert np.abs(x - t).max() <= 1e-13 * np.abs(t).max()
+        assert not np.array_equal(x, O.pascal_nested(A, B, C, D, len(st) - 1, 1))
+    with pytest.raises(Exception):
+        O.pascal_starts(A, B, C, D, [0, 2, 200], 1)
