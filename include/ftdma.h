@@ -237,9 +237,14 @@ ftd_status ftd_solve_complex(ftd_handle* h, double* A, double* B, double* C, dou
  * share one n_row): n_systems independent systems, system s occupying
  * elements row_offsets[s] .. row_offsets[s] + n_rows[s] - 1 of each of A, B,
  * C, D (device arrays; row_offsets and n_rows are HOST arrays). Exact solve:
- * a system of <= 48 rows is plain Thomas, a longer one PaScaL with
- * ceil(n_rows/48) partitions inside one 128-lane workgroup (48: the default
- * chunk, see ftd_set_batch_chunk); all in one
+ * a system of <= 48 rows is plain Thomas, a longer one PaScaL over
+ * partitions of at most 48 rows inside one 128-lane workgroup (48: the
+ * default chunk, see ftd_set_batch_chunk). The partitions after a system's
+ * first start on 16-element boundaries of the arrays (128-byte lines of
+ * FP64 data), the first runs to 16 rows short of a full chunk past the
+ * system's first boundary and a last one under 16 rows takes 16 from the
+ * one before; a system too short for that split, or needing more than 128
+ * partitions, takes ceil(n_rows/48) para_range partitions. All in one
  * persistent launch (systems above 6144 rows, and a batch that is one 1-row
  * system at offset 0, in a second, lane-per-system launch that uses C as
  * scratch). D is overwritten with x; A and B are read only. No element past
