@@ -400,3 +400,30 @@ def test_batched_errors():
         be.solve_batched(t, t, t, t, [], [])  # empty batch is a no-op
     finally:
         be.close()
+
+
+@pytest.mark.parametrize("chunk", [32, 48])
+def test_batched_line_aligned_split_edges(chunk):
+    """The line-aligned split at its edges: systems starting at every offset
+    modulo 16 (a gap of 1 between systems walks the residues), with lengths
+    just under, at and above the smallest the split takes (chunk + 16 rows
+    past a boundary), last chunks of 1..16 rows that borrow 16 from the
+    chunk before, and the 128-chunk limit; bit for bit against the oracle
+    with the same split, and within rounding of Thomas."""
+    base = [chunk + 1, chunk + 15, chunk + 16, chunk + 17, chunk + 31, 2 * chunk + 1,
+            2 * chunk + 16, 3 * chunk + 3, 5 * chunk + 15, 5 * chunk + 17, 127 * chunk,
+            128 * chunk - 5, 128 * chunk]
+    lengths = [n for k in range(16) for n in base[k % len(base):] + base[:k % len(base)]][:160]
+    off, n_rows, A, B, C, D = general_uneven(lengths, seed=60 + chunk, gap=1)
+    assert len({int(o) % 16 for o in off}) == 16
+    note(f"batched line-aligned split edges, chunk {chunk}, {n_rows.size} systems")
+    be = TdmaBackendGPU("pascal", 1, 1, 1)
+    try:
+        be.set_batch_chunk(chunk)
+        _, _, X, path = run_batched(be, off, n_rows, A, B, C, D)
+    finally:
+        be.close()
+    assert f"chunk={chunk}" in path, path
+    np.testing.assert_array_equal(X, oracle_solve(off, n_rows, A, B, C, D, chunk=chunk))
+    T = thomas_all(off, n_rows, A, B, C, D)
+    assert np.abs(X - T).max() <= 1e-12 * max(1.0, np.abs(T).max())
